@@ -1,0 +1,232 @@
+"""Host-side mirror of EPIC's call surface over the C-ABI of libepic_gpu.so.
+
+The method names follow the reference procedures they replace:
+``par2grid(l_reuse)`` (parcel_interpl.f90:146), ``grid2par(add)`` (:405),
+``vor2vel()`` (inversion.f90:25), ``vorticity_tendency()`` (:321) and
+``ls_rk_step(t)`` (ls_rk.f90:88).  Arrays are numpy float64 in the same
+layouts as the oracle's State: parcels (n, 3)/(n, 5)/(n,), fields
+(ncomp, nx+3, ny+3, nz+3) indexed [c, ix+1, iy+1, iz+1].
+
+The compute path is the HIP library only: importing this module fails loudly
+when libepic_gpu.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libepic_gpu.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "epic_gpu.h")
+
+FIELDS = {"volg": (0, 1), "vortg": (1, 3), "tbuoyg": (2, 1), "dbuoyg": (3, 1), "humg": (4, 1),
+          "velog": (5, 3), "velgradg": (6, 8), "vtend": (7, 3), "nparg": (8, 1), "nsparg": (9, 1)}
+
+ERRORS = {-1: "EPIC_E_INVALID", -2: "EPIC_E_HIP", -3: "EPIC_E_PARCEL_COUNT", -4: "EPIC_E_B33",
+          -5: "EPIC_E_OUT_OF_GRID", -6: "EPIC_E_RK_ORDER", -7: "EPIC_E_DT", -8: "EPIC_E_NOMEM",
+          -9: "EPIC_E_CAPACITY"}
+
+
+class EpicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("lower", C.c_double * 3),
+                ("extent", C.c_double * 3)]
+
+
+class Phys(C.Structure):
+    _fields_ = [("glat", C.c_double), ("lambda_c", C.c_double), ("q0", C.c_double), ("f_cor", C.c_double * 3)]
+
+
+class Flags(C.Structure):
+    _fields_ = [("zeta_bndry_zero", C.c_int * 2), ("rk_order", C.c_int), ("vmin", C.c_double)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libepic_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m epic_amd.build`")
+    L = C.CDLL(path)
+    P = C.POINTER
+    d, i, i64 = C.c_double, C.c_int, C.c_int64
+    dp, ip = P(d), P(i)
+    h = C.c_void_p
+    sig = {
+        "epic_create": ([P(Grid), P(Phys), P(Flags), i64, P(h)], i),
+        "epic_destroy": ([h], i),
+        "epic_last_error": ([h], C.c_char_p),
+        "epic_set_parcels": ([h, i64, dp, dp, dp, dp, dp, dp], i),
+        "epic_get_parcels": ([h, i64, dp, dp, dp, dp, dp, dp], i),
+        "epic_get_parcel_tendencies": ([h, i64, dp, dp, dp], i),
+        "epic_set_parcel_tendencies": ([h, i64, dp, dp], i),
+        "epic_get_field": ([h, i, dp], i),
+        "epic_set_field": ([h, i, dp], i),
+        "epic_get_field_int": ([h, i, ip], i),
+        "epic_par2grid": ([h, i], i),
+        "epic_vor2vel": ([h], i),
+        "epic_vorticity_tendency": ([h], i),
+        "epic_grid2par": ([h, i], i),
+        "epic_ls_rk_step": ([h, dp, d, d], i),
+        "epic_cell_sort": ([h, ip, ip, ip, ip], i),
+        "epic_nearest_ncell": ([h], i64),
+        "epic_num_parcels": ([h], i64),
+        "epic_set_profiling": ([h, i], i),
+        "epic_get_timers": ([h, dp, ip, i], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def header_symbols(path=HEADER):
+    """Names of the functions declared in include/epic_gpu.h."""
+    text = open(path).read()
+    return sorted(set(re.findall(r"\b(epic_[a-z0-9_]+)\s*\(", text)))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _opt(flag):
+    return -1 if flag is None else int(bool(flag))
+
+
+class Member:
+    """One EPIC domain resident on the GPU."""
+
+    def __init__(self, nx, ny, nz, lower, extent, glat=0.0, lambda_c=0.0, q0=0.0, f_cor=(0.0, 0.0, 0.0),
+                 zeta_zero=(False, False), rk_order=4, vmin=0.0, max_parcels=0):
+        L = load_library()
+        g = Grid(nx, ny, nz, (C.c_double * 3)(*lower), (C.c_double * 3)(*extent))
+        p = Phys(glat, lambda_c, q0, (C.c_double * 3)(*f_cor))
+        f = Flags((C.c_int * 2)(*[int(bool(z)) for z in zeta_zero]), rk_order, vmin)
+        self._h = C.c_void_p()
+        rc = L.epic_create(C.byref(g), C.byref(p), C.byref(f), int(max_parcels), C.byref(self._h))
+        if rc != 0:
+            raise EpicError(rc, "epic_create failed")
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = 0
+
+    @classmethod
+    def from_workload(cls, mem, rk_order=4, max_parcels=None):
+        m = cls(mem.nx, mem.ny, mem.nz, mem.lower, mem.extent, mem.glat, mem.lambda_c, mem.q0, mem.f_cor,
+                mem.zeta_zero, rk_order=rk_order, max_parcels=max_parcels or mem.n)
+        m.set_parcels(mem.position, mem.vorticity, mem.B, mem.volume, mem.buoyancy, mem.humidity)
+        return m
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = load_library().epic_last_error(self._h)
+            raise EpicError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load_library().epic_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ----
+    def set_parcels(self, position, vorticity, B, volume, buoyancy, humidity):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (position, vorticity, B, volume, buoyancy, humidity)]
+        n = arrs[0].shape[0]
+        self._chk(load_library().epic_set_parcels(self._h, n, *[_dp(a) for a in arrs]))
+        self.n = n
+
+    def get_parcels(self):
+        n = self.n
+        out = {"position": np.zeros((n, 3)), "vorticity": np.zeros((n, 3)), "B": np.zeros((n, 5)),
+               "volume": np.zeros(n), "buoyancy": np.zeros(n), "humidity": np.zeros(n)}
+        self._chk(load_library().epic_get_parcels(self._h, n, *[_dp(out[k]) for k in
+                                                               ("position", "vorticity", "B", "volume", "buoyancy",
+                                                                "humidity")]))
+        return out
+
+    def get_tendencies(self):
+        n = self.n
+        dp, dv, st = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 8))
+        self._chk(load_library().epic_get_parcel_tendencies(self._h, n, _dp(dp), _dp(dv), _dp(st)))
+        return dp, dv, st
+
+    def set_tendencies(self, delta_pos, delta_vor):
+        a = np.ascontiguousarray(delta_pos, dtype=np.float64)
+        b = np.ascontiguousarray(delta_vor, dtype=np.float64)
+        self._chk(load_library().epic_set_parcel_tendencies(self._h, self.n, _dp(a), _dp(b)))
+
+    def get_field(self, name):
+        which, nc = FIELDS[name]
+        shape = (nc, self.nx + 3, self.ny + 3, self.nz + 3)
+        if name in ("nparg", "nsparg"):
+            out = np.zeros(shape[1:], dtype=np.int32)
+            self._chk(load_library().epic_get_field_int(self._h, which, out.ctypes.data_as(C.POINTER(C.c_int))))
+            return out
+        out = np.zeros(shape)
+        self._chk(load_library().epic_get_field(self._h, which, _dp(out)))
+        return out[0] if nc == 1 else out
+
+    def set_field(self, name, values):
+        which, nc = FIELDS[name]
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(nc, self.nx + 3, self.ny + 3, self.nz + 3)
+        self._chk(load_library().epic_set_field(self._h, which, _dp(a)))
+
+    # ---- reference call surface ----
+    def par2grid(self, l_reuse=None):
+        self._chk(load_library().epic_par2grid(self._h, _opt(l_reuse)))
+
+    def vor2vel(self):
+        self._chk(load_library().epic_vor2vel(self._h))
+
+    def vorticity_tendency(self):
+        self._chk(load_library().epic_vorticity_tendency(self._h))
+
+    def grid2par(self, add=None):
+        self._chk(load_library().epic_grid2par(self._h, _opt(add)))
+
+    def ls_rk_step(self, t, dt, t_initial=0.0):
+        tt = C.c_double(t)
+        self._chk(load_library().epic_ls_rk_step(self._h, C.byref(tt), dt, t_initial))
+        return tt.value
+
+    TIMERS = ("par2grid", "cell_sort", "vor2vel", "vorticity_tendency", "push", "grid2par")
+
+    def set_profiling(self, on=True):
+        self._chk(load_library().epic_set_profiling(self._h, int(bool(on))))
+
+    def timers(self, reset=True):
+        ms = np.zeros(len(self.TIMERS))
+        calls = np.zeros(len(self.TIMERS), dtype=np.int32)
+        self._chk(load_library().epic_get_timers(self._h, _dp(ms), calls.ctypes.data_as(C.POINTER(C.c_int)),
+                                                 int(bool(reset))))
+        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(self.TIMERS)}
+
+    def cell_sort(self):
+        L = load_library()
+        ncell = L.epic_nearest_ncell(self._h)
+        loca = np.zeros(self.n, dtype=np.int32)
+        node = np.zeros(self.n, dtype=np.int32)
+        nppc = np.zeros(ncell, dtype=np.int32)
+        kc1 = np.zeros(ncell, dtype=np.int32)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))   # noqa: E731
+        self._chk(L.epic_cell_sort(self._h, ip(loca), ip(nppc), ip(kc1), ip(node)))
+        return loca, nppc, kc1, node

@@ -1,0 +1,92 @@
+// Shared definitions of the MI355X parcel-grid core (gfx950, HIP).
+//
+// Device data layouts (one member = one EPIC domain):
+//   parcels  structure of arrays in "physical" order; positions FP64,
+//            every other attribute FP32 (see ParcelsDev)
+//   fields   FP32 node fields f[k+1][j][i], k in [-1, nz+1], j in [0, ny),
+//            i in [0, nx) (x fastest, x/y periodic so no stored x/y halo)
+//   spectra  complex FP32 s[ky][kx][z], z in [0, nzp) padded to a multiple
+//            of ZB planes, kx in [0, nx/2], ky in [0, ny)
+//   cells    parcel-centre cells c = ((k+1) ny + j) nx + i, k in [-1, nz]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace epic {
+
+constexpr int ZB = 16;           // z planes per FFT work-group
+constexpr int P2G_TX = 8;        // p2g tile cells in x
+constexpr int P2G_TY = 8;        // p2g tile cells in y
+constexpr int P2G_BOX = 11;      // node box per tile side: 1 lower + 8 + 2 upper
+constexpr int P2G_THREADS = 320; // 300 extended cells (10 x 10 x 3) + spare
+constexpr int P2G_MAXP = 640;    // parcels per LDS batch
+constexpr int P2G_NF = 7;        // vol, xi, eta, zeta, tbuoy, dbuoy, hum
+
+// error bits reported through the member's device error word
+enum : int {
+    ERR_PARCEL_COUNT = 1,   // parcel centre outside layers -1..nz
+    ERR_B33 = 2,            // |B11 B22 - B12^2| <= eps
+    ERR_OUT_OF_GRID = 4,    // support point outside the node grid / neighbour cells
+    ERR_CELL_MISMATCH = 8,  // a sorted parcel is not in its sorted cell
+};
+
+// field slots inside Member::fld (FP32 node fields)
+enum FieldSlot : int {
+    F_VOLG = 0,
+    F_VORTG = 1,    // 3 components
+    F_TBUOYG = 4,
+    F_DBUOYG = 5,
+    F_HUMG = 6,
+    F_VELOG = 7,    // 3
+    F_VELGRADG = 10, // 8: dudx dudy dudz dvdx dvdy dvdz dwdx dwdy
+    F_VTEND = 18,   // 3
+    F_COUNT = 21
+};
+
+enum { DUDX = 0, DUDY, DUDZ, DVDX, DVDY, DVDZ, DWDX, DWDY };
+
+struct Grid {
+    int nx, ny, nz;
+    int nxh;          // nx/2 + 1
+    int nzp;          // nz+1 rounded up to ZB
+    int zeta_zero[2];
+    double lower[3], extent[3], upper[3], dx[3], dxi[3], center[3], hli[3];
+    double vcell, vmin;
+    double glat, lambda_c, q0, f_cor[3];
+};
+
+// parcel attribute arrays (n entries each)
+struct ParcelsDev {
+    int64_t n;
+    double *x, *y, *z;
+    float *vor[3];
+    float *B[5];
+    float *vol, *buoy, *hum;
+    float *dpos[3], *dvor[3], *dB[5];
+    float *veta[3], *vtau[3];   // support-point half axes cached for l_reuse
+    float *strain[8];           // only written by the stand-alone grid2par
+    int *orig;                  // caller index of each physical slot
+};
+
+struct FftPlan {
+    int n;          // complex length
+    int nst;        // number of stages
+    int radix[16];
+};
+
+__host__ __device__ inline int64_t field_stride(const Grid &g) { return (int64_t)(g.nz + 3) * g.ny * g.nx; }
+__host__ __device__ inline int64_t plane_stride(const Grid &g) { return (int64_t)g.ny * g.nx; }
+__host__ __device__ inline int64_t spec_size(const Grid &g) { return (int64_t)g.ny * g.nxh * g.nzp; }
+__host__ __device__ inline int64_t ncells(const Grid &g) { return (int64_t)g.nx * g.ny * (g.nz + 2); }
+
+__device__ inline int wrap(int i, int n)
+{
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// Fortran int(): truncation toward zero
+__device__ inline double ftrunc(double v) { return trunc(v); }
+
+}  // namespace epic

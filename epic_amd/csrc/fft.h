@@ -1,0 +1,96 @@
+// Batched complex FFT over LDS (Stockham autosort, radix 4/2/3) shared by
+// the horizontal FP32 transforms and the FP64 vertical transforms.
+#pragma once
+
+#include "common.h"
+
+namespace epic {
+
+template <typename T2> struct Cx;
+template <> struct Cx<float2> { using R = float; };
+template <> struct Cx<double2> { using R = double; };
+
+template <typename T2> __device__ inline T2 cadd(T2 a, T2 b) { return T2{a.x + b.x, a.y + b.y}; }
+template <typename T2> __device__ inline T2 csub(T2 a, T2 b) { return T2{a.x - b.x, a.y - b.y}; }
+template <typename T2> __device__ inline T2 cmul(T2 a, T2 b) { return T2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename T2> __device__ inline T2 cconj(T2 a) { return T2{a.x, -a.y}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T2> __device__ inline T2 cmuli(T2 a, bool inv) { return inv ? T2{-a.y, a.x} : T2{a.y, -a.x}; }
+template <typename T2, typename R> __device__ inline T2 cscale(T2 a, R s) { return T2{a.x * s, a.y * s}; }
+
+// In-place R-point DFT, R in {2, 3, 4}; forward uses e^{-2 pi i/R}.
+template <typename T2> __device__ inline void butterfly(int R, T2 *v, bool inv)
+{
+    using Rl = typename Cx<T2>::R;
+    if (R == 2) {
+        const T2 a = cadd(v[0], v[1]), b = csub(v[0], v[1]);
+        v[0] = a;
+        v[1] = b;
+    } else if (R == 4) {
+        const T2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const T2 t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), inv);
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    } else {   // R == 3
+        const Rl h = (Rl)0.5, s3 = (Rl)0.86602540378443864676;
+        const T2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const T2 m = T2{v[0].x - h * s.x, v[0].y - h * s.y};
+        const T2 rd = cmuli(cscale(d, s3), inv);   // -i s3 d (forward)
+        v[0] = cadd(v[0], s);
+        v[1] = cadd(m, rd);
+        v[2] = csub(m, rd);
+    }
+}
+
+// L lines of n points each, line l at a[l*n]; every thread of the block
+// calls this.  tw[t*tws] = e^{-2 pi i t / n}.  Returns the buffer holding
+// the result (a or b).  Ends with a barrier.
+template <typename T2>
+__device__ T2 *lds_fft(T2 *a, T2 *b, int L, const FftPlan &pl, const T2 *__restrict__ tw, int tws, bool inv)
+{
+    const int n = pl.n;
+    int Ns = 1;
+    T2 *in = a, *out = b;
+    for (int st = 0; st < pl.nst; ++st) {
+        const int R = pl.radix[st];
+        const int nb = n / R;
+        const int tstep = n / (Ns * R);
+        for (int t = threadIdx.x; t < L * nb; t += blockDim.x) {
+            const int l = t / nb, jj = t - l * nb;
+            const T2 *src = in + l * n;
+            T2 v[4];
+            for (int r = 0; r < R; ++r) v[r] = src[jj + r * nb];
+            const int kk = jj % Ns;
+            for (int r = 1; r < R; ++r) {
+                T2 w = tw[(r * kk * tstep) * tws];
+                if (inv) w = cconj(w);
+                v[r] = cmul(v[r], w);
+            }
+            butterfly(R, v, inv);
+            T2 *dst = out + l * n + (jj / Ns) * Ns * R + kk;
+            for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
+        }
+        __syncthreads();
+        T2 *tmp = in;
+        in = out;
+        out = tmp;
+        Ns *= R;
+    }
+    return in;
+}
+
+inline FftPlan make_plan(int n)
+{
+    FftPlan p{};
+    p.n = n;
+    int m = n;
+    while (m % 4 == 0) { p.radix[p.nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { p.radix[p.nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { p.radix[p.nst++] = 3; m /= 3; }
+    if (m != 1) p.nst = -1;   // unsupported length
+    return p;
+}
+
+}  // namespace epic

@@ -1,0 +1,183 @@
+// Horizontal 2D real FFTs of the node planes z = 0..nz (FP32), the transform
+// pair of fftxyp2s / fftxys2p (sta3dfft.f90:134-257) in complex half-spectrum
+// form: S(kx, ky) = sum_{i,j} f(i, j) e^{-2 pi i (kx i/nx + ky j/ny)},
+// kx in [0, nx/2].  The inverse carries the 1/(nx ny) normalisation.
+//
+// Two passes per direction; the intermediate T[j][kx][z] and the spectra
+// S[ky][kx][z] keep ZB consecutive z planes contiguous, so every pass moves
+// 128-byte segments and the vertical (z) column kernels read whole columns.
+//
+// Loaders fuse the physical products of the vorticity tendency into the
+// forward pass; the inverse y pass can form i kx F1 + i ky F2 on the fly and
+// the inverse x pass can add the second-order z derivative of a product.
+#include "common.h"
+#include "fft.h"
+#include "launch.h"
+
+namespace epic {
+
+// ---------------------------------------------------------------------------
+// forward x pass: real rows -> half spectra
+// ---------------------------------------------------------------------------
+__device__ inline float load_phys(const FftXArgs &a, int k, int j, int i)
+{
+    const Grid &g = a.g;
+    const int64_t FS = field_stride(g), PS = plane_stride(g);
+    const int64_t o = (int64_t)(k + 1) * PS + (int64_t)j * g.nx + i;
+    if (a.mode == 0) return a.fld[a.slot * FS + o];
+    // vorticity-tendency flux: (omega_a + f_a) * vel_b + sgn * tbuoy
+    float v = (a.fld[(F_VORTG + a.comp_a) * FS + o] + (float)g.f_cor[a.comp_a]) * a.fld[(F_VELOG + a.comp_b) * FS + o];
+    if (a.sgn != 0) v += (float)a.sgn * a.fld[F_TBUOYG * FS + o];
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_fft_xf(FftXArgs a)
+{
+    const Grid &g = a.g;
+    const int j = blockIdx.x;
+    const int zp0 = blockIdx.y * ZB;
+    const int m = g.nx / 2;
+    extern __shared__ float2 lds[];
+    float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
+    for (int t = threadIdx.x; t < ZB * m; t += blockDim.x) {
+        const int r = t / m, q = t - r * m;
+        const int k = zp0 + r;
+        float2 z = {0.0f, 0.0f};
+        if (k <= g.nz) z = float2{load_phys(a, k, j, 2 * q), load_phys(a, k, j, 2 * q + 1)};
+        bufa[r * m + q] = z;
+    }
+    __syncthreads();
+    float2 *Z = lds_fft(bufa, bufb, ZB, a.plan, a.tw, 2, false);
+    float2 *other = (Z == bufa) ? bufb : bufa;
+    // untangle the packed real transform: X_k, k = 0..m
+    for (int t = threadIdx.x; t < ZB * (m + 1); t += blockDim.x) {
+        const int kx = t / ZB, r = t - kx * ZB;
+        const float2 zk = Z[r * m + (kx % m)];
+        const float2 zc = cconj(Z[r * m + ((m - kx) % m)]);
+        const float2 w = a.tw[kx];   // e^{-2 pi i kx / nx}
+        const float2 s = cadd(zk, zc), d = csub(zk, zc);
+        const float2 wd = cmul(w, d);
+        // X = (s - i w d) / 2
+        other[r * (m + 1) + kx] = float2{0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x)};
+    }
+    __syncthreads();
+    const int nxh = m + 1;
+    for (int t = threadIdx.x; t < ZB * nxh; t += blockDim.x) {
+        const int kx = t / ZB, r = t - kx * ZB;
+        a.out[((int64_t)j * nxh + kx) * g.nzp + zp0 + r] = other[r * (m + 1) + kx];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y pass (c2c), forward or inverse
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fft_y(FftYArgs a)
+{
+    const Grid &g = a.g;
+    const int kx = blockIdx.x;
+    const int zp0 = blockIdx.y * ZB;
+    const int ny = g.ny, nxh = g.nxh;
+    extern __shared__ float2 lds[];
+    float2 *bufa = lds, *bufb = lds + ZB * ny;
+    // inverse combine factors: i kx (0 at the x Nyquist) and i ky (0 at the y Nyquist)
+    const float kxw = (2 * kx == g.nx) ? 0.0f : (float)(2.0 * 3.14159265358979323846 * kx / g.extent[0]);
+    for (int t = threadIdx.x; t < ZB * ny; t += blockDim.x) {
+        const int jy = t / ZB, r = t - jy * ZB;
+        const int64_t idx = ((int64_t)jy * nxh + kx) * g.nzp + zp0 + r;
+        float2 v;
+        if (a.mode == 2) {
+            const int kys = (jy <= ny / 2) ? jy : jy - ny;
+            const float kyw = (2 * jy == ny) ? 0.0f : (float)(2.0 * 3.14159265358979323846 * kys / g.extent[1]);
+            const float2 f1 = a.in[idx], f2 = a.in2[idx];
+            // i kx f1 + i ky f2
+            v = float2{-(kxw * f1.y + kyw * f2.y), kxw * f1.x + kyw * f2.x};
+        } else {
+            v = a.in[idx];
+        }
+        bufa[r * ny + jy] = v;
+    }
+    __syncthreads();
+    float2 *R = lds_fft(bufa, bufb, ZB, a.plan, a.tw, 1, a.mode != 0);
+    for (int t = threadIdx.x; t < ZB * ny; t += blockDim.x) {
+        const int jy = t / ZB, r = t - jy * ZB;
+        a.out[((int64_t)jy * nxh + kx) * g.nzp + zp0 + r] = R[r * ny + jy];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inverse x pass: half spectra -> real rows, with the output stage
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fft_xi(FftXArgs a)
+{
+    const Grid &g = a.g;
+    const int j = blockIdx.x;
+    const int zp0 = blockIdx.y * ZB;
+    const int m = g.nx / 2, nxh = m + 1;
+    extern __shared__ float2 lds[];
+    float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
+    for (int t = threadIdx.x; t < ZB * nxh; t += blockDim.x) {
+        const int kx = t / ZB, r = t - kx * ZB;
+        bufb[r * nxh + kx] = a.in[((int64_t)j * nxh + kx) * g.nzp + zp0 + r];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ZB * m; t += blockDim.x) {
+        const int r = t / m, k = t - r * m;
+        const float2 xk = bufb[r * nxh + k];
+        const float2 xc = cconj(bufb[r * nxh + (m - k)]);
+        const float2 w = cconj(a.tw[k]);   // e^{+2 pi i k / nx}
+        const float2 s = cadd(xk, xc), d = csub(xk, xc);
+        const float2 wd = cmul(w, d);
+        bufa[r * m + k] = float2{s.x - wd.y, s.y + wd.x};   // s + i w d
+    }
+    __syncthreads();
+    float2 *Z = lds_fft(bufa, bufb, ZB, a.plan, a.tw, 2, true);
+    const float scale = 1.0f / ((float)g.nx * (float)g.ny);
+    const int64_t FS = field_stride(g), PS = plane_stride(g);
+    for (int t = threadIdx.x; t < ZB * g.nx; t += blockDim.x) {
+        const int r = t / g.nx, i = t - r * g.nx;
+        const int k = zp0 + r;
+        if (k > g.nz) continue;
+        const float2 z = Z[r * m + (i >> 1)];
+        float v = ((i & 1) ? z.y : z.x) * scale;
+        const int64_t o = (int64_t)(k + 1) * PS + (int64_t)j * g.nx + i;
+        if (a.mode == 2) {
+            // + d/dz of f3 = (zeta + f_z) vel_c, second order (central_diffz,
+            // inversion_utils.f90:488-514)
+            const float fz = (float)g.f_cor[2];
+            auto f3 = [&](int kk) {
+                const int64_t oo = (int64_t)(kk + 1) * PS + (int64_t)j * g.nx + i;
+                return (a.fld[(F_VORTG + 2) * FS + oo] + fz) * a.fld[(F_VELOG + a.comp_b) * FS + oo];
+            };
+            const float dzi = (float)g.dxi[2];
+            float dd;
+            if (k == 0) dd = dzi * (f3(1) - f3(0));
+            else if (k == g.nz) dd = dzi * (f3(g.nz) - f3(g.nz - 1));
+            else dd = (f3(k + 1) - f3(k - 1)) * (0.5f * dzi);
+            v = v + dd;
+        }
+        a.fld_out[a.slot * FS + o] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static size_t x_lds(const Grid &g) { return 2 * ZB * (size_t)(g.nx / 2 + 1) * sizeof(float2); }
+static size_t y_lds(const Grid &g) { return 2 * ZB * (size_t)g.ny * sizeof(float2); }
+
+void launch_fft_xf(const FftXArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fft_xf, dim3(a.g.ny, a.g.nzp / ZB), dim3(256), x_lds(a.g), s, a);
+}
+
+void launch_fft_y(const FftYArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fft_y, dim3(a.g.nxh, a.g.nzp / ZB), dim3(256), y_lds(a.g), s, a);
+}
+
+void launch_fft_xi(const FftXArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fft_xi, dim3(a.g.ny, a.g.nzp / ZB), dim3(256), x_lds(a.g), s, a);
+}
+
+}  // namespace epic

@@ -1,0 +1,157 @@
+/*
+ * epic_gpu.h -- C-ABI of the MI355X parcel-grid core (libepic_gpu.so).
+ *
+ * Drop-in for EPIC's per-timestep core.  The reference's entry points are
+ * Fortran module procedures without array arguments that read and write
+ * module globals (parcels, field arrays, inversion tables); here one handle
+ * owns that state on the GPU.  A Fortran caller binds these functions with
+ * ISO_C_BINDING (see epic_amd/fortran/epic_gpu_mod.f90).
+ *
+ * Conventions
+ *  - Return value: 0 on success, a negative EPIC_E_* code otherwise; the
+ *    message of the last failure is returned by epic_last_error().  The
+ *    reference aborts on these conditions (mpi_utils.f90:15-33); the caller
+ *    decides here.
+ *  - Host arrays are caller-owned FP64 arrays in the reference's Fortran
+ *    layouts, copied on every set/get call; no pointer is kept after return:
+ *      parcels  position(3,n), vorticity(3,n), B(5,n), volume(n),
+ *               buoyancy(n), humidity(n)      (parcel_container.f90:29-57)
+ *      fields   f(-1:nz+1, -1:ny+1, -1:nx+1[, ncomp])   (fields.f90:66-104)
+ *  - One host thread per handle.  Work is queued on the handle's HIP stream;
+ *    every entry point below returns after its work has completed.
+ *  - Device state (parcels, fields, spectra) stays resident between calls.
+ */
+#ifndef EPIC_GPU_H
+#define EPIC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct epic_member *epic_handle;
+
+/* domain: parameters.f90:93-165 (nx, ny, nz, lower, extent) */
+typedef struct {
+    int nx, ny, nz;
+    double lower[3];
+    double extent[3];
+} epic_grid_t;
+
+/* physical constants used on the path: physics.f90:170-185 */
+typedef struct {
+    double glat;        /* g L_v / (c_p theta_0) */
+    double lambda_c;    /* 1 / height_c */
+    double q0;          /* saturation humidity at ground level */
+    double f_cor[3];    /* planetary vorticity */
+} epic_phys_t;
+
+/* run options: l_bndry_zeta_zero (parameters.f90:168-217), rk_order
+ * (options.f90:32), vmin = vcell / parcel%min_vratio (parameters.f90:148) */
+typedef struct {
+    int zeta_bndry_zero[2];
+    int rk_order;       /* 3 or 4 */
+    double vmin;        /* <= 0 selects vcell / 20 */
+} epic_flags_t;
+
+enum {
+    EPIC_OK = 0,
+    EPIC_E_INVALID = -1,       /* bad argument / unsupported grid */
+    EPIC_E_HIP = -2,           /* HIP runtime failure */
+    EPIC_E_PARCEL_COUNT = -3,  /* "par2grid: Wrong total number of parcels!" (parcel_interpl.f90:248-250) */
+    EPIC_E_B33 = -4,           /* "get_B33: Division by small number!" (parcel_ellipsoid.f90:391-394) */
+    EPIC_E_OUT_OF_GRID = -5,   /* a support point left the node grid / its neighbour cells */
+    EPIC_E_RK_ORDER = -6,      /* "Only third and fourth order RK supported." (ls_rk.f90:76-78) */
+    EPIC_E_DT = -7,            /* time step must be positive (rk_utils.f90:179-182) */
+    EPIC_E_NOMEM = -8,
+    EPIC_E_CAPACITY = -9       /* more parcels than max_parcels */
+};
+
+/* field selectors for epic_get_field / epic_set_field (fields.f90:21-51) */
+enum {
+    EPIC_FIELD_VOLG = 0,      /* 1 comp */
+    EPIC_FIELD_VORTG = 1,     /* 3 */
+    EPIC_FIELD_TBUOYG = 2,    /* 1 */
+    EPIC_FIELD_DBUOYG = 3,    /* 1 */
+    EPIC_FIELD_HUMG = 4,      /* 1 */
+    EPIC_FIELD_VELOG = 5,     /* 3 */
+    EPIC_FIELD_VELGRADG = 6,  /* 8: du/dx du/dy du/dz dv/dx dv/dy dv/dz dw/dx dw/dy */
+    EPIC_FIELD_VTEND = 7,     /* 3 */
+    EPIC_FIELD_NPARG = 8,     /* integer, 1 */
+    EPIC_FIELD_NSPARG = 9     /* integer, 1 */
+};
+
+/* Allocates a member: field_alloc (fields.f90:66-104), parcels%allocate
+ * (parcel_ellipsoid.f90:84-134), init_inversion (inversion_utils.f90:79-275),
+ * ls_rk_setup (ls_rk.f90:64-80).  nx, ny: multiples of 8, lengths 2^a 3^b;
+ * nz: 2^a 3^b >= 4. */
+int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_flags_t *flags,
+                int64_t max_parcels, epic_handle *out);
+/* finalise_inversion + field_dealloc + parcels%deallocate */
+int epic_destroy(epic_handle h);
+const char *epic_last_error(epic_handle h);
+
+/* parcel state in / out (FP64 host arrays, Fortran layout).  Positions are
+ * kept in FP64 on the device, the other attributes in FP32. */
+int epic_set_parcels(epic_handle h, int64_t n, const double *position, const double *vorticity, const double *B,
+                     const double *volume, const double *buoyancy, const double *humidity);
+int epic_get_parcels(epic_handle h, int64_t n, double *position, double *vorticity, double *B, double *volume,
+                     double *buoyancy, double *humidity);
+/* RK work arrays delta_pos(3,n), delta_vor(3,n), strain(8,n) (any may be NULL) */
+int epic_get_parcel_tendencies(epic_handle h, int64_t n, double *delta_pos, double *delta_vor, double *strain);
+int epic_set_parcel_tendencies(epic_handle h, int64_t n, const double *delta_pos, const double *delta_vor);
+
+/* gridded fields (FP64 host arrays, Fortran halo layout, components last) */
+int epic_get_field(epic_handle h, int which, double *out);
+int epic_set_field(epic_handle h, int which, const double *in);
+int epic_get_field_int(epic_handle h, int which, int *out);
+
+/* ---- the reference call surface ---- */
+/* par2grid(l_reuse) (parcel_interpl.f90:146-323); l_reuse: -1 absent, 0/1 */
+int epic_par2grid(epic_handle h, int l_reuse);
+/* vor2vel + vel2vgrad (inversion.f90:25-317) */
+int epic_vor2vel(epic_handle h);
+/* vorticity_tendency (inversion.f90:321-366) */
+int epic_vorticity_tendency(epic_handle h);
+/* grid2par(add) (parcel_interpl.f90:405-470); add: -1 absent, 0/1 */
+int epic_grid2par(epic_handle h, int add);
+/* ls_rk_step(t) with a fixed dt (ls_rk.f90:88-136; get_time_step's fixed-dt
+ * branch rk_utils.f90:83-88); t is advanced by dt.  t_initial is
+ * time%initial: the first par2grid reuses the cached support points when
+ * t > t_initial. */
+int epic_ls_rk_step(epic_handle h, double *t, double dt, double t_initial);
+
+/* Cell counting sort of the parcels' current positions with the reference's
+ * key ijk = 1 + ix + hx iy + hx hy iz on the halo box
+ * (parcel_nearest.f90:196-257, 411-424, 531-541): loca(n) 1-based cell of
+ * each parcel, nppc/kc1 (ncell = (nx+3)(ny+3)(nz+2)) counts and 1-based
+ * first slots, node(n) parcel indices (0-based) in sorted order. */
+int epic_cell_sort(epic_handle h, int *loca, int *nppc, int *kc1, int *node);
+int64_t epic_nearest_ncell(epic_handle h);
+
+int64_t epic_num_parcels(epic_handle h);
+
+/* Per-phase GPU timers, the device counterpart of the reference's named
+ * wall-clock timers (mpi_timer.f90:53-104, names registered in
+ * epic3d.f90:79-95).  With profiling on, every phase is bracketed by HIP
+ * events on the handle's stream; epic_get_timers returns the accumulated
+ * milliseconds and call counts per phase (ms/calls may be NULL) and resets
+ * them when reset != 0.  "push" is the fused grid2par + dB/dt + RK push
+ * kernel of ls_rk_step; "grid2par" is the stand-alone grid2par. */
+enum {
+    EPIC_TIMER_PAR2GRID = 0,    /* cell sort + tile scatter + finalise */
+    EPIC_TIMER_CELL_SORT = 1,   /* keys, scan, placement, per-cell order (inside par2grid) */
+    EPIC_TIMER_VOR2VEL = 2,
+    EPIC_TIMER_VTEND = 3,
+    EPIC_TIMER_PUSH = 4,
+    EPIC_TIMER_GRID2PAR = 5,
+    EPIC_NUM_TIMERS = 6
+};
+int epic_set_profiling(epic_handle h, int on);
+int epic_get_timers(epic_handle h, double *ms, int *calls, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EPIC_GPU_H */

@@ -1,0 +1,38 @@
+"""The C-ABI library loads and exports every entry point declared in
+include/epic_gpu.h (no compute calls: runs without a GPU)."""
+import ctypes as C
+import os
+
+import pytest
+
+from epic_amd import core
+
+
+def test_header_declares_reference_surface():
+    names = core.header_symbols()
+    for required in ("epic_par2grid", "epic_grid2par", "epic_vor2vel", "epic_vorticity_tendency",
+                     "epic_ls_rk_step", "epic_create", "epic_destroy", "epic_cell_sort"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(core.LIB_PATH):
+        pytest.skip("libepic_gpu.so not built")
+    lib = C.CDLL(core.LIB_PATH)
+    missing = [n for n in core.header_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_create_rejects_bad_grid_without_touching_the_gpu():
+    """Argument validation happens before any HIP call."""
+    if not os.path.exists(core.LIB_PATH):
+        pytest.skip("libepic_gpu.so not built")
+    lib = core.load_library()
+    g = core.Grid(12, 16, 16, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))   # nx not a multiple of 8
+    p = core.Phys(0.0, 0.0, 0.0, (C.c_double * 3)(0, 0, 0))
+    f = core.Flags((C.c_int * 2)(0, 0), 4, 0.0)
+    h = C.c_void_p()
+    assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 10, C.byref(h)) == -1
+    g = core.Grid(16, 16, 16, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
+    f = core.Flags((C.c_int * 2)(0, 0), 2, 0.0)   # unsupported RK order
+    assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 10, C.byref(h)) == -6
