@@ -208,7 +208,7 @@ class Member:
         self._chk(load_library().epic_ls_rk_step(self._h, C.byref(tt), dt, t_initial))
         return tt.value
 
-    TIMERS = ("par2grid", "cell_sort", "vor2vel", "vorticity_tendency", "push", "grid2par")
+    TIMERS = ("par2grid", "cell_sort", "vor2vel", "vorticity_tendency", "push", "grid2par", "axes")
 
     def set_profiling(self, on=True):
         self._chk(load_library().epic_set_profiling(self._h, int(bool(on))))

@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/epic_gpu.h"
+#include "epic_gpu.h"
 #include "common.h"
 #include "fft.h"
 #include "launch.h"
@@ -42,6 +42,8 @@ struct epic_member {
     double *pos = nullptr;   // 3 cap
     float *attr = nullptr;   // NATTR cap
     int *orig = nullptr;
+    float *axbuf = nullptr;            // 12 cap: two sets of half axes
+    float *ax_cur[6] = {}, *ax_stale[6] = {};
     ParcelsDev pd{};
     // sort
     int *keys = nullptr, *count = nullptr, *scount = nullptr, *count2 = nullptr, *scount2 = nullptr;
@@ -123,6 +125,11 @@ void bind_parcels(epic_member *m)
     for (int i = 0; i < 3; ++i) p.vtau[i] = a + (25 + i) * c;
     for (int i = 0; i < 8; ++i) p.strain[i] = a + (28 + i) * c;
     p.orig = m->orig;
+    // the stale set is the reference's Vetas/Vtaus cache (par2grid l_reuse)
+    for (int i = 0; i < 3; ++i) {
+        p.veta[i] = m->ax_stale[i];
+        p.vtau[i] = m->ax_stale[3 + i];
+    }
 }
 
 void collect_timers(epic_member *m);
@@ -232,16 +239,18 @@ void ensure_keys(epic_member *m)
     m->keys_valid = true;
 }
 
+// cell sort: histogram -> offsets -> placement (order inside a cell is
+// arbitrary here; par2grid orders each cell by caller index in LDS)
 void run_sort(epic_member *m)
 {
     const int64_t nc = ncells(m->g);
     launch_exclusive_scan(m->count, nc, m->cell_start, m->bsum, m->s);
     launch_fill_int(m->cursor, nc, 0, m->s);
-    launch_place(m->n, m->keys, m->cell_start, m->cursor, m->perm_tmp, m->s);
-    launch_cell_order(nc, m->cell_start, m->perm_tmp, m->orig, m->perm, m->s);
+    launch_place(m->n, m->keys, m->cell_start, m->cursor, m->perm, m->s);
 }
 
-void run_par2grid(epic_member *m, int reuse, int store_axes)
+// par2grid with the given half axes (cur: this stage's, stale: the cached set)
+void run_par2grid(epic_member *m, float *const *axes)
 {
     PhaseTimer t(m, EPIC_TIMER_PAR2GRID);
     {
@@ -249,8 +258,14 @@ void run_par2grid(epic_member *m, int reuse, int store_axes)
         ensure_keys(m);
         run_sort(m);
     }
-    launch_p2g(m->g, m->pd, m->perm, m->cell_start, m->count, m->scount, m->staging, m->fld, m->nparg, m->nsparg,
-               reuse, store_axes, m->err_d, m->s);
+    launch_p2g(m->g, m->pd, axes, m->perm, m->cell_start, m->count, m->scount, m->staging, m->fld, m->nparg,
+               m->nsparg, m->err_d, m->s);
+}
+
+void run_axes(epic_member *m, float *const *axes)
+{
+    PhaseTimer t(m, EPIC_TIMER_AXES);
+    launch_axes(m->g, m->pd, axes, m->err_d, m->s);
 }
 
 FftXArgs xargs(epic_member *m)
@@ -468,6 +483,7 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     if (bad(hipStreamCreate(&m->s))) return EPIC_E_HIP;
     const int64_t c = m->cap, nc = ncells(g);
     if (bad(dalloc(&m->pos, 3 * c)) || bad(dalloc(&m->attr, (int64_t)NATTR * c)) || bad(dalloc(&m->orig, c)) ||
+        bad(dalloc(&m->axbuf, 12 * c)) ||
         bad(dalloc(&m->keys, c)) || bad(dalloc(&m->perm_tmp, c)) || bad(dalloc(&m->perm, c)) ||
         bad(dalloc(&m->count, nc)) || bad(dalloc(&m->scount, nc)) || bad(dalloc(&m->count2, nc)) ||
         bad(dalloc(&m->scount2, nc)) || bad(dalloc(&m->cell_start, nc + 1)) || bad(dalloc(&m->cursor, nc)) ||
@@ -496,6 +512,10 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
         bad(hipMemcpy(m->twy, ty.data(), sizeof(float2) * ty.size(), hipMemcpyHostToDevice)) ||
         bad(hipMemcpy(m->twz, tz.data(), sizeof(double2) * tz.size(), hipMemcpyHostToDevice)))
         return EPIC_E_HIP;
+    for (int i = 0; i < 6; ++i) {
+        m->ax_cur[i] = m->axbuf + i * c;
+        m->ax_stale[i] = m->axbuf + (6 + i) * c;
+    }
     bind_parcels(m);
     *out = m;
     return EPIC_OK;
@@ -505,7 +525,7 @@ int epic_destroy(epic_handle m)
 {
     if (!m) return EPIC_OK;
     if (m->s) (void)hipStreamSynchronize(m->s);
-    void *ptrs[] = {m->pos, m->attr, m->orig, m->keys, m->count, m->scount, m->count2, m->scount2, m->cell_start,
+    void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->count, m->scount, m->count2, m->scount2, m->cell_start,
                     m->cursor, m->perm_tmp, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging, m->T,
                     m->twx, m->twy, m->twz, m->err_d};
     for (void *p : ptrs)
@@ -678,8 +698,9 @@ int epic_set_field(epic_handle m, int which, const double *in)
 int epic_par2grid(epic_handle m, int l_reuse)
 {
     if (!m) return EPIC_E_INVALID;
-    const int reuse = (l_reuse == 1);
-    run_par2grid(m, reuse, reuse ? 0 : 1);
+    // get_points stores the axes it computes (parcel_ellipsoid.f90:216-247)
+    if (l_reuse != 1) run_axes(m, m->ax_stale);
+    run_par2grid(m, m->ax_stale);
     return check(m, "epic_par2grid");
 }
 
@@ -716,19 +737,28 @@ int epic_ls_rk_step(epic_handle m, double *t, double dt, double t_initial)
     const int ns = (m->rk_order == 4) ? 5 : 3;
     const int64_t nc = ncells(m->g);
     for (int s = 1; s <= ns; ++s) {
-        const int reuse = (s == 1 && *t > t_initial) ? 1 : 0;
-        run_par2grid(m, reuse, 0);
+        // the first par2grid of a step reuses the cached axes (ls_rk.f90:93);
+        // grid2par and later stages use the axes of the current B
+        const bool reuse = (s == 1 && *t > t_initial);
+        run_axes(m, m->ax_cur);
+        run_par2grid(m, reuse ? m->ax_stale : m->ax_cur);
         run_vor2vel(m);
         run_vtend(m);
         launch_fill_int(m->count2, nc, 0, m->s);
         launch_fill_int(m->scount2, nc, 0, m->s);
-        PhaseTimer t(m, EPIC_TIMER_PUSH);
-        launch_rk_stage(m->g, m->pd, m->fld, s, ns, cb[s - 1] * dt, ca[s - 1], s == ns ? 1 : 0, s == ns ? 1 : 0,
-                        m->keys, m->count2, m->scount2, m->err_d, m->s);
+        {
+            PhaseTimer t(m, EPIC_TIMER_PUSH);
+            launch_rk_stage_tiles(m->g, m->pd, m->ax_cur, m->perm, m->cell_start, m->fld, s, ns, cb[s - 1] * dt,
+                                  ca[s - 1], s == ns ? 1 : 0, m->keys, m->count2, m->scount2, m->err_d, m->s);
+        }
         std::swap(m->count, m->count2);
         std::swap(m->scount, m->scount2);
         m->keys_valid = true;
     }
+    // the axes of the last stage become the cache of the next step's first
+    // par2grid (grid2par stores them, parcel_interpl.f90:442)
+    for (int i = 0; i < 6; ++i) std::swap(m->ax_cur[i], m->ax_stale[i]);
+    bind_parcels(m);
     const int rc = check(m, "epic_ls_rk_step");
     if (rc == EPIC_OK) *t = *t + dt;
     return rc;

@@ -89,4 +89,27 @@ __device__ inline int wrap(int i, int n)
 // Fortran int(): truncation toward zero
 __device__ inline double ftrunc(double v) { return trunc(v); }
 
+// ---------------------------------------------------------------------------
+// wave-aggregated integer tickets: lanes of one wavefront holding the same key
+// in consecutive lanes share one atomicAdd on counter[key]; returns the lane's
+// ticket (old counter value + its rank within the run).  key < 0: no ticket.
+// Must be called by all lanes of the wavefront.
+// ---------------------------------------------------------------------------
+__device__ inline int wave_run_ticket(int key, int *counter)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const bool head = (lane == 0) || (prev != key);
+    const unsigned long long heads = __ballot(head);
+    // run head: highest head lane <= lane; run end: next head lane
+    const unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+    const int hl = 63 - __clzll(below);
+    const unsigned long long above = heads & ~((2ull << lane) - 1 | (lane == 63 ? ~0ull : 0ull));
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    int base = 0;
+    if (head && key >= 0) base = atomicAdd(&counter[key], next - lane);
+    base = __shfl(base, hl, 64);
+    return base + (lane - hl);
+}
+
 }  // namespace epic

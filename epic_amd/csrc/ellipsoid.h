@@ -64,7 +64,8 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         const double j2 = 0.5 * (aa00 + aa11 + aa22);
         const double j3 = A[0] * (A[4] * A[8] - A[5] * A[5]) - A[1] * (A[1] * A[8] - A[2] * A[5]) +
                           A[2] * (A[1] * A[5] - A[2] * A[4]);
-        double smp = 0.5 * j3 * pow(3.0 / j2, 1.5);
+        const double q3 = 3.0 / j2;
+        double smp = 0.5 * j3 * (q3 * sqrt(q3));   // (3/j2)^(3/2)
         smp = fmin(fmax(-1.0, smp), 1.0);
         double alpha = (1.0 / 3.0) * acos(smp);
         if (alpha > 3.14159265358979323846 / 6.0) alpha = alpha + (2.0 / 3.0) * 3.14159265358979323846;
@@ -83,8 +84,9 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         if (l1 > lm) { im = 1; lm = l1; }
         if (l2 > lm) { im = 2; lm = l2; }
         double s1[3];
+        const double ilm = 1.0 / lm;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) s1[i] = r[im][i] / lm;
+        for (int i = 0; i < 3; ++i) s1[i] = r[im][i] * ilm;
         if (im == 1) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) r[1][i] = r[0][i];
@@ -103,12 +105,11 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         }
         l1 = dnorm3(r[1][0], r[1][1], r[1][2]);
         l2 = dnorm3(r[2][0], r[2][1], r[2][2]);
-        if (l2 > l1) {
+        {
+            const bool use2 = l2 > l1;
+            const double il = 1.0 / (use2 ? l2 : l1);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) r[1][i] = r[2][i] / l2;
-        } else {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) r[1][i] = r[1][i] / l1;
+            for (int i = 0; i < 3; ++i) r[1][i] = (use2 ? r[2][i] : r[1][i]) * il;
         }
         double a1[3], a2[3];
 #pragma unroll
@@ -143,12 +144,11 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         }
         const double m0 = dnorm3(u0[0], u0[1], u0[2]);
         const double m1 = dnorm3(u1[0], u1[1], u1[2]);
-        if (m1 > m0) {
+        {
+            const bool use1 = m1 > m0;
+            const double im_ = 1.0 / (use1 ? m1 : m0);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) V[2][i] = u1[i] / m1;
-        } else {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) V[2][i] = u0[i] / m0;
+            for (int i = 0; i < 3; ++i) V[2][i] = (use1 ? u1[i] : u0[i]) * im_;
         }
         V[1][0] = V[2][1] * V[0][2] - V[2][2] * V[0][1];
         V[1][1] = V[2][2] * V[0][0] - V[2][0] * V[0][2];
@@ -158,9 +158,9 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         V[2][2] = V[0][0] * V[1][1] - V[0][1] * V[1][0];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double l = dnorm3(V[c][0], V[c][1], V[c][2]);
+            const double il = 1.0 / dnorm3(V[c][0], V[c][1], V[c][2]);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) V[c][i] /= l;
+            for (int i = 0; i < 3; ++i) V[c][i] *= il;
         }
     }
     // three-swap descending sort (scherzinger.f90:85-110, 270-300)

@@ -194,6 +194,251 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a)
     if (e) atomicOr(a.err, e);
 }
 
+// ---------------------------------------------------------------------------
+// Tiled form used by ls_rk_step: one work-group per (8x8 column tile, z
+// chunk) marching up the chunk one cell layer at a time, like par2grid.
+// The 14 gathered fields of the node planes k-1..k+2 are staged in an LDS
+// ring (node-major, 16 floats per node) so every corner is four 16-byte LDS
+// reads; parcels are read through the cell sort, support points come from
+// the half axes cached for this stage.
+// ---------------------------------------------------------------------------
+struct G2PTileArgs {
+    Grid g;
+    ParcelsDev p;
+    const float *axes[6];
+    const int *perm;
+    const int *cell_start;
+    const float *fld;
+    int stage, nstages, reflect;
+    double cbdt, ca;
+    int *keys, *count, *scount;
+    int *err;
+    int tz, ntx, nty, nchz;
+};
+
+constexpr int G2P_THREADS = 256;
+constexpr int NODE_PAD = 16;   // floats per staged node (14 used)
+
+__global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(G2PTileArgs a)
+{
+    const Grid &g = a.g;
+    const int tx = blockIdx.x % a.ntx;
+    const int ty = (blockIdx.x / a.ntx) % a.nty;
+    const int cz = blockIdx.x / (a.ntx * a.nty);
+    const int i0 = tx * P2G_TX, j0 = ty * P2G_TY;
+    const int k0 = -1 + cz * a.tz;
+    const int k1 = min(k0 + a.tz, g.nz + 1);
+    const int tid = threadIdx.x;
+    const int64_t PS = plane_stride(g), FS = field_stride(g);
+    constexpr int NN = P2G_BOX * P2G_BOX;
+
+    __shared__ float4 ring[4 * NN * (NODE_PAD / 4)];
+    __shared__ int row_slot[P2G_TY], row_off[P2G_TY + 1];
+    int e = 0;
+
+    auto load_plane = [&](int P) {
+        float *dst = (float *)(ring + ((P + 8) & 3) * NN * (NODE_PAD / 4));
+        const bool inside = (P >= -1 && P <= g.nz + 1);
+        for (int u = tid; u < 14 * NN; u += G2P_THREADS) {
+            const int f = u / NN, node = u - f * NN;
+            const int ly = node / P2G_BOX, lx = node - ly * P2G_BOX;
+            float v = 0.0f;
+            if (inside) {
+                const int gi = wrap(i0 - 1 + lx, g.nx), gj = wrap(j0 - 1 + ly, g.ny);
+                v = a.fld[(F_VELOG + f) * FS + (int64_t)(P + 1) * PS + (int64_t)gj * g.nx + gi];
+            }
+            dst[node * NODE_PAD + f] = v;
+        }
+    };
+
+    load_plane(k0 - 1);
+    load_plane(k0);
+    load_plane(k0 + 1);
+    for (int k = k0; k < k1; ++k) {
+        __syncthreads();   // the previous layer no longer reads plane k-2's slot
+        load_plane(k + 2);
+        if (tid < P2G_TY) {
+            const int c0 = ((k + 1) * g.ny + (j0 + tid)) * g.nx + i0;
+            row_slot[tid] = a.cell_start[c0];
+            row_off[tid + 1] = a.cell_start[c0 + P2G_TX] - a.cell_start[c0];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            row_off[0] = 0;
+            for (int r = 0; r < P2G_TY; ++r) row_off[r + 1] += row_off[r];
+        }
+        __syncthreads();
+        const int total = row_off[P2G_TY];
+        for (int base = 0; base < total; base += G2P_THREADS) {   // uniform rounds: the ballot needs every lane
+            const int q = base + tid;
+            int key = -1;
+            bool small = false;
+            auto process = [&]() {
+            if (q >= total) return;
+            int r = 0;
+            while (r + 1 < P2G_TY && row_off[r + 1] <= q) ++r;
+            const int pi = a.perm[row_slot[r] + (q - row_off[r])];
+            double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
+            double B[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) B[c] = (double)a.p.B[c][pi];
+            const double vol = (double)a.p.vol[pi];
+            const double ve[3] = {a.axes[0][pi], a.axes[1][pi], a.axes[2][pi]};
+            const double vt[3] = {a.axes[3][pi], a.axes[4][pi], a.axes[5][pi]};
+            int cc[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
+            const int cx = wrap(cc[0], g.nx) - i0, cy = wrap(cc[1], g.ny) - j0;
+            if (cx < 0 || cx >= P2G_TX || cy < 0 || cy >= P2G_TY || cc[2] != k) {
+                e |= ERR_CELL_MISMATCH;
+                return;
+            }
+            const bool acc_prev = a.stage > 1;
+            float dp[3], dv[3], st[8];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                dp[c] = acc_prev ? a.p.dpos[c][pi] : 0.0f;
+                dv[c] = acc_prev ? a.p.dvor[c][pi] : 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) st[c] = 0.0f;
+#pragma unroll
+            for (int jp = 0; jp < 4; ++jp) {
+                const double cs = support_cos(jp), sn = support_sin(jp);
+                int off[3];
+                float w[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double pt = pos[d] + ve[d] * cs + vt[d] * sn;
+                    const double xyz = (pt - g.lower[d]) * g.dxi[d];
+                    const double fl = floor(xyz);
+                    off[d] = (int)fl - cc[d];
+                    w[d] = (float)(xyz - fl);
+                }
+                if (off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1 ||
+                    k + off[2] < -1 || k + off[2] > g.nz) {
+                    e |= ERR_OUT_OF_GRID;
+                    continue;
+                }
+                const int lx = cx + off[0] + 1, ly = cy + off[1] + 1, P = k + off[2];
+                const float pxc = 1.0f - w[0], pyc = 1.0f - w[1], pzc = 1.0f - w[2];
+                const float w00 = pyc * pxc, w10 = pyc * w[0], w01 = w[1] * pxc, w11 = w[1] * w[0];
+                // corners in Fortran element order of ww(0:1,0:1,0:1): k fastest
+                const float wt[8] = {pzc * w00, w[2] * w00, pzc * w01, w[2] * w01,
+                                     pzc * w10, w[2] * w10, pzc * w11, w[2] * w11};
+                float s[14];
+#pragma unroll
+                for (int f = 0; f < 14; ++f) s[f] = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int dz = m & 1, dy = (m >> 1) & 1, dx = (m >> 2) & 1;
+                    const float4 *nd = ring + (((P + dz + 8) & 3) * NN + (ly + dy) * P2G_BOX + (lx + dx)) * (NODE_PAD / 4);
+                    const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+                    const float vv[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
+#pragma unroll
+                    for (int f = 0; f < 14; ++f) s[f] += wt[m] * vv[f];
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dp[c] += 0.25f * s[c];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) st[c] += 0.25f * s[3 + c];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dv[c] += 0.25f * s[11 + c];
+            }
+            // dB/dt (FP64, B33 from the volume constraint)
+            double S[8], dB[5];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) S[c] = (double)st[c];
+            if (!dBdt(B, S, vol, dB)) {
+                e |= ERR_B33;
+                dB[0] = dB[1] = dB[2] = dB[3] = dB[4] = 0.0;
+            }
+            float db[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) db[c] = acc_prev ? a.p.dB[c][pi] + (float)dB[c] : (float)dB[c];
+            // push (ls_rk.f90:183-191), periodic wrap (parcel_bc.f90:17-21)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pos[c] = pos[c] + a.cbdt * (double)dp[c];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a.p.vor[c][pi] = (float)((double)a.p.vor[c][pi] + a.cbdt * (double)dv[c]);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) B[c] = B[c] + a.cbdt * (double)db[c];
+            pos[0] = pos[0] - g.extent[0] * trunc((pos[0] - g.center[0]) * g.hli[0]);
+            pos[1] = pos[1] - g.extent[1] * trunc((pos[1] - g.center[1]) * g.hli[1]);
+            if (a.reflect) {   // parcel_bc.f90:26-40
+                if (pos[2] > g.upper[2]) {
+                    pos[2] = 2.0 * g.upper[2] - pos[2];
+                    B[2] = -B[2];
+                    B[4] = -B[4];
+                } else if (pos[2] < g.lower[2]) {
+                    pos[2] = 2.0 * g.lower[2] - pos[2];
+                    B[2] = -B[2];
+                    B[4] = -B[4];
+                }
+            }
+            a.p.x[pi] = pos[0];
+            a.p.y[pi] = pos[1];
+            a.p.z[pi] = pos[2];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) a.p.B[c][pi] = (float)B[c];
+            if (a.stage < a.nstages) {
+                const float ca = (float)a.ca;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    a.p.dpos[c][pi] = ca * dp[c];
+                    a.p.dvor[c][pi] = ca * dv[c];
+                }
+#pragma unroll
+                for (int c = 0; c < 5; ++c) a.p.dB[c][pi] = ca * db[c];
+            }
+            // cell of the new position for the next stage's sort
+            const int ni = (int)floor((pos[0] - g.lower[0]) * g.dxi[0]);
+            const int nj = (int)floor((pos[1] - g.lower[1]) * g.dxi[1]);
+            const int nk = (int)floor((pos[2] - g.lower[2]) * g.dxi[2]);
+            if (nk < -1 || nk > g.nz) e |= ERR_PARCEL_COUNT;
+            else key = ((nk + 1) * g.ny + wrap(nj, g.ny)) * g.nx + wrap(ni, g.nx);
+            a.keys[pi] = key;
+            small = (vol <= g.vmin);
+            };
+            process();
+            // next stage's cell histogram, one atomic per run of equal keys
+            wave_run_ticket(key, a.count);
+            wave_run_ticket(small ? key : -1, a.scount);
+        }
+    }
+    if (e) atomicOr(a.err, e);
+}
+
+void launch_rk_stage_tiles(const Grid &g, const ParcelsDev &p, const float *const axes[6], const int *perm,
+                           const int *cell_start, const float *fld, int stage, int nstages, double cbdt, double ca,
+                           int reflect, int *keys, int *count, int *scount, int *err, hipStream_t s)
+{
+    if (p.n == 0) return;
+    const P2GLayout L = p2g_layout(g);
+    G2PTileArgs a;
+    a.g = g;
+    a.p = p;
+    for (int c = 0; c < 6; ++c) a.axes[c] = axes[c];
+    a.perm = perm;
+    a.cell_start = cell_start;
+    a.fld = fld;
+    a.stage = stage;
+    a.nstages = nstages;
+    a.reflect = reflect;
+    a.cbdt = cbdt;
+    a.ca = ca;
+    a.keys = keys;
+    a.count = count;
+    a.scount = scount;
+    a.err = err;
+    a.tz = L.tz;
+    a.ntx = L.ntx;
+    a.nty = L.nty;
+    a.nchz = L.nchz;
+    hipLaunchKernelGGL(k_g2p_tiles, dim3(L.ntx * L.nty * L.nchz), dim3(G2P_THREADS), 0, s, a);
+}
+
 static int g2p_blocks(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 16384); }
 
 void launch_grid2par(const Grid &g, const ParcelsDev &p, const float *fld, int add, int *err, hipStream_t s)

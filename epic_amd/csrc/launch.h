@@ -17,15 +17,20 @@ void launch_cell_keys(const Grid &g, const ParcelsDev &p, int *keys, int *count,
 void launch_place(int64_t n, const int *keys, const int *cell_start, int *cursor, int *perm_tmp, hipStream_t s);
 void launch_cell_order(int64_t nc, const int *cell_start, const int *perm_tmp, const int *orig, int *perm,
                        hipStream_t s);
-void launch_p2g(const Grid &g, const ParcelsDev &p, const int *perm, const int *cell_start, const int *count,
-                const int *scount, float *staging, float *fld, int *nparg, int *nsparg, int reuse, int store_axes,
-                int *err, hipStream_t s);
+void launch_axes(const Grid &g, const ParcelsDev &p, float *const axes[6], int *err, hipStream_t s);
+void launch_p2g(const Grid &g, const ParcelsDev &p, const float *const axes[6], const int *perm,
+                const int *cell_start, const int *count, const int *scount, float *staging, float *fld, int *nparg,
+                int *nsparg, int *err, hipStream_t s);
 
 // ---- grid2par + push (g2p.hip) ----
 void launch_grid2par(const Grid &g, const ParcelsDev &p, const float *fld, int add, int *err, hipStream_t s);
 void launch_rk_stage(const Grid &g, const ParcelsDev &p, const float *fld, int stage, int nstages, double cbdt,
                      double ca, int reflect, int store_axes, int *keys, int *count, int *scount, int *err,
                      hipStream_t s);
+
+void launch_rk_stage_tiles(const Grid &g, const ParcelsDev &p, const float *const axes[6], const int *perm,
+                           const int *cell_start, const float *fld, int stage, int nstages, double cbdt, double ca,
+                           int reflect, int *keys, int *count, int *scount, int *err, hipStream_t s);
 
 // ---- horizontal FFTs (fft.hip) ----
 struct FftXArgs {

@@ -159,14 +159,18 @@ int64_t scan_scratch_ints(int64_t n) { return (n + SCAN_BLOCK - 1) / SCAN_BLOCK 
 // ---------------------------------------------------------------------------
 // 3./4. placement and per-cell ordering
 // ---------------------------------------------------------------------------
+// Runs of equal keys in consecutive lanes (the parcels are nearly in cell
+// order) take one ticket per run.
 __global__ void k_place(int64_t n, const int *__restrict__ keys, const int *__restrict__ cell_start,
                         int *__restrict__ cursor, int *__restrict__ perm_tmp)
 {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        const int key = keys[q];
-        if (key < 0) continue;
-        const int slot = cell_start[key] + atomicAdd(&cursor[key], 1);
-        perm_tmp[slot] = (int)q;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nround = (n + stride - 1) / stride;
+    for (int64_t rnd = 0; rnd < nround; ++rnd) {   // every lane of a wave runs the same rounds
+        const int64_t q = rnd * stride + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+        const int key = (q < n) ? keys[q] : -1;
+        const int base = wave_run_ticket(key, cursor);
+        if (key >= 0) perm_tmp[cell_start[key] + base] = (int)q;
     }
 }
 
@@ -214,15 +218,16 @@ __global__ void k_cell_order(int64_t nc, const int *__restrict__ cell_start, con
 struct P2GArgs {
     Grid g;
     ParcelsDev p;
+    const float *axes[6];   // Veta (3), Vtau (3) per parcel, FP32
     const int *perm;
     const int *cell_start;
     float *staging;
     int *err;
-    int tz;        // cell layers per chunk
+    int tz;                 // cell layers per chunk
     int ntx, nty, nchz;
-    int reuse;       // use cached support axes (l_reuse)
-    int store_axes;  // cache the recomputed axes (par2grid called on its own)
 };
+
+constexpr int NEXT = 300;   // extended cells of a layer: 10 x 10 x 3
 
 __device__ inline int box_planes(int tz) { return tz + 3; }
 
@@ -238,22 +243,30 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
     const int tid = threadIdx.x;
 
     constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
+    constexpr int NCELL = P2G_TX * P2G_TY;
     __shared__ float ring[4 * PLANE];
     __shared__ float frac[P2G_MAXP * 12];
     __shared__ float vals[P2G_MAXP * 7];
-    __shared__ int codes[P2G_MAXP];
-    __shared__ int row_slot[P2G_TY];     // first sorted slot of each row
-    __shared__ int row_off[P2G_TY + 1];  // concatenated offset of each row
-    __shared__ int cell_off[P2G_TY][P2G_TX + 1];   // concatenated offsets of cells
-    __shared__ int s_err;
+    __shared__ unsigned short extc[P2G_MAXP * 4];  // extended cell of every support point (511: none)
+    __shared__ unsigned short spos[P2G_MAXP];      // position of the parcel in caller order within the batch
+    __shared__ unsigned char cell_of[P2G_MAXP];    // own cell of each batch entry
+    __shared__ __attribute__((aligned(16))) unsigned short items[P2G_MAXP * 4];   // points binned by extended cell
+    int *const origk = (int *)items;                // caller indices, only until the binning
+    __shared__ int bin_cnt[NEXT], bin_start[NEXT + 1], bin_fill[NEXT];
+    __shared__ int cell_off[NCELL + 1];            // concatenated offsets of the layer's cells
+    __shared__ int row_slot[P2G_TY];
+    __shared__ int row_off[P2G_TY + 1];
+    __shared__ int batch_cell[NCELL + 1];          // cell-aligned batch boundaries
+    __shared__ int s_nbatch, s_err;
 
     for (int u = tid; u < 4 * PLANE; u += P2G_THREADS) ring[u] = 0.0f;
+    for (int u = tid; u < NEXT; u += P2G_THREADS) bin_cnt[u] = bin_fill[u] = 0;
     if (tid == 0) s_err = 0;
 
     const int nbox = box_planes(a.tz);
     float *stage = a.staging + (int64_t)blockIdx.x * nbox * PLANE;
-    // extended cell owned by this thread (300 used)
-    const bool ecell = tid < 300;
+    // extended cell owned by this thread in the accumulation (300 used)
+    const bool ecell = tid < NEXT;
     const int ez = tid / 100 - 1;
     const int ey = (tid % 100) / 10 - 1;
     const int ex = tid % 10 - 1;
@@ -269,63 +282,66 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
 
     for (int k = k0; k < k1; ++k) {
         __syncthreads();
-        // row ranges of this layer
+        // ---- the layer's cells: rows of 8 consecutive cells in the sort ----
         if (tid < P2G_TY) {
             const int c0 = ((k + 1) * g.ny + (j0 + tid)) * g.nx + i0;
             row_slot[tid] = a.cell_start[c0];
-            for (int c = 0; c <= P2G_TX; ++c) cell_off[tid][c] = a.cell_start[c0 + c] - a.cell_start[c0];
+            row_off[tid + 1] = a.cell_start[c0 + P2G_TX] - a.cell_start[c0];
         }
         __syncthreads();
         if (tid == 0) {
-            int acc = 0;
-            for (int r = 0; r < P2G_TY; ++r) {
-                row_off[r] = acc;
-                acc += cell_off[r][P2G_TX];
+            row_off[0] = 0;
+            for (int r = 0; r < P2G_TY; ++r) row_off[r + 1] += row_off[r];
+            // cell-aligned batches of at most P2G_MAXP parcels
+            int nb = 0, acc = 0;
+            batch_cell[0] = 0;
+            for (int c = 0; c < NCELL; ++c) {
+                const int r = c / P2G_TX, cx = c % P2G_TX;
+                const int base = ((k + 1) * g.ny + (j0 + r)) * g.nx + i0 + cx;
+                const int m = a.cell_start[base + 1] - a.cell_start[base];
+                cell_off[c] = row_off[r] + (a.cell_start[base] - row_slot[r]);
+                if (m > P2G_MAXP) s_err |= ERR_OUT_OF_GRID;   // one cell beyond a batch: not supported
+                if (acc + m > P2G_MAXP) {
+                    batch_cell[++nb] = c;
+                    acc = 0;
+                }
+                acc += m;
             }
-            row_off[P2G_TY] = acc;
+            cell_off[NCELL] = row_off[P2G_TY];
+            batch_cell[++nb] = NCELL;
+            s_nbatch = nb;
         }
         __syncthreads();
-        if (tid < P2G_TY)
-            for (int c = 0; c <= P2G_TX; ++c) cell_off[tid][c] += row_off[tid];
-        __syncthreads();
-        const int total = row_off[P2G_TY];
+        const int nbatch = s_nbatch;
 
-        for (int b0 = 0; b0 < total; b0 += P2G_MAXP) {
-            const int nb = min(P2G_MAXP, total - b0);
-            // ---- load + support points (FP64) ----
+        for (int bt = 0; bt < nbatch; ++bt) {
+            const int cb0 = batch_cell[bt], cb1 = batch_cell[bt + 1];
+            const int q0 = cell_off[cb0];
+            const int nb = cell_off[cb1] - q0;
+            if (nb > P2G_MAXP) continue;   // flagged above
+            // owner cell of every entry
+            for (int c = cb0 + tid; c < cb1; c += P2G_THREADS)
+                for (int q = cell_off[c]; q < cell_off[c + 1]; ++q) cell_of[q - q0] = (unsigned char)c;
+            __syncthreads();
+            // ---- load: support points (FP64 positions, cached FP32 half axes) ----
             for (int q = tid; q < nb; q += P2G_THREADS) {
-                const int gq = b0 + q;
-                int r = 0;
-                while (r + 1 < P2G_TY && row_off[r + 1] <= gq) ++r;
+                const int gq = q0 + q;
+                const int c = cell_of[q];
+                const int r = c / P2G_TX, cx = c % P2G_TX;
                 const int slot = row_slot[r] + (gq - row_off[r]);
                 const int pi = a.perm[slot];
+                origk[q] = a.p.orig[pi];
                 const double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
-                const double vol = (double)a.p.vol[pi];
-                double ve[3], vt[3];
-                if (a.reuse) {
-                    for (int d = 0; d < 3; ++d) {
-                        ve[d] = (double)a.p.veta[d][pi];
-                        vt[d] = (double)a.p.vtau[d][pi];
-                    }
-                } else {
-                    const double B[5] = {a.p.B[0][pi], a.p.B[1][pi], a.p.B[2][pi], a.p.B[3][pi], a.p.B[4][pi]};
-                    if (!support_axes(B, vol, ve, vt)) {
-                        atomicOr(&s_err, ERR_B33);
-                        for (int d = 0; d < 3; ++d) ve[d] = vt[d] = 0.0;
-                    }
-                    if (a.store_axes) {
-                        for (int d = 0; d < 3; ++d) {
-                            a.p.veta[d][pi] = (float)ve[d];
-                            a.p.vtau[d][pi] = (float)vt[d];
-                        }
-                    }
-                }
+                const double ve[3] = {a.axes[0][pi], a.axes[1][pi], a.axes[2][pi]};
+                const double vt[3] = {a.axes[3][pi], a.axes[4][pi], a.axes[5][pi]};
                 int cc[3];
+#pragma unroll
                 for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
-                int code = 0;
+#pragma unroll
                 for (int jp = 0; jp < 4; ++jp) {
                     const double cs = support_cos(jp), sn = support_sin(jp);
                     int off[3];
+#pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         const double pt = pos[d] + ve[d] * cs + vt[d] * sn;
                         const double xyz = (pt - g.lower[d]) * g.dxi[d];
@@ -333,19 +349,18 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
                         off[d] = (int)fl - cc[d];
                         frac[q * 12 + jp * 3 + d] = (float)(xyz - fl);
                     }
-                    if (off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1 ||
-                        cc[2] + off[2] < -1 || cc[2] + off[2] > g.nz) {
+                    int e = 511;
+                    if (off[0] >= -1 && off[0] <= 1 && off[1] >= -1 && off[1] <= 1 && off[2] >= -1 && off[2] <= 1 &&
+                        cc[2] + off[2] >= -1 && cc[2] + off[2] <= g.nz)
+                        e = (off[2] + 1) * 100 + (r + off[1] + 1) * 10 + (cx + off[0] + 1);
+                    else
                         atomicOr(&s_err, ERR_OUT_OF_GRID);
-                        off[0] = off[1] = off[2] = 7;   // never matched
-                    }
-                    const int dcode = (off[0] == 7) ? 31 : (off[0] + 1) + 3 * (off[1] + 1) + 9 * (off[2] + 1);
-                    code |= dcode << (5 * jp);
+                    extc[q * 4 + jp] = (unsigned short)e;
                 }
-                codes[q] = code;
                 const double qc = fmax(0.0, (double)a.p.hum[pi] - g.q0 * exp(g.lambda_c * (g.lower[2] - pos[2])));
                 const double btot = (double)a.p.buoy[pi] + g.glat * qc;
                 float *v = vals + q * 7;
-                v[0] = (float)(0.25 * vol);
+                v[0] = (float)(0.25 * (double)a.p.vol[pi]);
                 v[1] = a.p.vor[0][pi];
                 v[2] = a.p.vor[1][pi];
                 v[3] = a.p.vor[2][pi];
@@ -354,7 +369,62 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
                 v[6] = a.p.hum[pi];
             }
             __syncthreads();
-            // ---- gather into extended cells (registers) ----
+            // ---- order inside each cell by caller index (placement order is arbitrary) ----
+            for (int c = cb0 + tid; c < cb1; c += P2G_THREADS) {
+                const int lo = cell_off[c] - q0, hi = cell_off[c + 1] - q0;
+                for (int q = lo; q < hi; ++q) {
+                    const int oq = origk[q];
+                    int rank = 0;
+                    for (int q2 = lo; q2 < hi; ++q2) rank += (origk[q2] < oq);
+                    spos[q] = (unsigned short)(lo + rank);
+                }
+            }
+            __syncthreads();
+            // ---- bin the support points by extended cell ----
+            for (int u = tid; u < nb * 4; u += P2G_THREADS) {
+                const int e = extc[u];
+                if (e < NEXT) atomicAdd(&bin_cnt[e], 1);
+            }
+            __syncthreads();
+            if (tid < 64) {   // exclusive scan of 300 counts by one wavefront
+                int run = 0;
+                for (int base = 0; base < NEXT; base += 64) {
+                    const int e = base + tid;
+                    const int v = (e < NEXT) ? bin_cnt[e] : 0;
+                    int incl = v;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const int t = __shfl_up(incl, o, 64);
+                        if (tid >= o) incl += t;
+                    }
+                    if (e < NEXT) bin_start[e] = run + incl - v;
+                    run += __shfl(incl, 63, 64);
+                }
+                if (tid == 0) bin_start[NEXT] = run;
+            }
+            __syncthreads();
+            for (int u = tid; u < nb * 4; u += P2G_THREADS) {
+                const int e = extc[u];
+                if (e < NEXT) items[bin_start[e] + atomicAdd(&bin_fill[e], 1)] = (unsigned short)u;
+            }
+            __syncthreads();
+            // sort every bin by (caller order, point): the accumulation order is a
+            // function of the parcel set alone
+            if (ecell) {
+                const int lo = bin_start[tid], hi = bin_start[tid + 1];
+                for (int x = lo + 1; x < hi; ++x) {
+                    const unsigned short it = items[x];
+                    const int key = spos[it >> 2] * 4 + (it & 3);
+                    int y = x - 1;
+                    while (y >= lo && (spos[items[y] >> 2] * 4 + (items[y] & 3)) > key) {
+                        items[y + 1] = items[y];
+                        --y;
+                    }
+                    items[y + 1] = it;
+                }
+            }
+            __syncthreads();
+            // ---- every extended cell accumulates its points (registers) ----
             float acc[8][P2G_NF];
 #pragma unroll
             for (int c = 0; c < 8; ++c)
@@ -362,40 +432,27 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
                 for (int f = 0; f < P2G_NF; ++f) acc[c][f] = 0.0f;
             bool any = false;
             if (ecell) {
-                for (int dy = -1; dy <= 1; ++dy) {
-                    const int cy = ey - dy;
-                    if (cy < 0 || cy >= P2G_TY) continue;
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int cx = ex - dx;
-                        if (cx < 0 || cx >= P2G_TX) continue;
-                        const int target = (dx + 1) + 3 * (dy + 1) + 9 * (ez + 1);
-                        const int qa = max(cell_off[cy][cx] - b0, 0);
-                        const int qb = min(cell_off[cy][cx + 1] - b0, nb);
-                        for (int q = qa; q < qb; ++q) {
-                            const int code = codes[q];
-                            for (int jp = 0; jp < 4; ++jp) {
-                                if (((code >> (5 * jp)) & 31) != target) continue;
-                                any = true;
-                                const float fx = frac[q * 12 + jp * 3 + 0];
-                                const float fy = frac[q * 12 + jp * 3 + 1];
-                                const float fz = frac[q * 12 + jp * 3 + 2];
-                                const float *v = vals + q * 7;
-                                const float wxy[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx),
-                                                      fy * fx};
+                const int lo = bin_start[tid], hi = bin_start[tid + 1];
+                any = hi > lo;
+                for (int x = lo; x < hi; ++x) {
+                    const int it = items[x];
+                    const int q = it >> 2, jp = it & 3;
+                    const float fx = frac[q * 12 + jp * 3 + 0];
+                    const float fy = frac[q * 12 + jp * 3 + 1];
+                    const float fz = frac[q * 12 + jp * 3 + 2];
+                    const float *v = vals + q * 7;
+                    const float wxy[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx), fy * fx};
 #pragma unroll
-                                for (int c = 0; c < 8; ++c) {
-                                    const float wz = (c & 4) ? fz : 1.0f - fz;
-                                    const float wt = v[0] * (wz * wxy[c & 3]);
-                                    acc[c][0] += wt;
+                    for (int c = 0; c < 8; ++c) {
+                        const float wz = (c & 4) ? fz : 1.0f - fz;
+                        const float wt = v[0] * (wz * wxy[c & 3]);
+                        acc[c][0] += wt;
 #pragma unroll
-                                    for (int f = 1; f < P2G_NF; ++f) acc[c][f] += wt * v[f];
-                                }
-                            }
-                        }
+                        for (int f = 1; f < P2G_NF; ++f) acc[c][f] += wt * v[f];
                     }
                 }
             }
-            // ---- parity-phased add into the node ring ----
+            // ---- parity-phased add into the node ring (equal parity: disjoint nodes) ----
             const int kz = k + ez;
             const int par = ((ex + 2) & 1) | (((ey + 2) & 1) << 1) | (((kz + 4) & 1) << 2);
             for (int ph = 0; ph < 8; ++ph) {
@@ -413,7 +470,9 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
                 }
             }
             __syncthreads();
+            for (int u = tid; u < NEXT; u += P2G_THREADS) bin_cnt[u] = bin_fill[u] = 0;
         }
+        __syncthreads();
         // node plane k-1 is complete
         flush_plane(k - 1);
     }
@@ -423,7 +482,6 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
     flush_plane(k1);
     __syncthreads();
     flush_plane(k1 + 1);
-    // layers without parcels leave zeros: planes k0-1 .. k1+1 all written
     if (tid == 0 && s_err) atomicOr(a.err, s_err);
 }
 
@@ -561,6 +619,38 @@ __global__ void k_p2g_finalize(FinalizeArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// support-point half axes, once per parcel per stage (FP64 eigen-solve)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_axes(Grid g, ParcelsDev p, float *ax0, float *ax1, float *ax2, float *ax3,
+                                              float *ax4, float *ax5, int *err)
+{
+    int e = 0;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < p.n; q += (int64_t)gridDim.x * blockDim.x) {
+        const double B[5] = {p.B[0][q], p.B[1][q], p.B[2][q], p.B[3][q], p.B[4][q]};
+        double ve[3], vt[3];
+        if (!support_axes(B, (double)p.vol[q], ve, vt)) {
+            e |= ERR_B33;
+            ve[0] = ve[1] = ve[2] = vt[0] = vt[1] = vt[2] = 0.0;
+        }
+        ax0[q] = (float)ve[0];
+        ax1[q] = (float)ve[1];
+        ax2[q] = (float)ve[2];
+        ax3[q] = (float)vt[0];
+        ax4[q] = (float)vt[1];
+        ax5[q] = (float)vt[2];
+    }
+    if (e) atomicOr(err, e);
+}
+
+void launch_axes(const Grid &g, const ParcelsDev &p, float *const axes[6], int *err, hipStream_t s)
+{
+    if (p.n == 0) return;
+    const int nb = (int)std::min<int64_t>((p.n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_axes, dim3(nb), dim3(256), 0, s, g, p, axes[0], axes[1], axes[2], axes[3], axes[4], axes[5],
+                       err);
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 void launch_cell_keys(const Grid &g, const ParcelsDev &p, int *keys, int *count, int *scount, int *err, hipStream_t s)
@@ -594,14 +684,15 @@ P2GLayout p2g_layout(const Grid &g)
     return L;
 }
 
-void launch_p2g(const Grid &g, const ParcelsDev &p, const int *perm, const int *cell_start, const int *count,
-                const int *scount, float *staging, float *fld, int *nparg, int *nsparg, int reuse, int store_axes,
-                int *err, hipStream_t s)
+void launch_p2g(const Grid &g, const ParcelsDev &p, const float *const axes[6], const int *perm,
+                const int *cell_start, const int *count, const int *scount, float *staging, float *fld, int *nparg,
+                int *nsparg, int *err, hipStream_t s)
 {
     const P2GLayout L = p2g_layout(g);
     P2GArgs a;
     a.g = g;
     a.p = p;
+    for (int c = 0; c < 6; ++c) a.axes[c] = axes[c];
     a.perm = perm;
     a.cell_start = cell_start;
     a.staging = staging;
@@ -610,8 +701,6 @@ void launch_p2g(const Grid &g, const ParcelsDev &p, const int *perm, const int *
     a.ntx = L.ntx;
     a.nty = L.nty;
     a.nchz = L.nchz;
-    a.reuse = reuse;
-    a.store_axes = store_axes;
     hipLaunchKernelGGL(k_p2g_tiles, dim3(L.ntx * L.nty * L.nchz), dim3(P2G_THREADS), 0, s, a);
     FinalizeArgs f;
     f.g = g;

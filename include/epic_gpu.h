@@ -146,7 +146,8 @@ enum {
     EPIC_TIMER_VTEND = 3,
     EPIC_TIMER_PUSH = 4,
     EPIC_TIMER_GRID2PAR = 5,
-    EPIC_NUM_TIMERS = 6
+    EPIC_TIMER_AXES = 6,        /* support-point half axes (FP64 eigen-solve), once per stage */
+    EPIC_NUM_TIMERS = 7
 };
 int epic_set_profiling(epic_handle h, int on);
 int epic_get_timers(epic_handle h, double *ms, int *calls, int reset);
