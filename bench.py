@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of 1 GiB (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -98,8 +100,8 @@ def main():
     p2g_ms, p2g_calls = tm["par2grid"]
     phases = {k: round(v[0] / max(1, args.profile_steps), 4) for k, v in tm.items() if v[1]}
     push_avg = push_ms / max(1, push_calls)
-    achieved = BYTES_PUSH * mem.n / (push_avg * 1e-3) / 1e9
-    roof = {"kernel": "k_g2p (fused grid2par + dB/dt + RK push)", "bound": "hbm", "achieved": round(achieved, 1),
+    achieved = BYTES_PUSH * mem.n / (push_avg * 1e-3) / 1e9 if push_avg > 0 else 0.0
+    roof = {"kernel": "k_g2p_tiles (fused grid2par + dB/dt + RK push)", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_launch_ms": round(push_avg, 4), "bytes_per_parcel": BYTES_PUSH}
     line = {
@@ -123,6 +125,15 @@ def main():
         "phases_ms_per_step": phases,
         "roofline": roof,
     }
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("bench: copy calibration: 3 copies of 1 GiB")
     if not args.no_cpu:
         cb, cdt = cpu_baseline(mem)
         line["cpu_baseline"] = cb

@@ -58,7 +58,42 @@ def build(verbose=False, jobs=5):
     tmp = LIB + ".tmp"
     subprocess.run([cc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs], check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_DRIVER = os.path.join(BUILD, "drive_beltrami")
+
+
+def flang():
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose=False):
+    """Compiles the ISO_C_BINDING module and its example driver (linked to
+    libepic_gpu.so).  Skipped, with a message, where no Fortran compiler is
+    installed: the library itself does not need it."""
+    fc = flang()
+    if fc is None:
+        print("build: no Fortran compiler, Fortran binding not built", flush=True)
+        return None
+    out = os.path.join(BUILD, "fortran")
+    os.makedirs(out, exist_ok=True)
+    mod_obj = os.path.join(out, "epic_gpu_mod.o")
+    cmds = [
+        [fc, "-O2", "-fPIC", "-J", out, "-c", os.path.join(FORTRAN, "epic_gpu_mod.f90"), "-o", mod_obj],
+        [fc, "-O2", "-J", out, os.path.join(FORTRAN, "drive_beltrami.f90"), mod_obj, "-L", HERE, "-lepic_gpu",
+         "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_DRIVER],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return FORTRAN_DRIVER
 
 
 if __name__ == "__main__":

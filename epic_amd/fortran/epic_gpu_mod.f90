@@ -1,0 +1,218 @@
+! =============================================================================
+! Fortran binding of libepic_gpu.so (ISO_C_BINDING).
+!
+! A Fortran driver replaces the reference's per-timestep core with these
+! procedures; the names and argument meanings follow the reference:
+!   par2grid(l_reuse)       parcel_interpl.f90:146
+!   grid2par(add)           parcel_interpl.f90:405
+!   vor2vel()               inversion.f90:25
+!   vorticity_tendency()    inversion.f90:321
+!   ls_rk_step(t)           ls_rk.f90:88   (fixed time step)
+! Parcel and field arrays keep the reference's layouts: position(3, n),
+! vorticity(3, n), B(5, n), volume(n), ... and f(-1:nz+1, -1:ny+1, -1:nx+1).
+! A non-zero status from the library stops the run with its message, as the
+! reference's mpi_exit_on_error does (mpi_utils.f90:15-33).
+! =============================================================================
+module epic_gpu_mod
+    use iso_c_binding
+    implicit none
+
+    private
+
+    type, bind(C) :: epic_grid_t
+        integer(c_int) :: nx, ny, nz
+        real(c_double) :: lower(3), extent(3)
+    end type epic_grid_t
+
+    type, bind(C) :: epic_phys_t
+        real(c_double) :: glat, lambda_c, q0
+        real(c_double) :: f_cor(3)
+    end type epic_phys_t
+
+    type, bind(C) :: epic_flags_t
+        integer(c_int) :: zeta_bndry_zero(2)
+        integer(c_int) :: rk_order
+        real(c_double) :: vmin
+    end type epic_flags_t
+
+    integer(c_int), parameter, public :: EPIC_FIELD_VOLG = 0, EPIC_FIELD_VORTG = 1, EPIC_FIELD_TBUOYG = 2, &
+                                         EPIC_FIELD_DBUOYG = 3, EPIC_FIELD_HUMG = 4, EPIC_FIELD_VELOG = 5, &
+                                         EPIC_FIELD_VELGRADG = 6, EPIC_FIELD_VTEND = 7
+
+    interface
+        integer(c_int) function c_epic_create(grid, phys, flags, max_parcels, h) bind(C, name="epic_create")
+            import :: c_int, c_int64_t, c_ptr, epic_grid_t, epic_phys_t, epic_flags_t
+            type(epic_grid_t), intent(in)  :: grid
+            type(epic_phys_t), intent(in)  :: phys
+            type(epic_flags_t), intent(in) :: flags
+            integer(c_int64_t), value      :: max_parcels
+            type(c_ptr), intent(out)       :: h
+        end function
+        integer(c_int) function c_epic_destroy(h) bind(C, name="epic_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        type(c_ptr) function c_epic_last_error(h) bind(C, name="epic_last_error")
+            import :: c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function c_epic_set_parcels(h, n, pos, vor, B, vol, buoy, hum) bind(C, name="epic_set_parcels")
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value        :: h
+            integer(c_int64_t), value :: n
+            real(c_double), intent(in) :: pos(*), vor(*), B(*), vol(*), buoy(*), hum(*)
+        end function
+        integer(c_int) function c_epic_get_parcels(h, n, pos, vor, B, vol, buoy, hum) bind(C, name="epic_get_parcels")
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value        :: h
+            integer(c_int64_t), value :: n
+            real(c_double), intent(out) :: pos(*), vor(*), B(*), vol(*), buoy(*), hum(*)
+        end function
+        integer(c_int) function c_epic_get_field(h, which, out) bind(C, name="epic_get_field")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: h
+            integer(c_int), value :: which
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function c_epic_par2grid(h, l_reuse) bind(C, name="epic_par2grid")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: h
+            integer(c_int), value :: l_reuse
+        end function
+        integer(c_int) function c_epic_grid2par(h, add) bind(C, name="epic_grid2par")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: h
+            integer(c_int), value :: add
+        end function
+        integer(c_int) function c_epic_vor2vel(h) bind(C, name="epic_vor2vel")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function c_epic_vorticity_tendency(h) bind(C, name="epic_vorticity_tendency")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function c_epic_ls_rk_step(h, t, dt, t_initial) bind(C, name="epic_ls_rk_step")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value          :: h
+            real(c_double), intent(inout) :: t
+            real(c_double), value       :: dt, t_initial
+        end function
+        integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+            import :: c_size_t, c_ptr
+            type(c_ptr), value :: s
+        end function
+    end interface
+
+    ! the member this run works on (the reference's module globals)
+    type(c_ptr) :: handle = c_null_ptr
+    integer(c_int64_t) :: num_parcels = 0
+    double precision :: fixed_dt = 0.0d0, time_initial = 0.0d0
+
+    public :: epic_gpu_init, epic_gpu_finalise, epic_gpu_upload_parcels, epic_gpu_download_parcels, &
+              epic_gpu_get_field, par2grid, grid2par, vor2vel, vorticity_tendency, ls_rk_step
+
+contains
+
+    subroutine check(rc, where)
+        integer(c_int), intent(in) :: rc
+        character(*), intent(in)   :: where
+        character(kind=c_char), pointer :: msg(:)
+        type(c_ptr) :: p
+        integer :: n
+        if (rc == 0) return
+        p = c_epic_last_error(handle)
+        n = int(c_strlen(p))
+        call c_f_pointer(p, msg, [n])
+        write(*, *) "epic_gpu: ", where, " failed (", rc, "): ", msg(1:n)
+        error stop 1
+    end subroutine check
+
+    ! epic_create: field_alloc + parcels%allocate + init_inversion + ls_rk_setup
+    subroutine epic_gpu_init(nx, ny, nz, lower, extent, glat, lambda_c, q0, f_cor, zeta_zero, rk_order, &
+                             max_parcels, dt, t_initial)
+        integer, intent(in)          :: nx, ny, nz, rk_order
+        double precision, intent(in) :: lower(3), extent(3), glat, lambda_c, q0, f_cor(3), dt, t_initial
+        logical, intent(in)          :: zeta_zero(2)
+        integer(c_int64_t), intent(in) :: max_parcels
+        type(epic_grid_t)  :: g
+        type(epic_phys_t)  :: p
+        type(epic_flags_t) :: f
+        g%nx = nx
+        g%ny = ny
+        g%nz = nz
+        g%lower = lower
+        g%extent = extent
+        p%glat = glat
+        p%lambda_c = lambda_c
+        p%q0 = q0
+        p%f_cor = f_cor
+        f%zeta_bndry_zero = merge(1, 0, zeta_zero)
+        f%rk_order = rk_order
+        f%vmin = 0.0d0
+        call check(c_epic_create(g, p, f, max_parcels, handle), "epic_create")
+        fixed_dt = dt
+        time_initial = t_initial
+    end subroutine epic_gpu_init
+
+    subroutine epic_gpu_finalise
+        integer(c_int) :: rc
+        rc = c_epic_destroy(handle)
+        handle = c_null_ptr
+    end subroutine epic_gpu_finalise
+
+    subroutine epic_gpu_upload_parcels(n, position, vorticity, B, volume, buoyancy, humidity)
+        integer, intent(in)          :: n
+        double precision, intent(in) :: position(3, n), vorticity(3, n), B(5, n), volume(n), buoyancy(n), humidity(n)
+        num_parcels = int(n, c_int64_t)
+        call check(c_epic_set_parcels(handle, num_parcels, position, vorticity, B, volume, buoyancy, humidity), &
+                   "epic_set_parcels")
+    end subroutine epic_gpu_upload_parcels
+
+    subroutine epic_gpu_download_parcels(n, position, vorticity, B, volume, buoyancy, humidity)
+        integer, intent(in)           :: n
+        double precision, intent(out) :: position(3, n), vorticity(3, n), B(5, n), volume(n), buoyancy(n), humidity(n)
+        call check(c_epic_get_parcels(handle, int(n, c_int64_t), position, vorticity, B, volume, buoyancy, humidity), &
+                   "epic_get_parcels")
+    end subroutine epic_gpu_download_parcels
+
+    ! f(-1:nz+1, -1:ny+1, -1:nx+1[, ncomp])
+    subroutine epic_gpu_get_field(which, f)
+        integer(c_int), intent(in)    :: which
+        double precision, intent(out) :: f(*)
+        call check(c_epic_get_field(handle, which, f), "epic_get_field")
+    end subroutine epic_gpu_get_field
+
+    subroutine par2grid(l_reuse)
+        logical, optional :: l_reuse
+        integer(c_int) :: flag
+        flag = -1
+        if (present(l_reuse)) flag = merge(1, 0, l_reuse)
+        call check(c_epic_par2grid(handle, flag), "par2grid")
+    end subroutine par2grid
+
+    subroutine grid2par(add)
+        logical, optional, intent(in) :: add
+        integer(c_int) :: flag
+        flag = -1
+        if (present(add)) flag = merge(1, 0, add)
+        call check(c_epic_grid2par(handle, flag), "grid2par")
+    end subroutine grid2par
+
+    subroutine vor2vel
+        call check(c_epic_vor2vel(handle), "vor2vel")
+    end subroutine vor2vel
+
+    subroutine vorticity_tendency
+        call check(c_epic_vorticity_tendency(handle), "vorticity_tendency")
+    end subroutine vorticity_tendency
+
+    subroutine ls_rk_step(t)
+        double precision, intent(inout) :: t
+        real(c_double) :: tt
+        tt = t
+        call check(c_epic_ls_rk_step(handle, tt, fixed_dt, time_initial), "ls_rk_step")
+        t = tt
+    end subroutine ls_rk_step
+
+end module epic_gpu_mod

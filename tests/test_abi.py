@@ -36,3 +36,12 @@ def test_create_rejects_bad_grid_without_touching_the_gpu():
     g = core.Grid(16, 16, 16, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
     f = core.Flags((C.c_int * 2)(0, 0), 2, 0.0)   # unsupported RK order
     assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 10, C.byref(h)) == -6
+
+
+def test_fortran_binding_names_declared():
+    """Every C function the Fortran module binds is declared in the header."""
+    import re
+
+    src = open(os.path.join(os.path.dirname(core.LIB_PATH), "fortran", "epic_gpu_mod.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="([a-z0-9_]+)"\)', src)) - {"strlen"}
+    assert bound and bound <= set(core.header_symbols())
