@@ -85,6 +85,13 @@ def load_library(path=LIB_PATH):
         "epic_num_parcels": ([h], i64),
         "epic_set_profiling": ([h, i], i),
         "epic_get_timers": ([h, dp, ip, i], i),
+        "epic_member_bytes": ([P(Grid), i64], i64),
+        "epic_autotune": ([h, ip, ip], i),
+        "epic_set_tiling": ([h, i, i], i),
+        "epic_batch_create": ([P(h), i, P(h)], i),
+        "epic_batch_destroy": ([h], i),
+        "epic_batch_last_error": ([h], C.c_char_p),
+        "epic_batch_step": ([h, dp, dp, dp, ip], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -230,3 +237,59 @@ class Member:
         ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))   # noqa: E731
         self._chk(L.epic_cell_sort(self._h, ip(loca), ip(nppc), ip(kc1), ip(node)))
         return loca, nppc, kc1, node
+
+    # ---- tiling ----
+    def autotune(self):
+        """Times the tile z chunks on this member; returns (tz_p2g, tz_g2p)."""
+        a, b = C.c_int(0), C.c_int(0)
+        self._chk(load_library().epic_autotune(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_tiling(self, tz_p2g, tz_g2p):
+        self._chk(load_library().epic_set_tiling(self._h, int(tz_p2g), int(tz_g2p)))
+
+
+def member_bytes(nx, ny, nz, max_parcels):
+    """Device bytes a member of this grid and capacity holds."""
+    g = Grid(nx, ny, nz, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
+    return int(load_library().epic_member_bytes(C.byref(g), int(max_parcels)))
+
+
+class Batch:
+    """Many members advanced by one batched LS-RK step (one launch per kernel
+    over all members, one captured HIP graph per step)."""
+
+    def __init__(self, members):
+        L = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        self._h = C.c_void_p()
+        rc = L.epic_batch_create(arr, len(self.members), C.byref(self._h))
+        if rc != 0:
+            raise EpicError(rc, "epic_batch_create failed")
+
+    def step(self, t, dt, t_initial=None):
+        """One step of every member; t, dt, t_initial are per-member sequences.
+        Returns (new t array, status array)."""
+        nm = len(self.members)
+        tt = np.ascontiguousarray(t, dtype=np.float64).copy()
+        dd = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (nm,)))
+        ti = np.zeros(nm) if t_initial is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(t_initial, dtype=np.float64), (nm,)))
+        st = np.zeros(nm, dtype=np.int32)
+        L = load_library()
+        rc = L.epic_batch_step(self._h, _dp(tt), _dp(dd), _dp(ti), st.ctypes.data_as(C.POINTER(C.c_int)))
+        if rc != 0:
+            raise EpicError(rc, (L.epic_batch_last_error(self._h) or b"").decode())
+        return tt, st
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load_library().epic_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -15,6 +15,7 @@
 #include "common.h"
 #include "fft.h"
 #include "launch.h"
+#include "plan.h"
 
 using namespace epic;
 
@@ -45,17 +46,18 @@ struct epic_member {
     float *axbuf = nullptr;            // 12 cap: two sets of half axes
     float *ax_cur[6] = {}, *ax_stale[6] = {};
     ParcelsDev pd{};
-    // sort
-    int *keys = nullptr, *count = nullptr, *scount = nullptr, *count2 = nullptr, *scount2 = nullptr;
-    int *cell_start = nullptr, *cursor = nullptr, *perm_tmp = nullptr, *perm = nullptr, *bsum = nullptr;
+    // sort: cnt[0]/scnt[0] hold the histogram of the current positions
+    // between calls; inside a step the stages alternate between the two
+    int *keys = nullptr, *cnt[2] = {}, *scnt[2] = {};
+    int *cell_start = nullptr, *cursor = nullptr, *perm = nullptr, *bsum = nullptr;
     bool keys_valid = false;
+    int tz_p2g = 16, tz_g2p = 16;   // z chunk of the tile kernels (autotuned)
     // fields
     float *fld = nullptr;
     int *nparg = nullptr, *nsparg = nullptr;
     float *staging = nullptr;
     // spectra
     float2 *spec[NSPEC] = {};
-    float2 *T = nullptr;
     float2 *twx = nullptr, *twy = nullptr;
     double2 *twz = nullptr;
     FftPlan px{}, py{}, pz{};
@@ -65,10 +67,28 @@ struct epic_member {
     std::vector<int> ev_phase;
     double timers_ms[EPIC_NUM_TIMERS] = {};
     int timer_calls[EPIC_NUM_TIMERS] = {};
-    // errors
+    // errors; flags_d[0] = reuse the cached axes in the first par2grid of a step
     int *err_d = nullptr;
     int *err_h = nullptr;
+    int *flags_d = nullptr;
     std::string msg;
+    // launch plans of the call surface (built on first use)
+    Plan pl_keys, pl_p2g_fresh, pl_p2g_reuse, pl_vor2vel, pl_vtend, pl_g2p[2], pl_step;
+    double step_dt = -1.0;
+    uint64_t gen = 0;   // bumped whenever plans built over this member go stale
+    void invalidate_plans()
+    {
+        ++gen;
+        pl_keys.release();
+        pl_p2g_fresh.release();
+        pl_p2g_reuse.release();
+        pl_vor2vel.release();
+        pl_vtend.release();
+        pl_g2p[0].release();
+        pl_g2p[1].release();
+        pl_step.release();
+        step_dt = -1.0;
+    }
 };
 
 namespace {
@@ -134,6 +154,17 @@ void bind_parcels(epic_member *m)
 
 void collect_timers(epic_member *m);
 
+// maps the device error bits to a status code and message
+int error_code(epic_member *m, int e, const char *where)
+{
+    if (e == 0) return EPIC_OK;
+    std::string w(where);
+    if (e & ERR_PARCEL_COUNT) return fail(m, EPIC_E_PARCEL_COUNT, w + ": par2grid: Wrong total number of parcels!");
+    if (e & ERR_B33) return fail(m, EPIC_E_B33, w + ": get_B33: Division by small number!");
+    if (e & ERR_OUT_OF_GRID) return fail(m, EPIC_E_OUT_OF_GRID, w + ": support point outside the node grid");
+    return fail(m, EPIC_E_INVALID, w + ": internal consistency error");
+}
+
 // drain the device error word; maps the first error bit to a code
 int check(epic_member *m, const char *where)
 {
@@ -145,16 +176,28 @@ int check(epic_member *m, const char *where)
     if (e == 0) return EPIC_OK;
     HIPOK(m, hipMemsetAsync(m->err_d, 0, sizeof(int), m->s));
     HIPOK(m, hipStreamSynchronize(m->s));
-    std::string w(where);
-    if (e & ERR_PARCEL_COUNT) return fail(m, EPIC_E_PARCEL_COUNT, w + ": par2grid: Wrong total number of parcels!");
-    if (e & ERR_B33) return fail(m, EPIC_E_B33, w + ": get_B33: Division by small number!");
-    if (e & ERR_OUT_OF_GRID) return fail(m, EPIC_E_OUT_OF_GRID, w + ": support point outside the node grid");
-    return fail(m, EPIC_E_INVALID, w + ": internal consistency error");
+    return error_code(m, e, where);
 }
 
-__global__ void k_fill_int(int *p, int64_t n, int v)
+__global__ __launch_bounds__(256) void k_fill_int(const FillArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+    const ItemBlock ib = item_block(bmap);
+    const FillArgs &a = tab[ib.item];
+#pragma unroll
+    for (int r = 0; r < PER_BLOCK / 256; ++r) {
+        const int64_t i = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
+        if (i < a.n) a.p[i] = a.v;
+    }
+}
+
+__global__ void k_gather_errors(const ErrArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+{
+    const ItemBlock ib = item_block(bmap);
+    if (threadIdx.x == 0) {
+        const ErrArgs &a = tab[ib.item];
+        *a.out = *a.err;
+        *a.err = 0;
+    }
 }
 
 // nearest-cell key of parcel_nearest.f90:168-221 (single rank)
@@ -188,36 +231,27 @@ __global__ void k_nearest_keys(Grid g, ParcelsDev p, int *keys, int *count, int6
 
 int grid_blocks(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
 
-// phase timers: record a (start, stop) event pair around a phase
-struct PhaseTimer {
-    epic_member *m;
-    int phase;
-    hipEvent_t stop = nullptr;
-    PhaseTimer(epic_member *mm, int ph) : m(mm), phase(ph)
-    {
-        if (!m->profiling) return;
-        hipEvent_t a, b;
-        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
-        (void)hipEventRecord(a, m->s);
-        m->ev.push_back(a);
-        m->ev.push_back(b);
-        m->ev_phase.push_back(ph);
-        stop = b;
-    }
-    ~PhaseTimer()
-    {
-        if (stop) (void)hipEventRecord(stop, m->s);
-    }
-};
-
-// after a stream synchronisation: fold the recorded pairs into the totals
+// after a stream synchronisation: fold the recorded event pairs into the
+// public timers (a par2grid is its cell sort plus the tile scatter)
 void collect_timers(epic_member *m)
 {
     for (size_t k = 0; k < m->ev_phase.size(); ++k) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, m->ev[2 * k], m->ev[2 * k + 1]) == hipSuccess) {
-            m->timers_ms[m->ev_phase[k]] += ms;
-            m->timer_calls[m->ev_phase[k]] += 1;
+            switch (m->ev_phase[k]) {
+            case PH_AXES: m->timers_ms[EPIC_TIMER_AXES] += ms; m->timer_calls[EPIC_TIMER_AXES] += 1; break;
+            case PH_SORT:
+                m->timers_ms[EPIC_TIMER_CELL_SORT] += ms;
+                m->timer_calls[EPIC_TIMER_CELL_SORT] += 1;
+                m->timers_ms[EPIC_TIMER_PAR2GRID] += ms;
+                break;
+            case PH_P2G: m->timers_ms[EPIC_TIMER_PAR2GRID] += ms; m->timer_calls[EPIC_TIMER_PAR2GRID] += 1; break;
+            case PH_VOR2VEL: m->timers_ms[EPIC_TIMER_VOR2VEL] += ms; m->timer_calls[EPIC_TIMER_VOR2VEL] += 1; break;
+            case PH_VTEND: m->timers_ms[EPIC_TIMER_VTEND] += ms; m->timer_calls[EPIC_TIMER_VTEND] += 1; break;
+            case PH_PUSH: m->timers_ms[EPIC_TIMER_PUSH] += ms; m->timer_calls[EPIC_TIMER_PUSH] += 1; break;
+            case PH_GRID2PAR: m->timers_ms[EPIC_TIMER_GRID2PAR] += ms; m->timer_calls[EPIC_TIMER_GRID2PAR] += 1; break;
+            default: break;
+            }
         }
         (void)hipEventDestroy(m->ev[2 * k]);
         (void)hipEventDestroy(m->ev[2 * k + 1]);
@@ -227,45 +261,116 @@ void collect_timers(epic_member *m)
 }
 
 // ---------------------------------------------------------------------------
-// orchestration
+// plan builders over a set of members (one member: the plain call surface;
+// many: the batched step).  Every builder appends grouped launches.
 // ---------------------------------------------------------------------------
-void ensure_keys(epic_member *m)
+using Members = std::vector<epic_member *>;
+
+P2GLayout layout_tz(const Grid &g, int tz)
 {
-    if (m->keys_valid) return;
-    const int64_t nc = ncells(m->g);
-    launch_fill_int(m->count, nc, 0, m->s);
-    launch_fill_int(m->scount, nc, 0, m->s);
-    launch_cell_keys(m->g, m->pd, m->keys, m->count, m->scount, m->err_d, m->s);
-    m->keys_valid = true;
+    P2GLayout L = p2g_layout(g);
+    L.tz = tz;
+    L.nchz = (g.nz + 2 + tz - 1) / tz;
+    L.staging_floats = (int64_t)L.ntx * L.nty * L.nchz * (tz + 3) * P2G_BOX * P2G_BOX * P2G_NF;
+    return L;
 }
 
-// cell sort: histogram -> offsets -> placement (order inside a cell is
-// arbitrary here; par2grid orders each cell by caller index in LDS)
-void run_sort(epic_member *m)
+// z chunks the autotuner may choose; the staging buffer fits all of them
+const int TZ_CANDIDATES[4] = {4, 8, 16, 32};
+
+int64_t staging_floats_max(const Grid &g)
 {
-    const int64_t nc = ncells(m->g);
-    launch_exclusive_scan(m->count, nc, m->cell_start, m->bsum, m->s);
-    launch_fill_int(m->cursor, nc, 0, m->s);
-    launch_place(m->n, m->keys, m->cell_start, m->cursor, m->perm, m->s);
+    int64_t mx = 0;
+    for (int tz : TZ_CANDIDATES) mx = std::max(mx, layout_tz(g, tz).staging_floats);
+    return mx;
 }
 
-// par2grid with the given half axes (cur: this stage's, stale: the cached set)
-void run_par2grid(epic_member *m, float *const *axes)
+void build_keys(Plan &P, const Members &ms)
 {
-    PhaseTimer t(m, EPIC_TIMER_PAR2GRID);
-    {
-        PhaseTimer t2(m, EPIC_TIMER_CELL_SORT);
-        ensure_keys(m);
-        run_sort(m);
+    std::vector<FillArgs> f;
+    std::vector<KeysArgs> k;
+    for (epic_member *m : ms) {
+        const int64_t nc = ncells(m->g);
+        f.push_back(FillArgs{m->cnt[0], nc, 0});
+        f.push_back(FillArgs{m->scnt[0], nc, 0});
+        k.push_back(KeysArgs{m->g, m->pd, m->keys, m->cnt[0], m->scnt[0], m->err_d});
     }
-    launch_p2g(m->g, m->pd, axes, m->perm, m->cell_start, m->count, m->scount, m->staging, m->fld, m->nparg,
-               m->nsparg, m->err_d, m->s);
+    add_fill_int(P, PH_SORT, f);
+    add_cell_keys(P, PH_SORT, k);
 }
 
-void run_axes(epic_member *m, float *const *axes)
+void build_axes(Plan &P, const Members &ms, const std::vector<float *const *> &ax)
 {
-    PhaseTimer t(m, EPIC_TIMER_AXES);
-    launch_axes(m->g, m->pd, axes, m->err_d, m->s);
+    std::vector<AxesArgs> v;
+    for (size_t i = 0; i < ms.size(); ++i) {
+        AxesArgs a{};
+        a.p = ms[i]->pd;
+        for (int c = 0; c < 6; ++c) a.ax[c] = ax[i][c];
+        a.err = ms[i]->err_d;
+        v.push_back(a);
+    }
+    add_axes(P, PH_AXES, v);
+}
+
+// sort by the histogram rd, scatter with the axes ax (ax_alt when *use_alt),
+// finalise; clears the histogram pair clr for the next push (may be null)
+struct P2GSel {
+    float *const *ax;
+    float *const *ax_alt;
+    const int *use_alt;
+    int rd;
+    int clr;   // -1: none
+};
+
+void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
+{
+    std::vector<ScanArgs> sc;
+    std::vector<PlaceArgs> pl;
+    std::vector<P2GArgs> ti;
+    std::vector<FinalizeArgs> fi;
+    for (size_t i = 0; i < ms.size(); ++i) {
+        epic_member *m = ms[i];
+        const P2GSel &q = sel[i];
+        const int64_t nc = ncells(m->g);
+        sc.push_back(ScanArgs{m->cnt[q.rd], nc, m->cell_start, m->bsum, m->cursor});
+        pl.push_back(PlaceArgs{m->n, m->keys, m->cell_start, m->cursor, m->perm});
+        const P2GLayout L = layout_tz(m->g, m->tz_p2g);
+        P2GArgs a{};
+        a.g = m->g;
+        a.p = m->pd;
+        for (int c = 0; c < 6; ++c) {
+            a.axes[c] = q.ax[c];
+            a.axes_alt[c] = q.ax_alt ? q.ax_alt[c] : q.ax[c];
+        }
+        a.use_alt = q.use_alt;
+        a.perm = m->perm;
+        a.cell_start = m->cell_start;
+        a.staging = m->staging;
+        a.err = m->err_d;
+        a.tz = L.tz;
+        a.ntx = L.ntx;
+        a.nty = L.nty;
+        a.nchz = L.nchz;
+        ti.push_back(a);
+        FinalizeArgs f{};
+        f.g = m->g;
+        f.staging = m->staging;
+        f.count = m->cnt[q.rd];
+        f.scount = m->scnt[q.rd];
+        f.fld = m->fld;
+        f.nparg = m->nparg;
+        f.nsparg = m->nsparg;
+        f.clear_count = q.clr >= 0 ? m->cnt[q.clr] : nullptr;
+        f.clear_scount = q.clr >= 0 ? m->scnt[q.clr] : nullptr;
+        f.tz = L.tz;
+        f.ntx = L.ntx;
+        f.nty = L.nty;
+        f.nchz = L.nchz;
+        fi.push_back(f);
+    }
+    add_exclusive_scan(P, PH_SORT, sc);
+    add_place(P, PH_SORT, pl);
+    add_p2g(P, PH_P2G, ti, fi);
 }
 
 FftXArgs xargs(epic_member *m)
@@ -288,81 +393,193 @@ FftYArgs yargs(epic_member *m)
     return a;
 }
 
-void run_vor2vel(epic_member *m)
+// vor2vel + vel2vgrad: 3 forward transforms, the column solve, 11 inverse
+// transforms, the physical-space completion
+void build_vor2vel(Plan &P, const Members &ms)
 {
-    PhaseTimer t(m, EPIC_TIMER_VOR2VEL);
-    for (int c = 0; c < 3; ++c) {
-        FftXArgs x = xargs(m);
-        x.mode = 0;
-        x.slot = F_VORTG + c;
-        x.out = m->T;
-        launch_fft_xf(x, m->s);
-        FftYArgs y = yargs(m);
-        y.mode = 0;
-        y.in = m->T;
-        y.out = m->spec[c];
-        launch_fft_y(y, m->s);
-    }
-    ColArgs ca{};
-    ca.g = m->g;
-    ca.zplan = m->pz;
-    ca.ztw = m->twz;
-    for (int c = 0; c < 3; ++c) ca.in[c] = m->spec[c];
-    for (int o = 0; o < 11; ++o) ca.out[o] = m->spec[3 + o];
-    launch_vor2vel_columns(ca, m->s);
     static const int slots[11] = {F_VORTG + 0, F_VORTG + 1, F_VORTG + 2, F_VELOG + 0, F_VELOG + 1, F_VELOG + 2,
                                   F_VELGRADG + DUDX, F_VELGRADG + DUDY, F_VELGRADG + DVDY, F_VELGRADG + DWDX,
                                   F_VELGRADG + DWDY};
-    for (int o = 0; o < 11; ++o) {
-        FftYArgs y = yargs(m);
-        y.mode = 1;
-        y.in = m->spec[3 + o];
-        y.out = m->T;
-        launch_fft_y(y, m->s);
-        FftXArgs x = xargs(m);
-        x.mode = 0;
-        x.slot = slots[o];
-        x.in = m->T;
-        launch_fft_xi(x, m->s);
-    }
-    launch_vor2vel_fixup(m->g, m->fld, m->s);
-}
-
-void run_vtend(epic_member *m)
-{
-    PhaseTimer t(m, EPIC_TIMER_VTEND);
-    for (int c = 0; c < 3; ++c) {
-        for (int a = 0; a < 2; ++a) {
+    std::vector<FftXArgs> xf, xi;
+    std::vector<FftYArgs> yf, yi;
+    std::vector<ColArgs> co;
+    std::vector<FixArgs> fx;
+    for (epic_member *m : ms) {
+        for (int c = 0; c < 3; ++c) {
             FftXArgs x = xargs(m);
-            x.mode = 1;
-            x.comp_a = a;
-            x.comp_b = c;
-            // f_x of the y tendency carries -b, f_y of the x tendency +b (inversion.f90:327-335)
-            x.sgn = (a == 0 && c == 1) ? -1 : ((a == 1 && c == 0) ? 1 : 0);
-            x.out = m->T;
-            launch_fft_xf(x, m->s);
+            x.mode = 0;
+            x.slot = F_VORTG + c;
+            x.out = m->spec[c];
+            xf.push_back(x);
             FftYArgs y = yargs(m);
             y.mode = 0;
-            y.in = m->T;
-            y.out = m->spec[2 * c + a];
-            launch_fft_y(y, m->s);
+            y.in = m->spec[c];
+            y.out = m->spec[c];
+            yf.push_back(y);
         }
+        ColArgs ca{};
+        ca.g = m->g;
+        ca.zplan = m->pz;
+        ca.ztw = m->twz;
+        for (int c = 0; c < 3; ++c) ca.in[c] = m->spec[c];
+        for (int o = 0; o < 11; ++o) ca.out[o] = m->spec[3 + o];
+        co.push_back(ca);
+        for (int o = 0; o < 11; ++o) {
+            FftYArgs y = yargs(m);
+            y.mode = 1;
+            y.in = m->spec[3 + o];
+            y.out = m->spec[3 + o];
+            yi.push_back(y);
+            FftXArgs x = xargs(m);
+            x.mode = 0;
+            x.slot = slots[o];
+            x.in = m->spec[3 + o];
+            xi.push_back(x);
+        }
+        fx.push_back(FixArgs{m->g, m->fld});
     }
-    for (int c = 0; c < 3; ++c) {
-        FftYArgs y = yargs(m);
-        y.mode = 2;
-        y.in = m->spec[2 * c];
-        y.in2 = m->spec[2 * c + 1];
-        y.out = m->T;
-        launch_fft_y(y, m->s);
-        FftXArgs x = xargs(m);
-        x.mode = 2;
-        x.comp_b = c;
-        x.slot = F_VTEND + c;
-        x.in = m->T;
-        launch_fft_xi(x, m->s);
+    add_fft_xf(P, PH_VOR2VEL, xf);
+    add_fft_y(P, PH_VOR2VEL, yf);
+    add_vor2vel_columns(P, PH_VOR2VEL, co);
+    add_fft_y(P, PH_VOR2VEL, yi);
+    add_fft_xi(P, PH_VOR2VEL, xi);
+    add_vor2vel_fixup(P, PH_VOR2VEL, fx);
+}
+
+// vorticity_tendency: six flux transforms, two combined inverse transforms
+// per component (the z derivative is added in the last pass)
+void build_vtend(Plan &P, const Members &ms)
+{
+    std::vector<FftXArgs> xf, xi;
+    std::vector<FftYArgs> yf, yi;
+    std::vector<FixArgs> fx;
+    for (epic_member *m : ms) {
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 2; ++a) {
+                FftXArgs x = xargs(m);
+                x.mode = 1;
+                x.comp_a = a;
+                x.comp_b = c;
+                // f_x of the y tendency carries -b, f_y of the x tendency +b (inversion.f90:327-335)
+                x.sgn = (a == 0 && c == 1) ? -1 : ((a == 1 && c == 0) ? 1 : 0);
+                x.out = m->spec[2 * c + a];
+                xf.push_back(x);
+                FftYArgs y = yargs(m);
+                y.mode = 0;
+                y.in = m->spec[2 * c + a];
+                y.out = m->spec[2 * c + a];
+                yf.push_back(y);
+            }
+        for (int c = 0; c < 3; ++c) {
+            FftYArgs y = yargs(m);
+            y.mode = 2;
+            y.in = m->spec[2 * c];
+            y.in2 = m->spec[2 * c + 1];
+            y.out = m->spec[2 * c];
+            yi.push_back(y);
+            FftXArgs x = xargs(m);
+            x.mode = 2;
+            x.comp_b = c;
+            x.slot = F_VTEND + c;
+            x.in = m->spec[2 * c];
+            xi.push_back(x);
+        }
+        fx.push_back(FixArgs{m->g, m->fld});
     }
-    launch_vtend_fixup(m->g, m->fld, m->s);
+    add_fft_xf(P, PH_VTEND, xf);
+    add_fft_y(P, PH_VTEND, yf);
+    add_fft_y(P, PH_VTEND, yi);
+    add_fft_xi(P, PH_VTEND, xi);
+    add_vtend_fixup(P, PH_VTEND, fx);
+}
+
+// one LS-RK step (ls_rk.f90:88-136) of every member.  Histograms: stage s
+// sorts by cnt[(s-1)&1] and its push fills cnt[s&1], the last push cnt[0].
+// Half axes: stages 1..ns-1 use ax_cur, the last stage ax_stale, which is
+// then the cache the next step's first par2grid takes when *reuse is set.
+struct StepItem {
+    epic_member *m;
+    double dt;
+    const int *reuse;
+};
+
+void build_step(Plan &P, const std::vector<StepItem> &items)
+{
+    int maxns = 0;
+    for (const StepItem &it : items) maxns = std::max(maxns, it.m->rk_order == 4 ? 5 : 3);
+    for (int s = 1; s <= maxns; ++s) {
+        Members ms;
+        std::vector<const StepItem *> act;
+        for (const StepItem &it : items)
+            if ((it.m->rk_order == 4 ? 5 : 3) >= s) {
+                ms.push_back(it.m);
+                act.push_back(&it);
+            }
+        std::vector<float *const *> ax;
+        std::vector<P2GSel> sel;
+        for (const StepItem *it : act) {
+            epic_member *m = it->m;
+            const int ns = (m->rk_order == 4) ? 5 : 3;
+            const bool last = (s == ns);
+            float *const *a = last ? m->ax_stale : m->ax_cur;
+            ax.push_back(a);
+            const int rd = (s - 1) & 1, wr = last ? 0 : (s & 1);
+            sel.push_back(P2GSel{a, m->ax_stale, s == 1 ? it->reuse : nullptr, rd, wr});
+        }
+        build_axes(P, ms, ax);
+        build_p2g(P, ms, sel);
+        build_vor2vel(P, ms);
+        build_vtend(P, ms);
+        std::vector<G2PTileArgs> gv;
+        for (size_t i = 0; i < act.size(); ++i) {
+            epic_member *m = act[i]->m;
+            const int ns = (m->rk_order == 4) ? 5 : 3;
+            const double *ca = (m->rk_order == 4) ? CAS4 : CAS3;
+            const double *cb = (m->rk_order == 4) ? CBS4 : CBS3;
+            const bool last = (s == ns);
+            const int wr = last ? 0 : (s & 1);
+            const P2GLayout L = layout_tz(m->g, m->tz_g2p);
+            G2PTileArgs a{};
+            a.g = m->g;
+            a.p = m->pd;
+            for (int c = 0; c < 6; ++c) a.axes[c] = ax[i][c];
+            a.perm = m->perm;
+            a.cell_start = m->cell_start;
+            a.fld = m->fld;
+            a.stage = s;
+            a.nstages = ns;
+            a.reflect = last ? 1 : 0;
+            a.cbdt = cb[s - 1] * act[i]->dt;
+            a.ca = ca[s - 1];
+            a.keys = m->keys;
+            a.count = m->cnt[wr];
+            a.scount = m->scnt[wr];
+            a.err = m->err_d;
+            a.tz = L.tz;
+            a.ntx = L.ntx;
+            a.nty = L.nty;
+            a.nchz = L.nchz;
+            gv.push_back(a);
+        }
+        add_rk_stage_tiles(P, PH_PUSH, gv);
+    }
+}
+
+// enqueue a member plan on the member stream (event timers when profiling)
+int run_plan(epic_member *m, Plan &P)
+{
+    hipError_t e = m->profiling ? P.run(m->s, &m->ev, &m->ev_phase) : P.run(m->s);
+    if (e != hipSuccess) return fail(m, EPIC_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+    return EPIC_OK;
+}
+
+int ensure_keys(epic_member *m)
+{
+    if (m->keys_valid) return EPIC_OK;
+    if (m->pl_keys.empty()) build_keys(m->pl_keys, {m});
+    const int rc = run_plan(m, m->pl_keys);
+    if (rc == EPIC_OK) m->keys_valid = true;
+    return rc;
 }
 
 int field_ncomp(int which)
@@ -419,10 +636,16 @@ template <typename T> int copy_field_out(epic_member *m, const T *dev, int ncomp
 }  // namespace
 
 namespace epic {
-void launch_fill_int(int *p, int64_t n, int v, hipStream_t s)
+void add_fill_int(Plan &P, int phase, const std::vector<FillArgs> &items)
 {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(n)), dim3(256), 0, s, p, n, v);
+    std::vector<int> nb;
+    for (const FillArgs &a : items) nb.push_back((int)((a.n + PER_BLOCK - 1) / PER_BLOCK));
+    P.add((const void *)k_fill_int, 256, 0, phase, items, nb);
+}
+
+void add_gather_errors(Plan &P, const std::vector<ErrArgs> &items)
+{
+    P.add((const void *)k_gather_errors, 64, 0, PH_OTHER, items, std::vector<int>(items.size(), 1));
 }
 }  // namespace epic
 
@@ -483,19 +706,18 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     if (bad(hipStreamCreate(&m->s))) return EPIC_E_HIP;
     const int64_t c = m->cap, nc = ncells(g);
     if (bad(dalloc(&m->pos, 3 * c)) || bad(dalloc(&m->attr, (int64_t)NATTR * c)) || bad(dalloc(&m->orig, c)) ||
-        bad(dalloc(&m->axbuf, 12 * c)) ||
-        bad(dalloc(&m->keys, c)) || bad(dalloc(&m->perm_tmp, c)) || bad(dalloc(&m->perm, c)) ||
-        bad(dalloc(&m->count, nc)) || bad(dalloc(&m->scount, nc)) || bad(dalloc(&m->count2, nc)) ||
-        bad(dalloc(&m->scount2, nc)) || bad(dalloc(&m->cell_start, nc + 1)) || bad(dalloc(&m->cursor, nc)) ||
+        bad(dalloc(&m->axbuf, 12 * c)) || bad(dalloc(&m->keys, c)) || bad(dalloc(&m->perm, c)) ||
+        bad(dalloc(&m->cnt[0], nc)) || bad(dalloc(&m->scnt[0], nc)) || bad(dalloc(&m->cnt[1], nc)) ||
+        bad(dalloc(&m->scnt[1], nc)) || bad(dalloc(&m->cell_start, nc + 1)) || bad(dalloc(&m->cursor, nc)) ||
         bad(dalloc(&m->bsum, scan_scratch_ints(std::max<int64_t>(nc, (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2))))))
         return EPIC_E_NOMEM;
     const int64_t FS = field_stride(g);
     if (bad(dalloc(&m->fld, (int64_t)F_COUNT * FS)) || bad(dalloc(&m->nparg, FS)) || bad(dalloc(&m->nsparg, FS)) ||
-        bad(dalloc(&m->staging, p2g_layout(g).staging_floats)))
+        bad(dalloc(&m->staging, staging_floats_max(g))))
         return EPIC_E_NOMEM;
     for (int i = 0; i < NSPEC; ++i)
         if (bad(dalloc(&m->spec[i], spec_size(g)))) return EPIC_E_NOMEM;
-    if (bad(dalloc(&m->T, spec_size(g))) || bad(dalloc(&m->err_d, 4))) return EPIC_E_NOMEM;
+    if (bad(dalloc(&m->err_d, 4)) || bad(dalloc(&m->flags_d, 4))) return EPIC_E_NOMEM;
     if (bad(hipHostMalloc((void **)&m->err_h, sizeof(int) * 4, hipHostMallocDefault))) return EPIC_E_NOMEM;
     // twiddles (FP64-accurate roots of unity)
     m->px = make_plan(g.nx / 2);
@@ -525,9 +747,10 @@ int epic_destroy(epic_handle m)
 {
     if (!m) return EPIC_OK;
     if (m->s) (void)hipStreamSynchronize(m->s);
-    void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->count, m->scount, m->count2, m->scount2, m->cell_start,
-                    m->cursor, m->perm_tmp, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging, m->T,
-                    m->twx, m->twy, m->twz, m->err_d};
+    m->invalidate_plans();
+    void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
+                    m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
+                    m->twx, m->twy, m->twz, m->err_d, m->flags_d};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < NSPEC; ++i)
@@ -566,6 +789,7 @@ int epic_set_parcels(epic_handle m, int64_t n, const double *position, const dou
     }
     m->n = n;
     bind_parcels(m);
+    m->invalidate_plans();   // the plans hold the parcel count
     for (int d = 0; d < 3 && n; ++d)
         HIPOK(m, hipMemcpyAsync(m->pos + d * c, hp.data() + d * n, sizeof(double) * n, hipMemcpyHostToDevice, m->s));
     for (int a = 0; a < 11 && n; ++a)
@@ -698,33 +922,53 @@ int epic_set_field(epic_handle m, int which, const double *in)
 int epic_par2grid(epic_handle m, int l_reuse)
 {
     if (!m) return EPIC_E_INVALID;
-    // get_points stores the axes it computes (parcel_ellipsoid.f90:216-247)
-    if (l_reuse != 1) run_axes(m, m->ax_stale);
-    run_par2grid(m, m->ax_stale);
+    int rc = ensure_keys(m);
+    if (rc) return rc;
+    // get_points stores the axes it computes (parcel_ellipsoid.f90:216-247);
+    // with l_reuse the stored ones are taken
+    Plan &P = (l_reuse == 1) ? m->pl_p2g_reuse : m->pl_p2g_fresh;
+    if (P.empty()) {
+        if (l_reuse != 1) build_axes(P, {m}, {m->ax_stale});
+        build_p2g(P, {m}, {P2GSel{m->ax_stale, nullptr, nullptr, 0, -1}});
+    }
+    rc = run_plan(m, P);
+    if (rc) return rc;
     return check(m, "epic_par2grid");
 }
 
 int epic_vor2vel(epic_handle m)
 {
     if (!m) return EPIC_E_INVALID;
-    run_vor2vel(m);
+    if (m->pl_vor2vel.empty()) build_vor2vel(m->pl_vor2vel, {m});
+    const int rc = run_plan(m, m->pl_vor2vel);
+    if (rc) return rc;
     return check(m, "epic_vor2vel");
 }
 
 int epic_vorticity_tendency(epic_handle m)
 {
     if (!m) return EPIC_E_INVALID;
-    run_vtend(m);
+    if (m->pl_vtend.empty()) build_vtend(m->pl_vtend, {m});
+    const int rc = run_plan(m, m->pl_vtend);
+    if (rc) return rc;
     return check(m, "epic_vorticity_tendency");
 }
 
 int epic_grid2par(epic_handle m, int add)
 {
     if (!m) return EPIC_E_INVALID;
-    {
-        PhaseTimer t(m, EPIC_TIMER_GRID2PAR);
-        launch_grid2par(m->g, m->pd, m->fld, add == 1, m->err_d, m->s);
+    Plan &P = m->pl_g2p[add == 1 ? 1 : 0];
+    if (P.empty()) {
+        G2PArgs a{};
+        a.g = m->g;
+        a.p = m->pd;
+        a.fld = m->fld;
+        a.add = (add == 1);
+        a.err = m->err_d;
+        add_grid2par(P, PH_GRID2PAR, {a});
     }
+    const int rc = run_plan(m, P);
+    if (rc) return rc;
     return check(m, "epic_grid2par");
 }
 
@@ -732,34 +976,23 @@ int epic_ls_rk_step(epic_handle m, double *t, double dt, double t_initial)
 {
     if (!m || !t) return EPIC_E_INVALID;
     if (!(dt > 0.0)) return fail(m, EPIC_E_DT, "epic_ls_rk_step: time step must be positive");
-    const double *ca = (m->rk_order == 4) ? CAS4 : CAS3;
-    const double *cb = (m->rk_order == 4) ? CBS4 : CBS3;
-    const int ns = (m->rk_order == 4) ? 5 : 3;
-    const int64_t nc = ncells(m->g);
-    for (int s = 1; s <= ns; ++s) {
-        // the first par2grid of a step reuses the cached axes (ls_rk.f90:93);
-        // grid2par and later stages use the axes of the current B
-        const bool reuse = (s == 1 && *t > t_initial);
-        run_axes(m, m->ax_cur);
-        run_par2grid(m, reuse ? m->ax_stale : m->ax_cur);
-        run_vor2vel(m);
-        run_vtend(m);
-        launch_fill_int(m->count2, nc, 0, m->s);
-        launch_fill_int(m->scount2, nc, 0, m->s);
-        {
-            PhaseTimer t(m, EPIC_TIMER_PUSH);
-            launch_rk_stage_tiles(m->g, m->pd, m->ax_cur, m->perm, m->cell_start, m->fld, s, ns, cb[s - 1] * dt,
-                                  ca[s - 1], s == ns ? 1 : 0, m->keys, m->count2, m->scount2, m->err_d, m->s);
-        }
-        std::swap(m->count, m->count2);
-        std::swap(m->scount, m->scount2);
-        m->keys_valid = true;
+    int rc = ensure_keys(m);
+    if (rc) return rc;
+    if (m->pl_step.empty() || m->step_dt != dt) {
+        m->pl_step.release();
+        build_step(m->pl_step, {StepItem{m, dt, m->flags_d}});
+        m->step_dt = dt;
     }
-    // the axes of the last stage become the cache of the next step's first
-    // par2grid (grid2par stores them, parcel_interpl.f90:442)
-    for (int i = 0; i < 6; ++i) std::swap(m->ax_cur[i], m->ax_stale[i]);
-    bind_parcels(m);
-    const int rc = check(m, "epic_ls_rk_step");
+    // the first par2grid of a step reuses the cached axes (ls_rk.f90:93)
+    m->err_h[1] = (*t > t_initial) ? 1 : 0;
+    HIPOK(m, hipMemcpyAsync(m->flags_d, m->err_h + 1, sizeof(int), hipMemcpyHostToDevice, m->s));
+    if (m->profiling) {
+        rc = run_plan(m, m->pl_step);
+        if (rc) return rc;
+    } else {
+        HIPOK(m, m->pl_step.run_graph(m->s));
+    }
+    rc = check(m, "epic_ls_rk_step");
     if (rc == EPIC_OK) *t = *t + dt;
     return rc;
 }
@@ -785,6 +1018,266 @@ int epic_get_timers(epic_handle m, double *ms, int *calls, int reset)
     return EPIC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// batched step, autotuner, memory estimate
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct epic_batch {
+    std::vector<epic_member *> ms;
+    hipStream_t s = nullptr;
+    int *flags_d = nullptr, *errs_d = nullptr;
+    int *flags_h = nullptr, *errs_h = nullptr;
+    Plan plan;
+    std::vector<double> dts;
+    std::vector<uint64_t> gens;
+    std::string msg;
+};
+
+namespace {
+
+int batch_fail(epic_batch *b, int code, const std::string &what)
+{
+    if (b) b->msg = what;
+    return code;
+}
+
+#define BHIPOK(b, expr)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return batch_fail(b, EPIC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// device time of one plan replay on the member stream (median of reps)
+float time_plan(epic_member *m, Plan &P, int reps)
+{
+    std::vector<float> t;
+    for (int r = 0; r <= reps; ++r) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess) return 1e30f;
+        if (hipEventCreate(&b) != hipSuccess) return 1e30f;
+        (void)hipEventRecord(a, m->s);
+        (void)P.run(m->s);
+        (void)hipEventRecord(b, m->s);
+        (void)hipEventSynchronize(b);
+        float ms = 1e30f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (r > 0) t.push_back(ms);   // the first replay warms up
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels)
+{
+    if (!grid || grid->nx < 8 || grid->ny < 8 || grid->nz < 4 || max_parcels < 0) return -1;
+    Grid g{};
+    g.nx = grid->nx;
+    g.ny = grid->ny;
+    g.nz = grid->nz;
+    g.nxh = g.nx / 2 + 1;
+    g.nzp = ((g.nz + 1 + ZB - 1) / ZB) * ZB;
+    const int64_t c = std::max<int64_t>(max_parcels, 1), nc = ncells(g), FS = field_stride(g);
+    int64_t b = 0;
+    b += c * (3 * 8 + (int64_t)NATTR * 4 + 4 + 12 * 4 + 4 + 4);               // parcels, axes, orig, keys, perm
+    b += 4 * (4 * nc + (nc + 1) + nc);                                        // histograms, cell_start, cursor
+    b += 4 * scan_scratch_ints(std::max<int64_t>(nc, (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2)));
+    b += 4 * ((int64_t)F_COUNT * FS + 2 * FS) + 4 * staging_floats_max(g);   // fields, counts, staging
+    b += (int64_t)NSPEC * 8 * spec_size(g);                                  // spectra
+    return b;
+}
+
+int epic_autotune(epic_handle m, int *tz_p2g, int *tz_g2p)
+{
+    if (!m) return EPIC_E_INVALID;
+    int rc = ensure_keys(m);
+    if (rc) return rc;
+    {
+        Plan P;
+        build_axes(P, {m}, {m->ax_cur});
+        HIPOK(m, P.run(m->s));
+    }
+    float best_p = 1e30f, best_g = 1e30f;
+    int tp = m->tz_p2g, tg = m->tz_g2p;
+    for (int tz : TZ_CANDIDATES) {
+        if (tz > m->g.nz + 2) continue;
+        m->tz_p2g = tz;
+        Plan P;
+        build_p2g(P, {m}, {P2GSel{m->ax_cur, nullptr, nullptr, 0, -1}});
+        const float t = time_plan(m, P, 3);
+        if (t < best_p) {
+            best_p = t;
+            tp = tz;
+        }
+    }
+    m->tz_p2g = tp;
+    {   // a p2g with the chosen layout leaves perm / cell_start for the probes below
+        Plan P;
+        build_p2g(P, {m}, {P2GSel{m->ax_cur, nullptr, nullptr, 0, -1}});
+        HIPOK(m, P.run(m->s));
+    }
+    for (int tz : TZ_CANDIDATES) {
+        if (tz > m->g.nz + 2) continue;
+        // a first stage with cb dt = 0 leaves position, vorticity and B
+        // unchanged; it rewrites the stage deltas (reset by every step's first
+        // stage) and recounts the current cells into the spare histogram
+        const P2GLayout L = layout_tz(m->g, tz);
+        G2PTileArgs a{};
+        a.g = m->g;
+        a.p = m->pd;
+        for (int c = 0; c < 6; ++c) a.axes[c] = m->ax_cur[c];
+        a.perm = m->perm;
+        a.cell_start = m->cell_start;
+        a.fld = m->fld;
+        a.stage = 1;
+        a.nstages = 2;
+        a.reflect = 0;
+        a.cbdt = 0.0;
+        a.ca = 0.0;
+        a.keys = m->keys;
+        a.count = m->cnt[1];
+        a.scount = m->scnt[1];
+        a.err = m->err_d;
+        a.tz = L.tz;
+        a.ntx = L.ntx;
+        a.nty = L.nty;
+        a.nchz = L.nchz;
+        Plan P;
+        add_rk_stage_tiles(P, PH_PUSH, {a});
+        const float t = time_plan(m, P, 3);
+        if (t < best_g) {
+            best_g = t;
+            tg = tz;
+        }
+    }
+    m->tz_g2p = tg;
+    m->invalidate_plans();
+    if (tz_p2g) *tz_p2g = m->tz_p2g;
+    if (tz_g2p) *tz_g2p = m->tz_g2p;
+    return check(m, "epic_autotune");
+}
+
+int epic_set_tiling(epic_handle m, int tz_p2g, int tz_g2p)
+{
+    if (!m) return EPIC_E_INVALID;
+    bool ok_p = false, ok_g = false;
+    for (int tz : TZ_CANDIDATES) {
+        ok_p |= (tz == tz_p2g);
+        ok_g |= (tz == tz_g2p);
+    }
+    if (!ok_p || !ok_g) return fail(m, EPIC_E_INVALID, "epic_set_tiling: z chunk must be 4, 8, 16 or 32");
+    m->tz_p2g = tz_p2g;
+    m->tz_g2p = tz_g2p;
+    m->invalidate_plans();
+    return EPIC_OK;
+}
+
+int epic_batch_create(const epic_handle *members, int count, epic_batch_handle *out)
+{
+    if (!members || count <= 0 || !out) return EPIC_E_INVALID;
+    *out = nullptr;
+    epic_batch *b = new epic_batch();
+    for (int i = 0; i < count; ++i) {
+        if (!members[i]) {
+            delete b;
+            return EPIC_E_INVALID;
+        }
+        for (int j = 0; j < i; ++j)
+            if (members[j] == members[i]) {
+                delete b;
+                return EPIC_E_INVALID;
+            }
+        b->ms.push_back(members[i]);
+    }
+    auto bad = [&](hipError_t e) {
+        if (e != hipSuccess) {
+            epic_batch_destroy(b);
+            return true;
+        }
+        return false;
+    };
+    if (bad(hipStreamCreate(&b->s))) return EPIC_E_HIP;
+    if (bad(dalloc(&b->flags_d, count)) || bad(dalloc(&b->errs_d, count))) return EPIC_E_NOMEM;
+    if (bad(hipHostMalloc((void **)&b->flags_h, sizeof(int) * count, hipHostMallocDefault)) ||
+        bad(hipHostMalloc((void **)&b->errs_h, sizeof(int) * count, hipHostMallocDefault)))
+        return EPIC_E_NOMEM;
+    *out = b;
+    return EPIC_OK;
+}
+
+int epic_batch_destroy(epic_batch_handle b)
+{
+    if (!b) return EPIC_OK;
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    b->plan.release();
+    if (b->flags_d) (void)hipFree(b->flags_d);
+    if (b->errs_d) (void)hipFree(b->errs_d);
+    if (b->flags_h) (void)hipHostFree(b->flags_h);
+    if (b->errs_h) (void)hipHostFree(b->errs_h);
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+    return EPIC_OK;
+}
+
+const char *epic_batch_last_error(epic_batch_handle b) { return b ? b->msg.c_str() : "null handle"; }
+
+int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const double *t_initial, int *status)
+{
+    if (!b || !t || !dt || !t_initial) return EPIC_E_INVALID;
+    const int nm = (int)b->ms.size();
+    for (int i = 0; i < nm; ++i)
+        if (!(dt[i] > 0.0)) return batch_fail(b, EPIC_E_DT, "epic_batch_step: time step must be positive");
+    for (int i = 0; i < nm; ++i) {
+        epic_member *m = b->ms[i];
+        if (!m->keys_valid) {
+            const int rc = ensure_keys(m);
+            if (rc) return batch_fail(b, rc, m->msg);
+            BHIPOK(b, hipStreamSynchronize(m->s));
+        }
+    }
+    bool same = b->dts.size() == (size_t)nm && b->gens.size() == (size_t)nm;
+    for (int i = 0; same && i < nm; ++i) same = (b->dts[i] == dt[i]) && (b->gens[i] == b->ms[i]->gen);
+    if (b->plan.empty() || !same) {
+        b->plan.release();
+        std::vector<StepItem> items;
+        std::vector<ErrArgs> errs;
+        for (int i = 0; i < nm; ++i) {
+            items.push_back(StepItem{b->ms[i], dt[i], b->flags_d + i});
+            errs.push_back(ErrArgs{b->ms[i]->err_d, b->errs_d + i});
+        }
+        build_step(b->plan, items);
+        add_gather_errors(b->plan, errs);
+        b->dts.assign(dt, dt + nm);
+        b->gens.clear();
+        for (int i = 0; i < nm; ++i) b->gens.push_back(b->ms[i]->gen);
+    }
+    // the first par2grid of a step reuses the cached axes (ls_rk.f90:93)
+    for (int i = 0; i < nm; ++i) b->flags_h[i] = (t[i] > t_initial[i]) ? 1 : 0;
+    BHIPOK(b, hipMemcpyAsync(b->flags_d, b->flags_h, sizeof(int) * nm, hipMemcpyHostToDevice, b->s));
+    BHIPOK(b, b->plan.run_graph(b->s));
+    BHIPOK(b, hipMemcpyAsync(b->errs_h, b->errs_d, sizeof(int) * nm, hipMemcpyDeviceToHost, b->s));
+    BHIPOK(b, hipStreamSynchronize(b->s));
+    BHIPOK(b, hipGetLastError());
+    int first = EPIC_OK;
+    for (int i = 0; i < nm; ++i) {
+        epic_member *m = b->ms[i];
+        const int rc = error_code(m, b->errs_h[i], "epic_batch_step");
+        if (status) status[i] = rc;
+        if (rc == EPIC_OK) t[i] += dt[i];
+        else if (first == EPIC_OK) {
+            first = rc;
+            b->msg = m->msg;
+        }
+    }
+    return first;
+}
+
 int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)
 {
     if (!m) return EPIC_E_INVALID;
@@ -799,9 +1292,14 @@ int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)
     HIPOK(m, dalloc(&pm, n));
     hipLaunchKernelGGL(k_nearest_keys, dim3(grid_blocks(std::max<int64_t>(n, 1))), dim3(256), 0, m->s, g, m->pd, keys,
                        cnt, ncell, m->err_d);
-    launch_exclusive_scan(cnt, ncell, start, m->bsum, m->s);
-    launch_place(n, keys, start, cur, ptmp, m->s);
-    launch_cell_order(ncell, start, ptmp, m->orig, pm, m->s);
+    {
+        Plan P;
+        add_exclusive_scan(P, PH_OTHER, {ScanArgs{cnt, ncell, start, m->bsum, cur}});
+        add_place(P, PH_OTHER, {PlaceArgs{n, keys, start, cur, ptmp}});
+        HIPOK(m, P.run(m->s));
+        launch_cell_order(ncell, start, ptmp, m->orig, pm, m->s);
+        HIPOK(m, hipStreamSynchronize(m->s));
+    }
     int rc = check(m, "epic_cell_sort");
     std::vector<int> hk(n), hp(n), ho(n), hs(ncell + 1), hc(ncell);
     if (rc == EPIC_OK) {

@@ -19,14 +19,14 @@ template <typename T2> __device__ inline T2 cmuli(T2 a, bool inv) { return inv ?
 template <typename T2, typename R> __device__ inline T2 cscale(T2 a, R s) { return T2{a.x * s, a.y * s}; }
 
 // In-place R-point DFT, R in {2, 3, 4}; forward uses e^{-2 pi i/R}.
-template <typename T2> __device__ inline void butterfly(int R, T2 *v, bool inv)
+template <int R, typename T2> __device__ inline void butterfly(T2 *v, bool inv)
 {
     using Rl = typename Cx<T2>::R;
-    if (R == 2) {
+    if constexpr (R == 2) {
         const T2 a = cadd(v[0], v[1]), b = csub(v[0], v[1]);
         v[0] = a;
         v[1] = b;
-    } else if (R == 4) {
+    } else if constexpr (R == 4) {
         const T2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
         const T2 t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), inv);
         v[0] = cadd(t0, t2);
@@ -44,6 +44,34 @@ template <typename T2> __device__ inline void butterfly(int R, T2 *v, bool inv)
     }
 }
 
+// one Stockham stage of radix R over L lines of n points (registers only:
+// the radix is a compile-time constant)
+template <int R, typename T2>
+__device__ inline void fft_stage(const T2 *in, T2 *out, int L, int n, int Ns, const T2 *__restrict__ tw, int tws,
+                                 bool inv)
+{
+    const int nb = n / R;
+    const int tstep = n / (Ns * R);
+    for (int t = threadIdx.x; t < L * nb; t += blockDim.x) {
+        const int l = t / nb, jj = t - l * nb;
+        const T2 *src = in + l * n;
+        T2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = src[jj + r * nb];
+        const int kk = jj % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            T2 w = tw[(r * kk * tstep) * tws];
+            if (inv) w = cconj(w);
+            v[r] = cmul(v[r], w);
+        }
+        butterfly<R>(v, inv);
+        T2 *dst = out + l * n + (jj / Ns) * Ns * R + kk;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
+    }
+}
+
 // L lines of n points each, line l at a[l*n]; every thread of the block
 // calls this.  tw[t*tws] = e^{-2 pi i t / n}.  Returns the buffer holding
 // the result (a or b).  Ends with a barrier.
@@ -55,23 +83,9 @@ __device__ T2 *lds_fft(T2 *a, T2 *b, int L, const FftPlan &pl, const T2 *__restr
     T2 *in = a, *out = b;
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.radix[st];
-        const int nb = n / R;
-        const int tstep = n / (Ns * R);
-        for (int t = threadIdx.x; t < L * nb; t += blockDim.x) {
-            const int l = t / nb, jj = t - l * nb;
-            const T2 *src = in + l * n;
-            T2 v[4];
-            for (int r = 0; r < R; ++r) v[r] = src[jj + r * nb];
-            const int kk = jj % Ns;
-            for (int r = 1; r < R; ++r) {
-                T2 w = tw[(r * kk * tstep) * tws];
-                if (inv) w = cconj(w);
-                v[r] = cmul(v[r], w);
-            }
-            butterfly(R, v, inv);
-            T2 *dst = out + l * n + (jj / Ns) * Ns * R + kk;
-            for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
-        }
+        if (R == 4) fft_stage<4>(in, out, L, n, Ns, tw, tws, inv);
+        else if (R == 2) fft_stage<2>(in, out, L, n, Ns, tw, tws, inv);
+        else fft_stage<3>(in, out, L, n, Ns, tw, tws, inv);
         __syncthreads();
         T2 *tmp = in;
         in = out;

@@ -14,6 +14,8 @@
 #include "fft.h"
 #include "launch.h"
 
+#include <algorithm>
+
 namespace epic {
 
 // ---------------------------------------------------------------------------
@@ -31,11 +33,13 @@ __device__ inline float load_phys(const FftXArgs &a, int k, int j, int i)
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_fft_xf(FftXArgs a)
+__global__ __launch_bounds__(256) void k_fft_xf(const FftXArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const FftXArgs &a = tab[ib.item];
     const Grid &g = a.g;
-    const int j = blockIdx.x;
-    const int zp0 = blockIdx.y * ZB;
+    const int j = ib.blk % g.ny;
+    const int zp0 = (ib.blk / g.ny) * ZB;
     const int m = g.nx / 2;
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
@@ -71,11 +75,15 @@ __global__ __launch_bounds__(256) void k_fft_xf(FftXArgs a)
 // ---------------------------------------------------------------------------
 // y pass (c2c), forward or inverse
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fft_y(FftYArgs a)
+// in place is allowed (out == in): a block reads and writes only its own
+// (kx, z block) slab
+__global__ __launch_bounds__(256) void k_fft_y(const FftYArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const FftYArgs &a = tab[ib.item];
     const Grid &g = a.g;
-    const int kx = blockIdx.x;
-    const int zp0 = blockIdx.y * ZB;
+    const int kx = ib.blk % g.nxh;
+    const int zp0 = (ib.blk / g.nxh) * ZB;
     const int ny = g.ny, nxh = g.nxh;
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * ny;
@@ -107,11 +115,13 @@ __global__ __launch_bounds__(256) void k_fft_y(FftYArgs a)
 // ---------------------------------------------------------------------------
 // inverse x pass: half spectra -> real rows, with the output stage
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fft_xi(FftXArgs a)
+__global__ __launch_bounds__(256) void k_fft_xi(const FftXArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const FftXArgs &a = tab[ib.item];
     const Grid &g = a.g;
-    const int j = blockIdx.x;
-    const int zp0 = blockIdx.y * ZB;
+    const int j = ib.blk % g.ny;
+    const int zp0 = (ib.blk / g.ny) * ZB;
     const int m = g.nx / 2, nxh = m + 1;
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
@@ -165,19 +175,37 @@ __global__ __launch_bounds__(256) void k_fft_xi(FftXArgs a)
 static size_t x_lds(const Grid &g) { return 2 * ZB * (size_t)(g.nx / 2 + 1) * sizeof(float2); }
 static size_t y_lds(const Grid &g) { return 2 * ZB * (size_t)g.ny * sizeof(float2); }
 
-void launch_fft_xf(const FftXArgs &a, hipStream_t s)
+void add_fft_xf(Plan &P, int phase, const std::vector<FftXArgs> &items)
 {
-    hipLaunchKernelGGL(k_fft_xf, dim3(a.g.ny, a.g.nzp / ZB), dim3(256), x_lds(a.g), s, a);
+    std::vector<int> nb;
+    size_t lds = 0;
+    for (const FftXArgs &a : items) {
+        nb.push_back(a.g.ny * (a.g.nzp / ZB));
+        lds = std::max(lds, x_lds(a.g));
+    }
+    P.add((const void *)k_fft_xf, 256, lds, phase, items, nb);
 }
 
-void launch_fft_y(const FftYArgs &a, hipStream_t s)
+void add_fft_y(Plan &P, int phase, const std::vector<FftYArgs> &items)
 {
-    hipLaunchKernelGGL(k_fft_y, dim3(a.g.nxh, a.g.nzp / ZB), dim3(256), y_lds(a.g), s, a);
+    std::vector<int> nb;
+    size_t lds = 0;
+    for (const FftYArgs &a : items) {
+        nb.push_back(a.g.nxh * (a.g.nzp / ZB));
+        lds = std::max(lds, y_lds(a.g));
+    }
+    P.add((const void *)k_fft_y, 256, lds, phase, items, nb);
 }
 
-void launch_fft_xi(const FftXArgs &a, hipStream_t s)
+void add_fft_xi(Plan &P, int phase, const std::vector<FftXArgs> &items)
 {
-    hipLaunchKernelGGL(k_fft_xi, dim3(a.g.ny, a.g.nzp / ZB), dim3(256), x_lds(a.g), s, a);
+    std::vector<int> nb;
+    size_t lds = 0;
+    for (const FftXArgs &a : items) {
+        nb.push_back(a.g.ny * (a.g.nzp / ZB));
+        lds = std::max(lds, x_lds(a.g));
+    }
+    P.add((const void *)k_fft_xi, 256, lds, phase, items, nb);
 }
 
 }  // namespace epic

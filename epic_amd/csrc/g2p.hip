@@ -1,13 +1,13 @@
-// grid2par, dB/dt and the low-storage RK push fused into one pass over the
-// parcels (parcel_interpl.f90:405-470, parcel_ellipsoid.f90:466-507,
-// ls_rk.f90:142-217, parcel_bc.f90:17-40).
+// grid2par (parcel_interpl.f90:405-470), and grid2par fused with dB/dt and
+// the low-storage RK push (parcel_ellipsoid.f90:466-507, ls_rk.f90:142-217,
+// parcel_bc.f90:17-40) for ls_rk_step.
 //
-// Per parcel: FP64 eigen-decomposition of B -> four support points; each point
-// gathers velocity (3), velocity gradient (8) and vorticity tendency (3) from
-// the eight corner nodes with trilinear weights (FP64 cell fractions); the
-// quarter-weighted sums update delta_pos / delta_vor, the gradient gives
-// dB/dt (B33 from det B in FP64), and the stage pushes position (FP64),
-// vorticity and B by cb dt delta, applies the periodic wrap (int()
+// Per parcel: four support points from the half axes; each point gathers
+// velocity (3), velocity gradient (8) and vorticity tendency (3) from the
+// eight corner nodes with trilinear weights (FP64 cell fractions); the
+// quarter-weighted sums update delta_pos / delta_vor.  In the fused form the
+// gradient gives dB/dt (B33 from det B in FP64), the stage pushes position
+// (FP64), vorticity and B by cb dt delta, applies the periodic wrap (int()
 // truncation), scales the deltas by ca for the next stage and computes the
 // next stage's cell key and cell histogram.
 #include "common.h"
@@ -16,28 +16,21 @@
 
 namespace epic {
 
-struct G2PArgs {
-    Grid g;
-    ParcelsDev p;
-    const float *fld;      // node fields (F_VELOG, F_VELGRADG, F_VTEND used)
-    int mode;              // 0: stand-alone grid2par, 1: fused RK stage
-    int add;               // grid2par: accumulate into delta_pos/delta_vor
-    int stage;             // RK stage 1..nstages
-    int nstages;
-    int reflect;           // apply the reflective z boundary after the push
-    int store_axes;        // cache Veta/Vtau for the next par2grid(l_reuse)
-    double cbdt, ca;
-    int *keys, *count, *scount;   // next-stage keys/histogram (may be null)
-    int *err;
-};
-
-__global__ __launch_bounds__(256) void k_g2p(G2PArgs a)
+// Stand-alone grid2par(add) (parcel_interpl.f90:405-470): support points
+// from the current B (stored as the cache of the next par2grid, :442),
+// quarter-weighted trilinear gathers of velocity, velocity gradient and
+// vorticity tendency; delta_pos / delta_vor are set or accumulated, strain is
+// set.  One parcel per thread.
+__global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const G2PArgs &a = tab[ib.item];
     const Grid &g = a.g;
     const int64_t PS = plane_stride(g), FS = field_stride(g);
     int e = 0;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.p.n; q += (int64_t)gridDim.x * blockDim.x) {
-        double pos[3] = {a.p.x[q], a.p.y[q], a.p.z[q]};
+    const int64_t q = ib.blk * (int64_t)blockDim.x + threadIdx.x;
+    if (q < a.p.n) {
+        const double pos[3] = {a.p.x[q], a.p.y[q], a.p.z[q]};
         double B[5];
 #pragma unroll
         for (int c = 0; c < 5; ++c) B[c] = (double)a.p.B[c][q];
@@ -47,23 +40,19 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a)
             e |= ERR_B33;
             ve[0] = ve[1] = ve[2] = vt[0] = vt[1] = vt[2] = 0.0;
         }
-        if (a.store_axes) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                a.p.veta[d][q] = (float)ve[d];
-                a.p.vtau[d][q] = (float)vt[d];
-            }
+        for (int d = 0; d < 3; ++d) {
+            a.p.veta[d][q] = (float)ve[d];
+            a.p.vtau[d][q] = (float)vt[d];
         }
         float dp[3], dv[3], st[8];
-        const bool accumulate = (a.mode == 0) ? (a.add != 0) : (a.stage > 1);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            dp[c] = accumulate ? a.p.dpos[c][q] : 0.0f;
-            dv[c] = accumulate ? a.p.dvor[c][q] : 0.0f;
+            dp[c] = a.add ? a.p.dpos[c][q] : 0.0f;
+            dv[c] = a.add ? a.p.dvor[c][q] : 0.0f;
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c) st[c] = 0.0f;
-
         for (int jp = 0; jp < 4; ++jp) {
             const double cs = support_cos(jp), sn = support_sin(jp);
             int ii[3];
@@ -118,78 +107,13 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a)
             }
         }
 
-        if (a.mode == 0) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                a.p.dpos[c][q] = dp[c];
-                a.p.dvor[c][q] = dv[c];
-            }
-#pragma unroll
-            for (int c = 0; c < 8; ++c) a.p.strain[c][q] = st[c];
-            continue;
+        for (int c = 0; c < 3; ++c) {
+            a.p.dpos[c][q] = dp[c];
+            a.p.dvor[c][q] = dv[c];
         }
-
-        // dB/dt (FP64, B33 from the volume constraint)
-        double S[8], dB[5];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) S[c] = (double)st[c];
-        if (!dBdt(B, S, vol, dB)) {
-            e |= ERR_B33;
-            dB[0] = dB[1] = dB[2] = dB[3] = dB[4] = 0.0;
-        }
-        float db[5];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) db[c] = (a.stage > 1) ? a.p.dB[c][q] + (float)dB[c] : (float)dB[c];
-
-        // push (ls_rk.f90:183-191)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) pos[c] = pos[c] + a.cbdt * (double)dp[c];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) a.p.vor[c][q] = (float)((double)a.p.vor[c][q] + a.cbdt * (double)dv[c]);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) B[c] = B[c] + a.cbdt * (double)db[c];
-        // periodic wrap with Fortran int() truncation (parcel_bc.f90:17-21)
-        pos[0] = pos[0] - g.extent[0] * trunc((pos[0] - g.center[0]) * g.hli[0]);
-        pos[1] = pos[1] - g.extent[1] * trunc((pos[1] - g.center[1]) * g.hli[1]);
-        if (a.reflect) {   // parcel_bc.f90:26-40
-            if (pos[2] > g.upper[2]) {
-                pos[2] = 2.0 * g.upper[2] - pos[2];
-                B[2] = -B[2];
-                B[4] = -B[4];
-            } else if (pos[2] < g.lower[2]) {
-                pos[2] = 2.0 * g.lower[2] - pos[2];
-                B[2] = -B[2];
-                B[4] = -B[4];
-            }
-        }
-        a.p.x[q] = pos[0];
-        a.p.y[q] = pos[1];
-        a.p.z[q] = pos[2];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) a.p.B[c][q] = (float)B[c];
-        if (a.stage < a.nstages) {
-            const float ca = (float)a.ca;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                a.p.dpos[c][q] = ca * dp[c];
-                a.p.dvor[c][q] = ca * dv[c];
-            }
-#pragma unroll
-            for (int c = 0; c < 5; ++c) a.p.dB[c][q] = ca * db[c];
-        }
-        if (a.keys) {
-            const int i = (int)floor((pos[0] - g.lower[0]) * g.dxi[0]);
-            const int j = (int)floor((pos[1] - g.lower[1]) * g.dxi[1]);
-            const int k = (int)floor((pos[2] - g.lower[2]) * g.dxi[2]);
-            int key = -1;
-            if (k < -1 || k > g.nz) e |= ERR_PARCEL_COUNT;
-            else key = ((k + 1) * g.ny + wrap(j, g.ny)) * g.nx + wrap(i, g.nx);
-            a.keys[q] = key;
-            if (key >= 0) {
-                atomicAdd(&a.count[key], 1);
-                if (vol <= g.vmin) atomicAdd(&a.scount[key], 1);
-            }
-        }
+        for (int c = 0; c < 8; ++c) a.p.strain[c][q] = st[c];
     }
     if (e) atomicOr(a.err, e);
 }
@@ -202,29 +126,17 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a)
 // reads; parcels are read through the cell sort, support points come from
 // the half axes cached for this stage.
 // ---------------------------------------------------------------------------
-struct G2PTileArgs {
-    Grid g;
-    ParcelsDev p;
-    const float *axes[6];
-    const int *perm;
-    const int *cell_start;
-    const float *fld;
-    int stage, nstages, reflect;
-    double cbdt, ca;
-    int *keys, *count, *scount;
-    int *err;
-    int tz, ntx, nty, nchz;
-};
-
 constexpr int G2P_THREADS = 256;
 constexpr int NODE_PAD = 16;   // floats per staged node (14 used)
 
-__global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(G2PTileArgs a)
+__global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const G2PTileArgs &a = tab[ib.item];
     const Grid &g = a.g;
-    const int tx = blockIdx.x % a.ntx;
-    const int ty = (blockIdx.x / a.ntx) % a.nty;
-    const int cz = blockIdx.x / (a.ntx * a.nty);
+    const int tx = ib.blk % a.ntx;
+    const int ty = (ib.blk / a.ntx) % a.nty;
+    const int cz = ib.blk / (a.ntx * a.nty);
     const int i0 = tx * P2G_TX, j0 = ty * P2G_TY;
     const int k0 = -1 + cz * a.tz;
     const int k1 = min(k0 + a.tz, g.nz + 1);
@@ -410,72 +322,18 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(G2PTileArgs a)
     if (e) atomicOr(a.err, e);
 }
 
-void launch_rk_stage_tiles(const Grid &g, const ParcelsDev &p, const float *const axes[6], const int *perm,
-                           const int *cell_start, const float *fld, int stage, int nstages, double cbdt, double ca,
-                           int reflect, int *keys, int *count, int *scount, int *err, hipStream_t s)
+void add_rk_stage_tiles(Plan &P, int phase, const std::vector<G2PTileArgs> &items)
 {
-    if (p.n == 0) return;
-    const P2GLayout L = p2g_layout(g);
-    G2PTileArgs a;
-    a.g = g;
-    a.p = p;
-    for (int c = 0; c < 6; ++c) a.axes[c] = axes[c];
-    a.perm = perm;
-    a.cell_start = cell_start;
-    a.fld = fld;
-    a.stage = stage;
-    a.nstages = nstages;
-    a.reflect = reflect;
-    a.cbdt = cbdt;
-    a.ca = ca;
-    a.keys = keys;
-    a.count = count;
-    a.scount = scount;
-    a.err = err;
-    a.tz = L.tz;
-    a.ntx = L.ntx;
-    a.nty = L.nty;
-    a.nchz = L.nchz;
-    hipLaunchKernelGGL(k_g2p_tiles, dim3(L.ntx * L.nty * L.nchz), dim3(G2P_THREADS), 0, s, a);
+    std::vector<int> nb;
+    for (const G2PTileArgs &a : items) nb.push_back(a.p.n > 0 ? a.ntx * a.nty * a.nchz : 0);
+    P.add((const void *)k_g2p_tiles, G2P_THREADS, 0, phase, items, nb);
 }
 
-static int g2p_blocks(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 16384); }
-
-void launch_grid2par(const Grid &g, const ParcelsDev &p, const float *fld, int add, int *err, hipStream_t s)
+void add_grid2par(Plan &P, int phase, const std::vector<G2PArgs> &items)
 {
-    if (p.n == 0) return;
-    G2PArgs a = {};
-    a.g = g;
-    a.p = p;
-    a.fld = fld;
-    a.mode = 0;
-    a.add = add;
-    a.store_axes = 1;
-    a.err = err;
-    hipLaunchKernelGGL(k_g2p, dim3(g2p_blocks(p.n)), dim3(256), 0, s, a);
-}
-
-void launch_rk_stage(const Grid &g, const ParcelsDev &p, const float *fld, int stage, int nstages, double cbdt,
-                     double ca, int reflect, int store_axes, int *keys, int *count, int *scount, int *err,
-                     hipStream_t s)
-{
-    if (p.n == 0) return;
-    G2PArgs a = {};
-    a.g = g;
-    a.p = p;
-    a.fld = fld;
-    a.mode = 1;
-    a.stage = stage;
-    a.nstages = nstages;
-    a.reflect = reflect;
-    a.store_axes = store_axes;
-    a.cbdt = cbdt;
-    a.ca = ca;
-    a.keys = keys;
-    a.count = count;
-    a.scount = scount;
-    a.err = err;
-    hipLaunchKernelGGL(k_g2p, dim3(g2p_blocks(p.n)), dim3(256), 0, s, a);
+    std::vector<int> nb;
+    for (const G2PArgs &a : items) nb.push_back((int)((a.p.n + 255) / 256));
+    P.add((const void *)k_grid2par, 256, 0, phase, items, nb);
 }
 
 }  // namespace epic

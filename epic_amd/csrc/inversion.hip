@@ -16,6 +16,8 @@
 #include "fft.h"
 #include "launch.h"
 
+#include <algorithm>
+
 namespace epic {
 
 #define EPIC_PI 3.14159265358979323846
@@ -81,11 +83,13 @@ __device__ void combine(const ColCtx &c, double2 *col, const double *phim, const
 
 __device__ inline double2 muli(double2 a, double s) { return double2{-s * a.y, s * a.x}; }   // i s a
 
-__global__ __launch_bounds__(64) void k_vor2vel_columns(ColArgs a)
+__global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const ColArgs &a = tab[ib.item];
     const Grid &g = a.g;
     const int nz = g.nz, nz1 = nz + 1;
-    const int col = blockIdx.x;
+    const int col = ib.blk;
     const int kx = col % g.nxh, ky = col / g.nxh;
     const bool zero_mode = (kx == 0 && ky == 0);
 
@@ -322,9 +326,12 @@ __global__ __launch_bounds__(64) void k_vor2vel_columns(ColArgs a)
 // ---------------------------------------------------------------------------
 // vor2vel: z halo of velocity (inversion.f90:241-248) and the derived
 // gradient components (:278-313); one thread per (i, j) column.
-__global__ void k_vor2vel_fixup(Grid g, float *fld)
+__global__ __launch_bounds__(256) void k_vor2vel_fixup(const FixArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
-    const int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const ItemBlock ib = item_block(bmap);
+    const Grid &g = tab[ib.item].g;
+    float *const fld = tab[ib.item].fld;
+    const int64_t col = ib.blk * (int64_t)blockDim.x + threadIdx.x;
     if (col >= (int64_t)g.nx * g.ny) return;
     const int64_t FS = field_stride(g), PS = plane_stride(g);
     const int nz = g.nz;
@@ -353,9 +360,12 @@ __global__ void k_vor2vel_fixup(Grid g, float *fld)
 }
 
 // vorticity_tendency: zeta walls and z extrapolation (inversion.f90:345-360)
-__global__ void k_vtend_fixup(Grid g, float *fld)
+__global__ __launch_bounds__(256) void k_vtend_fixup(const FixArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
-    const int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const ItemBlock ib = item_block(bmap);
+    const Grid &g = tab[ib.item].g;
+    float *const fld = tab[ib.item].fld;
+    const int64_t col = ib.blk * (int64_t)blockDim.x + threadIdx.x;
     if (col >= (int64_t)g.nx * g.ny) return;
     const int64_t FS = field_stride(g), PS = plane_stride(g);
     const int nz = g.nz;
@@ -374,22 +384,32 @@ size_t col_lds_bytes(const Grid &g)
     return (size_t)(6 * nz1 + 4 * g.nz) * sizeof(double2) + 2 * (size_t)nz1 * sizeof(double);
 }
 
-void launch_vor2vel_columns(const ColArgs &a, hipStream_t s)
+void add_vor2vel_columns(Plan &P, int phase, const std::vector<ColArgs> &items)
 {
-    const int ncol = a.g.ny * a.g.nxh;
-    hipLaunchKernelGGL(k_vor2vel_columns, dim3(ncol), dim3(64), col_lds_bytes(a.g), s, a);
+    std::vector<int> nb;
+    size_t lds = 0;
+    for (const ColArgs &a : items) {
+        nb.push_back(a.g.ny * a.g.nxh);
+        lds = std::max(lds, col_lds_bytes(a.g));
+    }
+    P.add((const void *)k_vor2vel_columns, 64, lds, phase, items, nb);
 }
 
-void launch_vor2vel_fixup(const Grid &g, float *fld, hipStream_t s)
+static std::vector<int> column_blocks(const std::vector<FixArgs> &items)
 {
-    const int64_t cols = (int64_t)g.nx * g.ny;
-    hipLaunchKernelGGL(k_vor2vel_fixup, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, g, fld);
+    std::vector<int> nb;
+    for (const FixArgs &a : items) nb.push_back((int)(((int64_t)a.g.nx * a.g.ny + 255) / 256));
+    return nb;
 }
 
-void launch_vtend_fixup(const Grid &g, float *fld, hipStream_t s)
+void add_vor2vel_fixup(Plan &P, int phase, const std::vector<FixArgs> &items)
 {
-    const int64_t cols = (int64_t)g.nx * g.ny;
-    hipLaunchKernelGGL(k_vtend_fixup, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, g, fld);
+    P.add((const void *)k_vor2vel_fixup, 256, 0, phase, items, column_blocks(items));
+}
+
+void add_vtend_fixup(Plan &P, int phase, const std::vector<FixArgs> &items)
+{
+    P.add((const void *)k_vtend_fixup, 256, 0, phase, items, column_blocks(items));
 }
 
 }  // namespace epic

@@ -1,36 +1,102 @@
-// Host-side launchers of the device kernels (internal to the library).
+// Per-launch argument tables and plan builders of the device kernels
+// (internal to the library).  Every builder appends ONE grouped launch whose
+// work items are the given argument structs (members, components or both).
 #pragma once
 
+#include <vector>
+
 #include "common.h"
+#include "plan.h"
 
 namespace epic {
 
-// ---- sort + par2grid (p2g.hip) ----
+// parcels of one item are processed PER_BLOCK at a time by a 256-thread block
+constexpr int PER_BLOCK = 1024;
+
+// ---- cell sort (p2g.hip) ----
+struct KeysArgs {
+    Grid g;
+    ParcelsDev p;
+    int *keys, *count, *scount, *err;
+};
+struct ScanArgs {
+    const int *in;
+    int64_t n;
+    int *out;      // n + 1 entries: exclusive prefix and total
+    int *bsum;     // scan_scratch_ints(n)
+    int *clear;    // n entries set to zero on the way (may be null)
+};
+struct PlaceArgs {
+    int64_t n;
+    const int *keys, *cell_start;
+    int *cursor, *perm;
+};
+int64_t scan_scratch_ints(int64_t n);
+void add_cell_keys(Plan &P, int phase, const std::vector<KeysArgs> &items);
+void add_exclusive_scan(Plan &P, int phase, const std::vector<ScanArgs> &items);   // three launches
+void add_place(Plan &P, int phase, const std::vector<PlaceArgs> &items);
+// order of every cell by caller index (epic_cell_sort only; a direct launch)
+void launch_cell_order(int64_t nc, const int *cell_start, const int *perm_tmp, const int *orig, int *perm,
+                       hipStream_t s);
+
+// ---- support-point half axes + par2grid (p2g.hip) ----
+struct AxesArgs {
+    ParcelsDev p;
+    float *ax[6];   // Veta (3), Vtau (3)
+    int *err;
+};
 struct P2GLayout {
     int tz, ntx, nty, nchz;
     int64_t staging_floats;
 };
 P2GLayout p2g_layout(const Grid &g);
-int64_t scan_scratch_ints(int64_t n);
-void launch_exclusive_scan(const int *in, int64_t n, int *out, int *bsum, hipStream_t s);
-void launch_cell_keys(const Grid &g, const ParcelsDev &p, int *keys, int *count, int *scount, int *err, hipStream_t s);
-void launch_place(int64_t n, const int *keys, const int *cell_start, int *cursor, int *perm_tmp, hipStream_t s);
-void launch_cell_order(int64_t nc, const int *cell_start, const int *perm_tmp, const int *orig, int *perm,
-                       hipStream_t s);
-void launch_axes(const Grid &g, const ParcelsDev &p, float *const axes[6], int *err, hipStream_t s);
-void launch_p2g(const Grid &g, const ParcelsDev &p, const float *const axes[6], const int *perm,
-                const int *cell_start, const int *count, const int *scount, float *staging, float *fld, int *nparg,
-                int *nsparg, int *err, hipStream_t s);
+struct P2GArgs {
+    Grid g;
+    ParcelsDev p;
+    const float *axes[6];       // Veta (3), Vtau (3) per parcel, FP32
+    const float *axes_alt[6];   // taken instead when *use_alt != 0
+    const int *use_alt;         // may be null
+    const int *perm;
+    const int *cell_start;
+    float *staging;
+    int *err;
+    int tz, ntx, nty, nchz;
+};
+struct FinalizeArgs {
+    Grid g;
+    const float *staging;
+    const int *count, *scount;
+    float *fld;     // F_VOLG .. F_HUMG
+    int *nparg, *nsparg;
+    int *clear_count, *clear_scount;   // histograms cleared for the next push (may be null)
+    int tz, ntx, nty, nchz;
+};
+void add_axes(Plan &P, int phase, const std::vector<AxesArgs> &items);
+void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin);
 
 // ---- grid2par + push (g2p.hip) ----
-void launch_grid2par(const Grid &g, const ParcelsDev &p, const float *fld, int add, int *err, hipStream_t s);
-void launch_rk_stage(const Grid &g, const ParcelsDev &p, const float *fld, int stage, int nstages, double cbdt,
-                     double ca, int reflect, int store_axes, int *keys, int *count, int *scount, int *err,
-                     hipStream_t s);
-
-void launch_rk_stage_tiles(const Grid &g, const ParcelsDev &p, const float *const axes[6], const int *perm,
-                           const int *cell_start, const float *fld, int stage, int nstages, double cbdt, double ca,
-                           int reflect, int *keys, int *count, int *scount, int *err, hipStream_t s);
+struct G2PArgs {
+    Grid g;
+    ParcelsDev p;      // p.veta / p.vtau receive the half axes
+    const float *fld;
+    int add;
+    int *err;
+};
+struct G2PTileArgs {
+    Grid g;
+    ParcelsDev p;
+    const float *axes[6];
+    const int *perm;
+    const int *cell_start;
+    const float *fld;
+    int stage, nstages, reflect;
+    double cbdt, ca;
+    int *keys, *count, *scount;
+    int *err;
+    int tz, ntx, nty, nchz;
+};
+void add_grid2par(Plan &P, int phase, const std::vector<G2PArgs> &items);
+void add_rk_stage_tiles(Plan &P, int phase, const std::vector<G2PTileArgs> &items);
 
 // ---- horizontal FFTs (fft.hip) ----
 struct FftXArgs {
@@ -42,8 +108,8 @@ struct FftXArgs {
     int comp_a, comp_b, sgn;
     const float *fld;      // node fields read
     float *fld_out;        // node fields written (inverse)
-    float2 *out;           // forward: T
-    const float2 *in;      // inverse: T
+    float2 *out;           // forward: T[j][kx][z]
+    const float2 *in;      // inverse: T[j][kx][z]
 };
 struct FftYArgs {
     Grid g;
@@ -51,11 +117,11 @@ struct FftYArgs {
     const float2 *tw;      // e^{-2 pi i t / ny}
     int mode;              // 0 forward, 1 inverse, 2 inverse of i kx in + i ky in2
     const float2 *in, *in2;
-    float2 *out;
+    float2 *out;           // may equal in
 };
-void launch_fft_xf(const FftXArgs &a, hipStream_t s);
-void launch_fft_y(const FftYArgs &a, hipStream_t s);
-void launch_fft_xi(const FftXArgs &a, hipStream_t s);
+void add_fft_xf(Plan &P, int phase, const std::vector<FftXArgs> &items);
+void add_fft_y(Plan &P, int phase, const std::vector<FftYArgs> &items);
+void add_fft_xi(Plan &P, int phase, const std::vector<FftXArgs> &items);
 
 // ---- vertical column kernel + fix-ups (inversion.hip) ----
 struct ColArgs {
@@ -65,12 +131,27 @@ struct ColArgs {
     const float2 *in[3];
     float2 *out[11];       // xi eta zeta u v w dudx dudy dvdy dwdx dwdy
 };
+struct FixArgs {
+    Grid g;
+    float *fld;
+};
 size_t col_lds_bytes(const Grid &g);
-void launch_vor2vel_columns(const ColArgs &a, hipStream_t s);
-void launch_vor2vel_fixup(const Grid &g, float *fld, hipStream_t s);
-void launch_vtend_fixup(const Grid &g, float *fld, hipStream_t s);
+void add_vor2vel_columns(Plan &P, int phase, const std::vector<ColArgs> &items);
+void add_vor2vel_fixup(Plan &P, int phase, const std::vector<FixArgs> &items);
+void add_vtend_fixup(Plan &P, int phase, const std::vector<FixArgs> &items);
 
-// ---- utilities (abi.cpp) ----
-void launch_fill_int(int *p, int64_t n, int v, hipStream_t s);
+// ---- utilities (abi.hip) ----
+struct FillArgs {
+    int *p;
+    int64_t n;
+    int v;
+};
+void add_fill_int(Plan &P, int phase, const std::vector<FillArgs> &items);
+// moves each item's device error word to out[item] and clears it
+struct ErrArgs {
+    int *err;
+    int *out;
+};
+void add_gather_errors(Plan &P, const std::vector<ErrArgs> &items);
 
 }  // namespace epic

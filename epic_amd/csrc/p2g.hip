@@ -28,6 +28,8 @@
 #include "ellipsoid.h"
 #include "launch.h"
 
+#include <algorithm>
+
 namespace epic {
 
 // ---------------------------------------------------------------------------
@@ -45,19 +47,24 @@ __device__ inline int cell_key(const Grid &g, double x, double y, double z, int 
     return ((k + 1) * g.ny + wrap(j, g.ny)) * g.nx + wrap(i, g.nx);
 }
 
-__global__ void k_cell_keys(Grid g, ParcelsDev p, int *__restrict__ keys, int *__restrict__ count,
-                            int *__restrict__ scount, int *__restrict__ err)
+__global__ __launch_bounds__(256) void k_cell_keys(const KeysArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const KeysArgs &a = tab[ib.item];
+    const Grid &g = a.g;
     int e = 0;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < p.n; q += (int64_t)gridDim.x * blockDim.x) {
-        const int key = cell_key(g, p.x[q], p.y[q], p.z[q], e);
-        keys[q] = key;
+#pragma unroll
+    for (int r = 0; r < PER_BLOCK / 256; ++r) {
+        const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
+        if (q >= a.p.n) break;
+        const int key = cell_key(g, a.p.x[q], a.p.y[q], a.p.z[q], e);
+        a.keys[q] = key;
         if (key >= 0) {
-            atomicAdd(&count[key], 1);
-            if (p.vol[q] <= g.vmin) atomicAdd(&scount[key], 1);
+            atomicAdd(&a.count[key], 1);
+            if (a.p.vol[q] <= g.vmin) atomicAdd(&a.scount[key], 1);
         }
     }
-    if (e) atomicOr(err, e);
+    if (e) atomicOr(a.err, e);
 }
 
 // ---------------------------------------------------------------------------
@@ -84,30 +91,35 @@ __device__ inline int block_excl_scan(int v, int *lds, int &total)
     return incl - v;
 }
 
-__global__ void k_scan_reduce(const int *__restrict__ in, int64_t n, int *__restrict__ bsum)
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const ScanArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     __shared__ int lds[SCAN_T];
-    const int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + threadIdx.x * SCAN_PER;
+    const ItemBlock ib = item_block(bmap);
+    const ScanArgs &a = tab[ib.item];
+    const int64_t base = ib.blk * (int64_t)SCAN_BLOCK + threadIdx.x * SCAN_PER;
     int s = 0;
 #pragma unroll
     for (int u = 0; u < SCAN_PER; ++u)
-        if (base + u < n) s += in[base + u];
+        if (base + u < a.n) s += a.in[base + u];
     int total;
     block_excl_scan(s, lds, total);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+    if (threadIdx.x == 0) a.bsum[ib.blk] = total;
 }
 
-// single block: exclusive scan of nb block sums in place
-__global__ void k_scan_blocks(int *__restrict__ bsum, int nb)
+// one block per item: exclusive scan of the item's block sums in place
+__global__ __launch_bounds__(SCAN_T) void k_scan_blocks(const ScanArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     __shared__ int lds[SCAN_T];
+    const ItemBlock ib = item_block(bmap);
+    const ScanArgs &a = tab[ib.item];
+    const int nb = (int)((a.n + SCAN_BLOCK - 1) / SCAN_BLOCK);
     int carry = 0;
     for (int base = 0; base < nb; base += SCAN_T * SCAN_PER) {
         const int i0 = base + threadIdx.x * SCAN_PER;
         int v[SCAN_PER], s = 0;
 #pragma unroll
         for (int u = 0; u < SCAN_PER; ++u) {
-            v[u] = (i0 + u < nb) ? bsum[i0 + u] : 0;
+            v[u] = (i0 + u < nb) ? a.bsum[i0 + u] : 0;
             s += v[u];
         }
         int total;
@@ -115,7 +127,7 @@ __global__ void k_scan_blocks(int *__restrict__ bsum, int nb)
         int run = carry + ex;
 #pragma unroll
         for (int u = 0; u < SCAN_PER; ++u) {
-            if (i0 + u < nb) bsum[i0 + u] = run;
+            if (i0 + u < nb) a.bsum[i0 + u] = run;
             run += v[u];
         }
         carry += total;
@@ -123,35 +135,45 @@ __global__ void k_scan_blocks(int *__restrict__ bsum, int nb)
     }
 }
 
-__global__ void k_scan_apply(const int *__restrict__ in, int64_t n, const int *__restrict__ bsum, int *__restrict__ out)
+// out[c] = exclusive prefix, out[n] = total; clears a.clear[c] (the
+// placement cursors) on the way
+__global__ __launch_bounds__(SCAN_T) void k_scan_apply(const ScanArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     __shared__ int lds[SCAN_T];
-    const int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + threadIdx.x * SCAN_PER;
+    const ItemBlock ib = item_block(bmap);
+    const ScanArgs &a = tab[ib.item];
+    const int64_t n = a.n;
+    const int64_t base = ib.blk * (int64_t)SCAN_BLOCK + threadIdx.x * SCAN_PER;
     int v[SCAN_PER], s = 0;
 #pragma unroll
     for (int u = 0; u < SCAN_PER; ++u) {
-        v[u] = (base + u < n) ? in[base + u] : 0;
+        v[u] = (base + u < n) ? a.in[base + u] : 0;
         s += v[u];
     }
     int total;
-    int run = block_excl_scan(s, lds, total) + bsum[blockIdx.x];
+    int run = block_excl_scan(s, lds, total) + a.bsum[ib.blk];
 #pragma unroll
     for (int u = 0; u < SCAN_PER; ++u) {
-        if (base + u < n) out[base + u] = run;
+        if (base + u < n) {
+            a.out[base + u] = run;
+            if (a.clear) a.clear[base + u] = 0;
+        }
         run += v[u];
     }
-    if (base + SCAN_PER >= n && base < n + SCAN_PER) {
-        // the thread owning element n-1 also writes the grand total out[n]
-        if (base <= n - 1 && n - 1 < base + SCAN_PER) out[n] = run;
-    }
+    // the thread owning element n-1 also writes the grand total out[n]
+    if (base <= n - 1 && n - 1 < base + SCAN_PER) a.out[n] = run;
 }
 
-void launch_exclusive_scan(const int *in, int64_t n, int *out, int *bsum, hipStream_t s)
+void add_exclusive_scan(Plan &P, int phase, const std::vector<ScanArgs> &items)
 {
-    const int nb = (int)((n + SCAN_BLOCK - 1) / SCAN_BLOCK);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s, in, n, bsum);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_T), 0, s, bsum, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, s, in, n, bsum, out);
+    std::vector<int> nb, one;
+    for (const ScanArgs &a : items) {
+        nb.push_back((int)((a.n + SCAN_BLOCK - 1) / SCAN_BLOCK));
+        one.push_back(1);
+    }
+    P.add((const void *)k_scan_reduce, SCAN_T, 0, phase, items, nb);
+    P.add((const void *)k_scan_blocks, SCAN_T, 0, phase, items, one);
+    P.add((const void *)k_scan_apply, SCAN_T, 0, phase, items, nb);
 }
 
 int64_t scan_scratch_ints(int64_t n) { return (n + SCAN_BLOCK - 1) / SCAN_BLOCK + 1; }
@@ -161,16 +183,16 @@ int64_t scan_scratch_ints(int64_t n) { return (n + SCAN_BLOCK - 1) / SCAN_BLOCK 
 // ---------------------------------------------------------------------------
 // Runs of equal keys in consecutive lanes (the parcels are nearly in cell
 // order) take one ticket per run.
-__global__ void k_place(int64_t n, const int *__restrict__ keys, const int *__restrict__ cell_start,
-                        int *__restrict__ cursor, int *__restrict__ perm_tmp)
+__global__ __launch_bounds__(256) void k_place(const PlaceArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t nround = (n + stride - 1) / stride;
-    for (int64_t rnd = 0; rnd < nround; ++rnd) {   // every lane of a wave runs the same rounds
-        const int64_t q = rnd * stride + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-        const int key = (q < n) ? keys[q] : -1;
-        const int base = wave_run_ticket(key, cursor);
-        if (key >= 0) perm_tmp[cell_start[key] + base] = (int)q;
+    const ItemBlock ib = item_block(bmap);
+    const PlaceArgs &a = tab[ib.item];
+#pragma unroll
+    for (int r = 0; r < PER_BLOCK / 256; ++r) {   // every lane of a wave runs the same rounds
+        const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
+        const int key = (q < a.n) ? a.keys[q] : -1;
+        const int base = wave_run_ticket(key, a.cursor);
+        if (key >= 0) a.perm[a.cell_start[key] + base] = (int)q;
     }
 }
 
@@ -215,28 +237,20 @@ __global__ void k_cell_order(int64_t nc, const int *__restrict__ cell_start, con
 // ---------------------------------------------------------------------------
 // 5. tile scatter
 // ---------------------------------------------------------------------------
-struct P2GArgs {
-    Grid g;
-    ParcelsDev p;
-    const float *axes[6];   // Veta (3), Vtau (3) per parcel, FP32
-    const int *perm;
-    const int *cell_start;
-    float *staging;
-    int *err;
-    int tz;                 // cell layers per chunk
-    int ntx, nty, nchz;
-};
-
 constexpr int NEXT = 300;   // extended cells of a layer: 10 x 10 x 3
 
 __device__ inline int box_planes(int tz) { return tz + 3; }
 
-__global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
+__global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const P2GArgs &a = tab[ib.item];
     const Grid &g = a.g;
-    const int tx = blockIdx.x % a.ntx;
-    const int ty = (blockIdx.x / a.ntx) % a.nty;
-    const int cz = blockIdx.x / (a.ntx * a.nty);
+    // stage 1 of a step after the first takes the cached axes (l_reuse)
+    const float *const *axes = (a.use_alt && *a.use_alt) ? a.axes_alt : a.axes;
+    const int tx = ib.blk % a.ntx;
+    const int ty = (ib.blk / a.ntx) % a.nty;
+    const int cz = ib.blk / (a.ntx * a.nty);
     const int i0 = tx * P2G_TX, j0 = ty * P2G_TY;
     const int k0 = -1 + cz * a.tz;
     const int k1 = min(k0 + a.tz, g.nz + 1);   // layers [k0, k1)
@@ -247,24 +261,22 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
     __shared__ float ring[4 * PLANE];
     __shared__ float frac[P2G_MAXP * 12];
     __shared__ float vals[P2G_MAXP * 7];
-    __shared__ unsigned short extc[P2G_MAXP * 4];  // extended cell of every support point (511: none)
-    __shared__ unsigned short spos[P2G_MAXP];      // position of the parcel in caller order within the batch
+    __shared__ __attribute__((aligned(8))) unsigned short extc[P2G_MAXP * 4];   // extended cell of every support point (511: none)
+    __shared__ unsigned short ord[P2G_MAXP];       // batch entries of each cell in caller order
     __shared__ unsigned char cell_of[P2G_MAXP];    // own cell of each batch entry
-    __shared__ __attribute__((aligned(16))) unsigned short items[P2G_MAXP * 4];   // points binned by extended cell
-    int *const origk = (int *)items;                // caller indices, only until the binning
-    __shared__ int bin_cnt[NEXT], bin_start[NEXT + 1], bin_fill[NEXT];
+    __shared__ int origk[P2G_MAXP];                // caller index of each batch entry
     __shared__ int cell_off[NCELL + 1];            // concatenated offsets of the layer's cells
+    __shared__ int cell_cnt[NCELL];
     __shared__ int row_slot[P2G_TY];
     __shared__ int row_off[P2G_TY + 1];
     __shared__ int batch_cell[NCELL + 1];          // cell-aligned batch boundaries
     __shared__ int s_nbatch, s_err;
 
     for (int u = tid; u < 4 * PLANE; u += P2G_THREADS) ring[u] = 0.0f;
-    for (int u = tid; u < NEXT; u += P2G_THREADS) bin_cnt[u] = bin_fill[u] = 0;
     if (tid == 0) s_err = 0;
 
     const int nbox = box_planes(a.tz);
-    float *stage = a.staging + (int64_t)blockIdx.x * nbox * PLANE;
+    float *stage = a.staging + (int64_t)ib.blk * nbox * PLANE;
     // extended cell owned by this thread in the accumulation (300 used)
     const bool ecell = tid < NEXT;
     const int ez = tid / 100 - 1;
@@ -292,22 +304,33 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
         if (tid == 0) {
             row_off[0] = 0;
             for (int r = 0; r < P2G_TY; ++r) row_off[r + 1] += row_off[r];
-            // cell-aligned batches of at most P2G_MAXP parcels
-            int nb = 0, acc = 0;
+        }
+        __syncthreads();
+        if (tid < NCELL) {   // one lane per cell of the layer
+            const int r = tid / P2G_TX, cx = tid % P2G_TX;
+            const int base = ((k + 1) * g.ny + (j0 + r)) * g.nx + i0 + cx;
+            const int s0 = a.cell_start[base];
+            cell_cnt[tid] = a.cell_start[base + 1] - s0;
+            cell_off[tid] = row_off[r] + (s0 - row_slot[r]);
+            if (tid == 0) cell_off[NCELL] = row_off[P2G_TY];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // cell-aligned batches of at most P2G_MAXP parcels (one batch unless crowded)
+            int nb = 0;
             batch_cell[0] = 0;
-            for (int c = 0; c < NCELL; ++c) {
-                const int r = c / P2G_TX, cx = c % P2G_TX;
-                const int base = ((k + 1) * g.ny + (j0 + r)) * g.nx + i0 + cx;
-                const int m = a.cell_start[base + 1] - a.cell_start[base];
-                cell_off[c] = row_off[r] + (a.cell_start[base] - row_slot[r]);
-                if (m > P2G_MAXP) s_err |= ERR_OUT_OF_GRID;   // one cell beyond a batch: not supported
-                if (acc + m > P2G_MAXP) {
-                    batch_cell[++nb] = c;
-                    acc = 0;
+            if (row_off[P2G_TY] > P2G_MAXP) {
+                int acc = 0;
+                for (int c = 0; c < NCELL; ++c) {
+                    const int m = cell_cnt[c];
+                    if (m > P2G_MAXP) s_err |= ERR_OUT_OF_GRID;   // one cell beyond a batch: not supported
+                    if (acc + m > P2G_MAXP) {
+                        batch_cell[++nb] = c;
+                        acc = 0;
+                    }
+                    acc += m;
                 }
-                acc += m;
             }
-            cell_off[NCELL] = row_off[P2G_TY];
             batch_cell[++nb] = NCELL;
             s_nbatch = nb;
         }
@@ -332,8 +355,8 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
                 const int pi = a.perm[slot];
                 origk[q] = a.p.orig[pi];
                 const double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
-                const double ve[3] = {a.axes[0][pi], a.axes[1][pi], a.axes[2][pi]};
-                const double vt[3] = {a.axes[3][pi], a.axes[4][pi], a.axes[5][pi]};
+                const double ve[3] = {axes[0][pi], axes[1][pi], axes[2][pi]};
+                const double vt[3] = {axes[3][pi], axes[4][pi], axes[5][pi]};
                 int cc[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
@@ -376,55 +399,14 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
                     const int oq = origk[q];
                     int rank = 0;
                     for (int q2 = lo; q2 < hi; ++q2) rank += (origk[q2] < oq);
-                    spos[q] = (unsigned short)(lo + rank);
+                    ord[lo + rank] = (unsigned short)q;
                 }
             }
             __syncthreads();
-            // ---- bin the support points by extended cell ----
-            for (int u = tid; u < nb * 4; u += P2G_THREADS) {
-                const int e = extc[u];
-                if (e < NEXT) atomicAdd(&bin_cnt[e], 1);
-            }
-            __syncthreads();
-            if (tid < 64) {   // exclusive scan of 300 counts by one wavefront
-                int run = 0;
-                for (int base = 0; base < NEXT; base += 64) {
-                    const int e = base + tid;
-                    const int v = (e < NEXT) ? bin_cnt[e] : 0;
-                    int incl = v;
-#pragma unroll
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const int t = __shfl_up(incl, o, 64);
-                        if (tid >= o) incl += t;
-                    }
-                    if (e < NEXT) bin_start[e] = run + incl - v;
-                    run += __shfl(incl, 63, 64);
-                }
-                if (tid == 0) bin_start[NEXT] = run;
-            }
-            __syncthreads();
-            for (int u = tid; u < nb * 4; u += P2G_THREADS) {
-                const int e = extc[u];
-                if (e < NEXT) items[bin_start[e] + atomicAdd(&bin_fill[e], 1)] = (unsigned short)u;
-            }
-            __syncthreads();
-            // sort every bin by (caller order, point): the accumulation order is a
-            // function of the parcel set alone
-            if (ecell) {
-                const int lo = bin_start[tid], hi = bin_start[tid + 1];
-                for (int x = lo + 1; x < hi; ++x) {
-                    const unsigned short it = items[x];
-                    const int key = spos[it >> 2] * 4 + (it & 3);
-                    int y = x - 1;
-                    while (y >= lo && (spos[items[y] >> 2] * 4 + (items[y] & 3)) > key) {
-                        items[y + 1] = items[y];
-                        --y;
-                    }
-                    items[y + 1] = it;
-                }
-            }
-            __syncthreads();
-            // ---- every extended cell accumulates its points (registers) ----
+            // ---- every extended cell gathers its points (registers): the 3x3
+            // cells of the layer around it in row-major order, their parcels in
+            // caller order, the points of a parcel in order: a fixed summation
+            // order that depends on the parcel set alone ----
             float acc[8][P2G_NF];
 #pragma unroll
             for (int c = 0; c < 8; ++c)
@@ -432,23 +414,37 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
                 for (int f = 0; f < P2G_NF; ++f) acc[c][f] = 0.0f;
             bool any = false;
             if (ecell) {
-                const int lo = bin_start[tid], hi = bin_start[tid + 1];
-                any = hi > lo;
-                for (int x = lo; x < hi; ++x) {
-                    const int it = items[x];
-                    const int q = it >> 2, jp = it & 3;
-                    const float fx = frac[q * 12 + jp * 3 + 0];
-                    const float fy = frac[q * 12 + jp * 3 + 1];
-                    const float fz = frac[q * 12 + jp * 3 + 2];
-                    const float *v = vals + q * 7;
-                    const float wxy[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx), fy * fx};
+                for (int r = ey - 1; r <= ey + 1; ++r) {
+                    if (r < 0 || r >= P2G_TY) continue;
+                    for (int cx = ex - 1; cx <= ex + 1; ++cx) {
+                        if (cx < 0 || cx >= P2G_TX) continue;
+                        const int c = r * P2G_TX + cx;
+                        if (c < cb0 || c >= cb1) continue;
+                        const int lo = cell_off[c] - q0, hi = cell_off[c + 1] - q0;
+                        for (int x = lo; x < hi; ++x) {
+                            const int q = ord[x];
+                            const ushort4 e4 = *(const ushort4 *)(extc + q * 4);
+                            const unsigned short ev[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        const float wz = (c & 4) ? fz : 1.0f - fz;
-                        const float wt = v[0] * (wz * wxy[c & 3]);
-                        acc[c][0] += wt;
+                            for (int jp = 0; jp < 4; ++jp) {
+                                if (ev[jp] != tid) continue;
+                                any = true;
+                                const float fx = frac[q * 12 + jp * 3 + 0];
+                                const float fy = frac[q * 12 + jp * 3 + 1];
+                                const float fz = frac[q * 12 + jp * 3 + 2];
+                                const float *v = vals + q * 7;
+                                const float wxy[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx),
+                                                      fy * fx};
 #pragma unroll
-                        for (int f = 1; f < P2G_NF; ++f) acc[c][f] += wt * v[f];
+                                for (int cc = 0; cc < 8; ++cc) {
+                                    const float wz = (cc & 4) ? fz : 1.0f - fz;
+                                    const float wt = v[0] * (wz * wxy[cc & 3]);
+                                    acc[cc][0] += wt;
+#pragma unroll
+                                    for (int f = 1; f < P2G_NF; ++f) acc[cc][f] += wt * v[f];
+                                }
+                            }
+                        }
                     }
                 }
             }
@@ -470,7 +466,6 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
                 }
             }
             __syncthreads();
-            for (int u = tid; u < NEXT; u += P2G_THREADS) bin_cnt[u] = bin_fill[u] = 0;
         }
         __syncthreads();
         // node plane k-1 is complete
@@ -488,15 +483,6 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(P2GArgs a)
 // ---------------------------------------------------------------------------
 // 6. staging sum + folds + normalisation (one thread per (i, j) column)
 // ---------------------------------------------------------------------------
-struct FinalizeArgs {
-    Grid g;
-    const float *staging;
-    const int *count, *scount;
-    float *fld;     // F_VOLG .. F_HUMG
-    int *nparg, *nsparg;
-    int tz, ntx, nty, nchz;
-};
-
 __device__ inline void stage_sum(const FinalizeArgs &a, int i, int j, int k, float out[P2G_NF])
 {
     constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
@@ -531,10 +517,12 @@ __device__ inline void stage_sum(const FinalizeArgs &a, int i, int j, int k, flo
     }
 }
 
-__global__ void k_p2g_finalize(FinalizeArgs a)
+__global__ __launch_bounds__(128) void k_p2g_finalize(const FinalizeArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const FinalizeArgs &a = tab[ib.item];
     const Grid &g = a.g;
-    const int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t col = ib.blk * (int64_t)blockDim.x + threadIdx.x;
     if (col >= (int64_t)g.nx * g.ny) return;
     const int i = (int)(col % g.nx), j = (int)(col / g.nx);
     const int nz = g.nz;
@@ -543,16 +531,16 @@ __global__ void k_p2g_finalize(FinalizeArgs a)
     auto F = [&](int slot, int k) -> float & { return a.fld[slot * FS + (int64_t)(k + 1) * PS + base]; };
 
     float lo[3][P2G_NF], hi[3][P2G_NF];   // raw planes -1,0,1 and nz-1,nz,nz+1
-    for (int k = -1; k <= nz + 1; ++k) {
+    // (constant indices only: the edge planes stay in registers)
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        stage_sum(a, i, j, e - 1, lo[e]);
+        stage_sum(a, i, j, nz - 1 + e, hi[e]);
+    }
+    for (int k = 2; k <= nz - 2; ++k) {
         float r[P2G_NF];
         stage_sum(a, i, j, k, r);
-        if (k <= 1) {
-#pragma unroll
-            for (int f = 0; f < P2G_NF; ++f) lo[k + 1][f] = r[f];
-        } else if (k >= nz - 1) {
-#pragma unroll
-            for (int f = 0; f < P2G_NF; ++f) hi[k - (nz - 1)][f] = r[f];
-        } else {
+        {
             const float v = r[0];
             F(F_VOLG, k) = v;
             F(F_VORTG + 0, k) = r[1] / v;
@@ -616,55 +604,73 @@ __global__ void k_p2g_finalize(FinalizeArgs a)
         a.nparg[(int64_t)(k + 1) * PS + base] = np;
         a.nsparg[(int64_t)(k + 1) * PS + base] = ns;
     }
+    // clear this column of the histogram the next push fills (it may be the
+    // one just read: every element is read above by this thread only)
+    if (a.clear_count)
+        for (int k = -1; k <= nz; ++k) {
+            const int64_t c = ((int64_t)(k + 1) * g.ny + j) * g.nx + i;
+            a.clear_count[c] = 0;
+            a.clear_scount[c] = 0;
+        }
 }
 
 // ---------------------------------------------------------------------------
 // support-point half axes, once per parcel per stage (FP64 eigen-solve)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_axes(Grid g, ParcelsDev p, float *ax0, float *ax1, float *ax2, float *ax3,
-                                              float *ax4, float *ax5, int *err)
+__global__ __launch_bounds__(256) void k_axes(const AxesArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
+    const ItemBlock ib = item_block(bmap);
+    const AxesArgs &a = tab[ib.item];
+    const ParcelsDev &p = a.p;
     int e = 0;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < p.n; q += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int r = 0; r < PER_BLOCK / 256; ++r) {
+        const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
+        if (q >= p.n) break;
         const double B[5] = {p.B[0][q], p.B[1][q], p.B[2][q], p.B[3][q], p.B[4][q]};
         double ve[3], vt[3];
         if (!support_axes(B, (double)p.vol[q], ve, vt)) {
             e |= ERR_B33;
             ve[0] = ve[1] = ve[2] = vt[0] = vt[1] = vt[2] = 0.0;
         }
-        ax0[q] = (float)ve[0];
-        ax1[q] = (float)ve[1];
-        ax2[q] = (float)ve[2];
-        ax3[q] = (float)vt[0];
-        ax4[q] = (float)vt[1];
-        ax5[q] = (float)vt[2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a.ax[d][q] = (float)ve[d];
+            a.ax[3 + d][q] = (float)vt[d];
+        }
     }
-    if (e) atomicOr(err, e);
-}
-
-void launch_axes(const Grid &g, const ParcelsDev &p, float *const axes[6], int *err, hipStream_t s)
-{
-    if (p.n == 0) return;
-    const int nb = (int)std::min<int64_t>((p.n + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_axes, dim3(nb), dim3(256), 0, s, g, p, axes[0], axes[1], axes[2], axes[3], axes[4], axes[5],
-                       err);
+    if (e) atomicOr(a.err, e);
 }
 
 // ---------------------------------------------------------------------------
-// host launchers
+// plan builders
 // ---------------------------------------------------------------------------
-void launch_cell_keys(const Grid &g, const ParcelsDev &p, int *keys, int *count, int *scount, int *err, hipStream_t s)
+static std::vector<int> parcel_blocks_of(const std::vector<int64_t> &n)
 {
-    if (p.n == 0) return;
-    const int nb = (int)std::min<int64_t>((p.n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(256), 0, s, g, p, keys, count, scount, err);
+    std::vector<int> b;
+    for (int64_t v : n) b.push_back((int)((v + PER_BLOCK - 1) / PER_BLOCK));
+    return b;
 }
 
-void launch_place(int64_t n, const int *keys, const int *cell_start, int *cursor, int *perm_tmp, hipStream_t s)
+void add_cell_keys(Plan &P, int phase, const std::vector<KeysArgs> &items)
 {
-    if (n == 0) return;
-    const int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_place, dim3(nb), dim3(256), 0, s, n, keys, cell_start, cursor, perm_tmp);
+    std::vector<int64_t> n;
+    for (const KeysArgs &a : items) n.push_back(a.p.n);
+    P.add((const void *)k_cell_keys, 256, 0, phase, items, parcel_blocks_of(n));
+}
+
+void add_place(Plan &P, int phase, const std::vector<PlaceArgs> &items)
+{
+    std::vector<int64_t> n;
+    for (const PlaceArgs &a : items) n.push_back(a.n);
+    P.add((const void *)k_place, 256, 0, phase, items, parcel_blocks_of(n));
+}
+
+void add_axes(Plan &P, int phase, const std::vector<AxesArgs> &items)
+{
+    std::vector<int64_t> n;
+    for (const AxesArgs &a : items) n.push_back(a.p.n);
+    P.add((const void *)k_axes, 256, 0, phase, items, parcel_blocks_of(n));
 }
 
 void launch_cell_order(int64_t nc, const int *cell_start, const int *perm_tmp, const int *orig, int *perm, hipStream_t s)
@@ -684,38 +690,13 @@ P2GLayout p2g_layout(const Grid &g)
     return L;
 }
 
-void launch_p2g(const Grid &g, const ParcelsDev &p, const float *const axes[6], const int *perm,
-                const int *cell_start, const int *count, const int *scount, float *staging, float *fld, int *nparg,
-                int *nsparg, int *err, hipStream_t s)
+void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin)
 {
-    const P2GLayout L = p2g_layout(g);
-    P2GArgs a;
-    a.g = g;
-    a.p = p;
-    for (int c = 0; c < 6; ++c) a.axes[c] = axes[c];
-    a.perm = perm;
-    a.cell_start = cell_start;
-    a.staging = staging;
-    a.err = err;
-    a.tz = L.tz;
-    a.ntx = L.ntx;
-    a.nty = L.nty;
-    a.nchz = L.nchz;
-    hipLaunchKernelGGL(k_p2g_tiles, dim3(L.ntx * L.nty * L.nchz), dim3(P2G_THREADS), 0, s, a);
-    FinalizeArgs f;
-    f.g = g;
-    f.staging = staging;
-    f.count = count;
-    f.scount = scount;
-    f.fld = fld;
-    f.nparg = nparg;
-    f.nsparg = nsparg;
-    f.tz = L.tz;
-    f.ntx = L.ntx;
-    f.nty = L.nty;
-    f.nchz = L.nchz;
-    const int64_t cols = (int64_t)g.nx * g.ny;
-    hipLaunchKernelGGL(k_p2g_finalize, dim3((unsigned)((cols + 127) / 128)), dim3(128), 0, s, f);
+    std::vector<int> nb, nf;
+    for (const P2GArgs &a : tiles) nb.push_back(a.ntx * a.nty * a.nchz);
+    for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny + 127) / 128));
+    P.add((const void *)k_p2g_tiles, P2G_THREADS, 0, phase, tiles, nb);
+    P.add((const void *)k_p2g_finalize, 128, 0, phase, fin, nf);
 }
 
 }  // namespace epic

@@ -31,6 +31,7 @@ extern "C" {
 #endif
 
 typedef struct epic_member *epic_handle;
+typedef struct epic_batch *epic_batch_handle;
 
 /* domain: parameters.f90:93-165 (nx, ny, nz, lower, extent) */
 typedef struct {
@@ -151,6 +152,29 @@ enum {
 };
 int epic_set_profiling(epic_handle h, int on);
 int epic_get_timers(epic_handle h, double *ms, int *calls, int reset);
+
+/* ---- ensembles: batched step, tiling autotuner, admission estimate ---- */
+/* Device bytes epic_create allocates for this grid and capacity (the
+ * admission figure of a server that keeps members resident in HBM). */
+int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels);
+/* Picks the z chunk (cell layers per work-group) of the par2grid and the
+ * fused grid2par/push tile kernels by timing each candidate (4, 8, 16, 32)
+ * on this member's parcels; leaves position, vorticity and B unchanged (the
+ * RK stage work arrays are overwritten).  Returns the choice. */
+int epic_autotune(epic_handle h, int *tz_p2g, int *tz_g2p);
+int epic_set_tiling(epic_handle h, int tz_p2g, int tz_g2p);
+/* A batch advances many members (uneven grids and parcel counts) through
+ * one ls_rk_step each (ls_rk.f90:88-136, per member exactly as
+ * epic_ls_rk_step): every kernel of the step is ONE launch over all members
+ * and the whole step is one captured HIP graph, replayed per call.  Members
+ * stay usable on their own; a member belongs to any number of batches but
+ * calls on it must not overlap. */
+int epic_batch_create(const epic_handle *members, int count, epic_batch_handle *out);
+int epic_batch_destroy(epic_batch_handle b);
+const char *epic_batch_last_error(epic_batch_handle b);
+/* t[i] += dt[i] for every member whose step succeeded; status[i] (may be
+ * NULL) gets each member's code; returns the first failure or EPIC_OK. */
+int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const double *t_initial, int *status);
 
 #ifdef __cplusplus
 }
