@@ -47,8 +47,8 @@ struct epic_member {
     float *ax_cur[6] = {}, *ax_stale[6] = {};
     ParcelsDev pd{};
     // sort: cnt[0]/scnt[0] hold the histogram of the current positions
-    // between calls; inside a step the stages alternate between the two
-    int *keys = nullptr, *cnt[2] = {}, *scnt[2] = {};
+    // between calls; inside a step the stages rotate through the three
+    int *keys = nullptr, *cnt[3] = {}, *scnt[3] = {};
     int *cell_start = nullptr, *cursor = nullptr, *perm = nullptr, *bsum = nullptr;
     bool keys_valid = false;
     int tz_p2g = 16, tz_g2p = 16;   // z chunk of the tile kernels (autotuned)
@@ -494,7 +494,8 @@ void build_vtend(Plan &P, const Members &ms)
 }
 
 // one LS-RK step (ls_rk.f90:88-136) of every member.  Histograms: stage s
-// sorts by cnt[(s-1)&1] and its push fills cnt[s&1], the last push cnt[0].
+// sorts by cnt[(s-1)%3] and its push fills cnt[s%3], the last push cnt[0]
+// (never the one the stage read, so par2grid can clear it in parallel).
 // Half axes: stages 1..ns-1 use ax_cur, the last stage ax_stale, which is
 // then the cache the next step's first par2grid takes when *reuse is set.
 struct StepItem {
@@ -523,7 +524,7 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
             const bool last = (s == ns);
             float *const *a = last ? m->ax_stale : m->ax_cur;
             ax.push_back(a);
-            const int rd = (s - 1) & 1, wr = last ? 0 : (s & 1);
+            const int rd = (s - 1) % 3, wr = last ? 0 : (s % 3);
             sel.push_back(P2GSel{a, m->ax_stale, s == 1 ? it->reuse : nullptr, rd, wr});
         }
         build_axes(P, ms, ax);
@@ -537,7 +538,7 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
             const double *ca = (m->rk_order == 4) ? CAS4 : CAS3;
             const double *cb = (m->rk_order == 4) ? CBS4 : CBS3;
             const bool last = (s == ns);
-            const int wr = last ? 0 : (s & 1);
+            const int wr = last ? 0 : (s % 3);
             const P2GLayout L = layout_tz(m->g, m->tz_g2p);
             G2PTileArgs a{};
             a.g = m->g;
@@ -708,7 +709,8 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     if (bad(dalloc(&m->pos, 3 * c)) || bad(dalloc(&m->attr, (int64_t)NATTR * c)) || bad(dalloc(&m->orig, c)) ||
         bad(dalloc(&m->axbuf, 12 * c)) || bad(dalloc(&m->keys, c)) || bad(dalloc(&m->perm, c)) ||
         bad(dalloc(&m->cnt[0], nc)) || bad(dalloc(&m->scnt[0], nc)) || bad(dalloc(&m->cnt[1], nc)) ||
-        bad(dalloc(&m->scnt[1], nc)) || bad(dalloc(&m->cell_start, nc + 1)) || bad(dalloc(&m->cursor, nc)) ||
+        bad(dalloc(&m->scnt[1], nc)) || bad(dalloc(&m->cnt[2], nc)) || bad(dalloc(&m->scnt[2], nc)) ||
+        bad(dalloc(&m->cell_start, nc + 1)) || bad(dalloc(&m->cursor, nc)) ||
         bad(dalloc(&m->bsum, scan_scratch_ints(std::max<int64_t>(nc, (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2))))))
         return EPIC_E_NOMEM;
     const int64_t FS = field_stride(g);
@@ -749,7 +751,7 @@ int epic_destroy(epic_handle m)
     if (m->s) (void)hipStreamSynchronize(m->s);
     m->invalidate_plans();
     void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
-                    m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
+                    m->cnt[2], m->scnt[2], m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
                     m->twx, m->twy, m->twz, m->err_d, m->flags_d};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1086,7 +1088,7 @@ int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels)
     const int64_t c = std::max<int64_t>(max_parcels, 1), nc = ncells(g), FS = field_stride(g);
     int64_t b = 0;
     b += c * (3 * 8 + (int64_t)NATTR * 4 + 4 + 12 * 4 + 4 + 4);               // parcels, axes, orig, keys, perm
-    b += 4 * (4 * nc + (nc + 1) + nc);                                        // histograms, cell_start, cursor
+    b += 4 * (6 * nc + (nc + 1) + nc);                                        // histograms, cell_start, cursor
     b += 4 * scan_scratch_ints(std::max<int64_t>(nc, (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2)));
     b += 4 * ((int64_t)F_COUNT * FS + 2 * FS) + 4 * staging_floats_max(g);   // fields, counts, staging
     b += (int64_t)NSPEC * 8 * spec_size(g);                                  // spectra

@@ -112,4 +112,18 @@ __device__ inline int wave_run_ticket(int key, int *counter)
     return base + (lane - hl);
 }
 
+// wave-aggregated integer increments: counter[key] += 1 for every lane, one
+// atomic per run of equal keys in consecutive lanes, no return value waited
+// for.  key < 0: nothing.  Must be called by all lanes of the wavefront.
+__device__ inline void wave_run_add(int key, int *counter)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const bool head = (lane == 0) || (prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long above = heads & ~((2ull << lane) - 1 | (lane == 63 ? ~0ull : 0ull));
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    if (head && key >= 0) (void)atomicAdd(&counter[key], next - lane);
+}
+
 }  // namespace epic

@@ -44,11 +44,27 @@ template <int R, typename T2> __device__ inline void butterfly(T2 *v, bool inv)
     }
 }
 
+// twiddle accessors: w(t) = e^{-2 pi i t / n}, t < n
+template <typename T2> struct TwStrided {   // table tw[t * stride]
+    const T2 *__restrict__ tw;
+    int stride;
+    __device__ T2 operator()(int t) const { return tw[t * stride]; }
+};
+template <typename T2> struct TwHalf {   // first half of the table only (n even): w(t + n/2) = -w(t)
+    const T2 *tw;
+    int half;
+    __device__ T2 operator()(int t) const
+    {
+        if (t < half) return tw[t];
+        const T2 w = tw[t - half];
+        return T2{-w.x, -w.y};
+    }
+};
+
 // one Stockham stage of radix R over L lines of n points (registers only:
 // the radix is a compile-time constant)
-template <int R, typename T2>
-__device__ inline void fft_stage(const T2 *in, T2 *out, int L, int n, int Ns, const T2 *__restrict__ tw, int tws,
-                                 bool inv)
+template <int R, typename T2, typename TW>
+__device__ inline void fft_stage(const T2 *in, T2 *out, int L, int n, int Ns, const TW &tw, bool inv)
 {
     const int nb = n / R;
     const int tstep = n / (Ns * R);
@@ -61,7 +77,7 @@ __device__ inline void fft_stage(const T2 *in, T2 *out, int L, int n, int Ns, co
         const int kk = jj % Ns;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-            T2 w = tw[(r * kk * tstep) * tws];
+            T2 w = tw(r * kk * tstep);
             if (inv) w = cconj(w);
             v[r] = cmul(v[r], w);
         }
@@ -73,19 +89,19 @@ __device__ inline void fft_stage(const T2 *in, T2 *out, int L, int n, int Ns, co
 }
 
 // L lines of n points each, line l at a[l*n]; every thread of the block
-// calls this.  tw[t*tws] = e^{-2 pi i t / n}.  Returns the buffer holding
-// the result (a or b).  Ends with a barrier.
-template <typename T2>
-__device__ T2 *lds_fft(T2 *a, T2 *b, int L, const FftPlan &pl, const T2 *__restrict__ tw, int tws, bool inv)
+// calls this.  Returns the buffer holding the result (a or b).  Ends with a
+// barrier.
+template <typename T2, typename TW>
+__device__ T2 *lds_fft_tw(T2 *a, T2 *b, int L, const FftPlan &pl, const TW &tw, bool inv)
 {
     const int n = pl.n;
     int Ns = 1;
     T2 *in = a, *out = b;
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.radix[st];
-        if (R == 4) fft_stage<4>(in, out, L, n, Ns, tw, tws, inv);
-        else if (R == 2) fft_stage<2>(in, out, L, n, Ns, tw, tws, inv);
-        else fft_stage<3>(in, out, L, n, Ns, tw, tws, inv);
+        if (R == 4) fft_stage<4>(in, out, L, n, Ns, tw, inv);
+        else if (R == 2) fft_stage<2>(in, out, L, n, Ns, tw, inv);
+        else fft_stage<3>(in, out, L, n, Ns, tw, inv);
         __syncthreads();
         T2 *tmp = in;
         in = out;
@@ -93,6 +109,13 @@ __device__ T2 *lds_fft(T2 *a, T2 *b, int L, const FftPlan &pl, const T2 *__restr
         Ns *= R;
     }
     return in;
+}
+
+// tw[t*tws] = e^{-2 pi i t / n}
+template <typename T2>
+__device__ T2 *lds_fft(T2 *a, T2 *b, int L, const FftPlan &pl, const T2 *__restrict__ tw, int tws, bool inv)
+{
+    return lds_fft_tw(a, b, L, pl, TwStrided<T2>{tw, tws}, inv);
 }
 
 inline FftPlan make_plan(int n)

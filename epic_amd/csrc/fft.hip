@@ -43,6 +43,8 @@ __global__ __launch_bounds__(256) void k_fft_xf(const FftXArgs *__restrict__ tab
     const int m = g.nx / 2;
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
+    float2 *tw = bufb + ZB * (m + 1);   // twiddles staged in LDS
+    for (int t = threadIdx.x; t < g.nx; t += blockDim.x) tw[t] = a.tw[t];
     for (int t = threadIdx.x; t < ZB * m; t += blockDim.x) {
         const int r = t / m, q = t - r * m;
         const int k = zp0 + r;
@@ -51,14 +53,14 @@ __global__ __launch_bounds__(256) void k_fft_xf(const FftXArgs *__restrict__ tab
         bufa[r * m + q] = z;
     }
     __syncthreads();
-    float2 *Z = lds_fft(bufa, bufb, ZB, a.plan, a.tw, 2, false);
+    float2 *Z = lds_fft(bufa, bufb, ZB, a.plan, tw, 2, false);
     float2 *other = (Z == bufa) ? bufb : bufa;
     // untangle the packed real transform: X_k, k = 0..m
     for (int t = threadIdx.x; t < ZB * (m + 1); t += blockDim.x) {
         const int kx = t / ZB, r = t - kx * ZB;
         const float2 zk = Z[r * m + (kx % m)];
         const float2 zc = cconj(Z[r * m + ((m - kx) % m)]);
-        const float2 w = a.tw[kx];   // e^{-2 pi i kx / nx}
+        const float2 w = tw[kx];   // e^{-2 pi i kx / nx}
         const float2 s = cadd(zk, zc), d = csub(zk, zc);
         const float2 wd = cmul(w, d);
         // X = (s - i w d) / 2
@@ -87,6 +89,8 @@ __global__ __launch_bounds__(256) void k_fft_y(const FftYArgs *__restrict__ tab,
     const int ny = g.ny, nxh = g.nxh;
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * ny;
+    float2 *tw = bufb + ZB * ny;   // twiddles staged in LDS
+    for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = a.tw[t];
     // inverse combine factors: i kx (0 at the x Nyquist) and i ky (0 at the y Nyquist)
     const float kxw = (2 * kx == g.nx) ? 0.0f : (float)(2.0 * 3.14159265358979323846 * kx / g.extent[0]);
     for (int t = threadIdx.x; t < ZB * ny; t += blockDim.x) {
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(256) void k_fft_y(const FftYArgs *__restrict__ tab,
         bufa[r * ny + jy] = v;
     }
     __syncthreads();
-    float2 *R = lds_fft(bufa, bufb, ZB, a.plan, a.tw, 1, a.mode != 0);
+    float2 *R = lds_fft(bufa, bufb, ZB, a.plan, tw, 1, a.mode != 0);
     for (int t = threadIdx.x; t < ZB * ny; t += blockDim.x) {
         const int jy = t / ZB, r = t - jy * ZB;
         a.out[((int64_t)jy * nxh + kx) * g.nzp + zp0 + r] = R[r * ny + jy];
@@ -125,6 +129,8 @@ __global__ __launch_bounds__(256) void k_fft_xi(const FftXArgs *__restrict__ tab
     const int m = g.nx / 2, nxh = m + 1;
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
+    float2 *tw = bufb + ZB * (m + 1);   // twiddles staged in LDS
+    for (int t = threadIdx.x; t < g.nx; t += blockDim.x) tw[t] = a.tw[t];
     for (int t = threadIdx.x; t < ZB * nxh; t += blockDim.x) {
         const int kx = t / ZB, r = t - kx * ZB;
         bufb[r * nxh + kx] = a.in[((int64_t)j * nxh + kx) * g.nzp + zp0 + r];
@@ -134,13 +140,13 @@ __global__ __launch_bounds__(256) void k_fft_xi(const FftXArgs *__restrict__ tab
         const int r = t / m, k = t - r * m;
         const float2 xk = bufb[r * nxh + k];
         const float2 xc = cconj(bufb[r * nxh + (m - k)]);
-        const float2 w = cconj(a.tw[k]);   // e^{+2 pi i k / nx}
+        const float2 w = cconj(tw[k]);   // e^{+2 pi i k / nx}
         const float2 s = cadd(xk, xc), d = csub(xk, xc);
         const float2 wd = cmul(w, d);
         bufa[r * m + k] = float2{s.x - wd.y, s.y + wd.x};   // s + i w d
     }
     __syncthreads();
-    float2 *Z = lds_fft(bufa, bufb, ZB, a.plan, a.tw, 2, true);
+    float2 *Z = lds_fft(bufa, bufb, ZB, a.plan, tw, 2, true);
     const float scale = 1.0f / ((float)g.nx * (float)g.ny);
     const int64_t FS = field_stride(g), PS = plane_stride(g);
     for (int t = threadIdx.x; t < ZB * g.nx; t += blockDim.x) {
@@ -172,8 +178,8 @@ __global__ __launch_bounds__(256) void k_fft_xi(const FftXArgs *__restrict__ tab
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static size_t x_lds(const Grid &g) { return 2 * ZB * (size_t)(g.nx / 2 + 1) * sizeof(float2); }
-static size_t y_lds(const Grid &g) { return 2 * ZB * (size_t)g.ny * sizeof(float2); }
+static size_t x_lds(const Grid &g) { return (2 * ZB * (size_t)(g.nx / 2 + 1) + g.nx) * sizeof(float2); }
+static size_t y_lds(const Grid &g) { return (2 * ZB * (size_t)g.ny + g.ny) * sizeof(float2); }
 
 void add_fft_xf(Plan &P, int phase, const std::vector<FftXArgs> &items)
 {

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
 // the half axes cached for this stage.
 // ---------------------------------------------------------------------------
 constexpr int G2P_THREADS = 256;
-constexpr int NODE_PAD = 16;   // floats per staged node (14 used)
+constexpr int NODE_PAD = 20;   // floats per staged node (14 used; 80-byte stride spreads the LDS banks)
 
 __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
@@ -315,8 +315,8 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__
             };
             process();
             // next stage's cell histogram, one atomic per run of equal keys
-            wave_run_ticket(key, a.count);
-            wave_run_ticket(small ? key : -1, a.scount);
+            wave_run_add(key, a.count);
+            wave_run_add(small ? key : -1, a.scount);
         }
     }
     if (e) atomicOr(a.err, e);

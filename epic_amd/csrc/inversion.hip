@@ -26,7 +26,7 @@ struct ColCtx {
     int nz;
     double2 *fa, *fb;
     const FftPlan *plan;
-    const double2 *tw;
+    const double2 *tw;   // LDS: e^{-2 pi i t / (2 nz)}, t < nz
 };
 
 // DST-I of col[1..nz-1] in place (col[0], col[nz] untouched)
@@ -40,7 +40,7 @@ __device__ void zdst(const ColCtx &c, double2 *col)
         c.fa[t] = v;
     }
     __syncthreads();
-    double2 *R = lds_fft(c.fa, c.fb, 1, *c.plan, c.tw, 1, false);
+    double2 *R = lds_fft_tw(c.fa, c.fb, 1, *c.plan, TwHalf<double2>{c.tw, nz}, false);
     const double s = 0.5 * sqrt(2.0 / (double)nz);
     for (int t = threadIdx.x + 1; t < nz; t += blockDim.x) col[t] = double2{-s * R[t].y, s * R[t].x};   // s i R
     __syncthreads();
@@ -52,7 +52,7 @@ __device__ void zdct(const ColCtx &c, double2 *col)
     const int nz = c.nz, n2 = 2 * nz;
     for (int t = threadIdx.x; t < n2; t += blockDim.x) c.fa[t] = (t <= nz) ? col[t] : col[n2 - t];
     __syncthreads();
-    double2 *R = lds_fft(c.fa, c.fb, 1, *c.plan, c.tw, 1, false);
+    double2 *R = lds_fft_tw(c.fa, c.fb, 1, *c.plan, TwHalf<double2>{c.tw, nz}, false);
     const double s = 0.5 * sqrt(2.0 / (double)nz);
     for (int t = threadIdx.x; t <= nz; t += blockDim.x) col[t] = double2{s * R[t].x, s * R[t].y};
     __syncthreads();
@@ -96,10 +96,12 @@ __global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restric
     extern __shared__ double2 sm[];
     double2 *SX = sm, *SY = SX + nz1, *C = SY + nz1, *E = C + nz1, *DS = E + nz1, *AS = DS + nz1;
     double2 *fa = AS + nz1, *fb = fa + 2 * nz;
-    double *phim = (double *)(fb + 2 * nz), *phip = phim + nz1;
+    double2 *twh = fb + 2 * nz;
+    double *phim = (double *)(twh + nz), *phip = phim + nz1;
     __shared__ double2 s_ds0, s_dsn, s_y0, s_yn, s_x0, s_xn;
+    for (int t = threadIdx.x; t < nz; t += blockDim.x) twh[t] = a.ztw[t];
 
-    ColCtx cc{nz, fa, fb, &a.zplan, a.ztw};
+    ColCtx cc{nz, fa, fb, &a.zplan, twh};
 
     // wavenumbers (sta3dfft.f90:88-107, sta2dfft.f90:59-64): r = (pi/L) 2k
     const int kys = (2 * ky <= g.ny) ? ky : ky - g.ny;
@@ -325,38 +327,44 @@ __global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restric
 // physical-space completion
 // ---------------------------------------------------------------------------
 // vor2vel: z halo of velocity (inversion.f90:241-248) and the derived
-// gradient components (:278-313); one thread per (i, j) column.
+// gradient components (:278-313); one thread per (column, plane k =
+// -1..nz+1).  A halo-plane thread forms its own halo values (linear
+// extrapolation, or antisymmetry for w and its horizontal derivatives) from
+// the interior planes before the derived components.
 __global__ __launch_bounds__(256) void k_vor2vel_fixup(const FixArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const Grid &g = tab[ib.item].g;
     float *const fld = tab[ib.item].fld;
-    const int64_t col = ib.blk * (int64_t)blockDim.x + threadIdx.x;
-    if (col >= (int64_t)g.nx * g.ny) return;
-    const int64_t FS = field_stride(g), PS = plane_stride(g);
     const int nz = g.nz;
-    auto F = [&](int slot, int k) -> float & { return fld[slot * FS + (int64_t)(k + 1) * PS + col]; };
-    auto lin = [&](int slot) {
-        F(slot, -1) = 2.0f * F(slot, 0) - F(slot, 1);
-        F(slot, nz + 1) = 2.0f * F(slot, nz) - F(slot, nz - 1);
-    };
-    auto anti = [&](int slot) {
-        F(slot, -1) = -F(slot, 1);
-        F(slot, nz + 1) = -F(slot, nz - 1);
-    };
-    lin(F_VELOG + 0);
-    lin(F_VELOG + 1);
-    anti(F_VELOG + 2);
-    lin(F_VELGRADG + DUDX);
-    lin(F_VELGRADG + DUDY);
-    anti(F_VELGRADG + DWDX);
-    lin(F_VELGRADG + DVDY);
-    anti(F_VELGRADG + DWDY);
-    for (int k = -1; k <= nz + 1; ++k) {
-        F(F_VELGRADG + DUDZ, k) = F(F_VELGRADG + DWDX, k) + F(F_VORTG + 1, k);
-        F(F_VELGRADG + DVDX, k) = F(F_VELGRADG + DUDY, k) + F(F_VORTG + 2, k);
-        F(F_VELGRADG + DVDZ, k) = F(F_VELGRADG + DWDY, k) - F(F_VORTG + 0, k);
+    const int64_t ncol = (int64_t)g.nx * g.ny;
+    const int64_t tq = ib.blk * (int64_t)blockDim.x + threadIdx.x;
+    if (tq >= ncol * (nz + 3)) return;
+    const int k = (int)(tq / ncol) - 1;
+    const int64_t col = tq - (int64_t)(k + 1) * ncol;
+    const int64_t FS = field_stride(g), PS = plane_stride(g);
+    auto F = [&](int slot, int kk) -> float & { return fld[slot * FS + (int64_t)(kk + 1) * PS + col]; };
+    float dwdx, dudy, dwdy;
+    if (k == -1 || k == nz + 1) {
+        const int kb = (k < 0) ? 0 : nz, kn = (k < 0) ? 1 : nz - 1;
+        auto lin = [&](int slot) { return F(slot, k) = 2.0f * F(slot, kb) - F(slot, kn); };
+        auto anti = [&](int slot) { return F(slot, k) = -F(slot, kn); };
+        lin(F_VELOG + 0);
+        lin(F_VELOG + 1);
+        anti(F_VELOG + 2);
+        lin(F_VELGRADG + DUDX);
+        dudy = lin(F_VELGRADG + DUDY);
+        dwdx = anti(F_VELGRADG + DWDX);
+        lin(F_VELGRADG + DVDY);
+        dwdy = anti(F_VELGRADG + DWDY);
+    } else {
+        dwdx = F(F_VELGRADG + DWDX, k);
+        dudy = F(F_VELGRADG + DUDY, k);
+        dwdy = F(F_VELGRADG + DWDY, k);
     }
+    F(F_VELGRADG + DUDZ, k) = dwdx + F(F_VORTG + 1, k);
+    F(F_VELGRADG + DVDX, k) = dudy + F(F_VORTG + 2, k);
+    F(F_VELGRADG + DVDZ, k) = dwdy - F(F_VORTG + 0, k);
 }
 
 // vorticity_tendency: zeta walls and z extrapolation (inversion.f90:345-360)
@@ -381,7 +389,7 @@ __global__ __launch_bounds__(256) void k_vtend_fixup(const FixArgs *__restrict__
 size_t col_lds_bytes(const Grid &g)
 {
     const int nz1 = g.nz + 1;
-    return (size_t)(6 * nz1 + 4 * g.nz) * sizeof(double2) + 2 * (size_t)nz1 * sizeof(double);
+    return (size_t)(6 * nz1 + 5 * g.nz) * sizeof(double2) + 2 * (size_t)nz1 * sizeof(double);
 }
 
 void add_vor2vel_columns(Plan &P, int phase, const std::vector<ColArgs> &items)
@@ -404,7 +412,9 @@ static std::vector<int> column_blocks(const std::vector<FixArgs> &items)
 
 void add_vor2vel_fixup(Plan &P, int phase, const std::vector<FixArgs> &items)
 {
-    P.add((const void *)k_vor2vel_fixup, 256, 0, phase, items, column_blocks(items));
+    std::vector<int> nb;
+    for (const FixArgs &a : items) nb.push_back((int)(((int64_t)a.g.nx * a.g.ny * (a.g.nz + 3) + 255) / 256));
+    P.add((const void *)k_vor2vel_fixup, 256, 0, phase, items, nb);
 }
 
 void add_vtend_fixup(Plan &P, int phase, const std::vector<FixArgs> &items)

@@ -517,101 +517,95 @@ __device__ inline void stage_sum(const FinalizeArgs &a, int i, int j, int k, flo
     }
 }
 
-__global__ __launch_bounds__(128) void k_p2g_finalize(const FinalizeArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+// one thread per (column, node plane k = -1..nz+1), consecutive threads on
+// consecutive columns.  Planes 0, 1, nz-1, nz carry the free-slip folds, the
+// halo planes -1 and nz+1 the raw sums and the z extrapolation; every thread
+// recomputes the raw planes it needs, so the planes are independent.
+__global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const FinalizeArgs &a = tab[ib.item];
     const Grid &g = a.g;
-    const int64_t col = ib.blk * (int64_t)blockDim.x + threadIdx.x;
-    if (col >= (int64_t)g.nx * g.ny) return;
-    const int i = (int)(col % g.nx), j = (int)(col / g.nx);
     const int nz = g.nz;
+    const int64_t ncol = (int64_t)g.nx * g.ny;
+    const int64_t tq = ib.blk * (int64_t)blockDim.x + threadIdx.x;
+    if (tq >= ncol * (nz + 3)) return;
+    const int k = (int)(tq / ncol) - 1;
+    const int64_t col = tq - (int64_t)(k + 1) * ncol;
+    const int i = (int)(col % g.nx), j = (int)(col / g.nx);
     const int64_t PS = plane_stride(g), FS = field_stride(g);
     const int64_t base = (int64_t)j * g.nx + i;
-    auto F = [&](int slot, int k) -> float & { return a.fld[slot * FS + (int64_t)(k + 1) * PS + base]; };
+    auto F = [&](int slot, int kk) -> float & { return a.fld[slot * FS + (int64_t)(kk + 1) * PS + base]; };
 
-    float lo[3][P2G_NF], hi[3][P2G_NF];   // raw planes -1,0,1 and nz-1,nz,nz+1
-    // (constant indices only: the edge planes stay in registers)
+    // the node value of plane kk (folds of parcel_interpl.f90:258-287)
+    auto folded = [&](int kk, float r[P2G_NF]) {
+        float x[P2G_NF], y[P2G_NF];
+        if (kk == 0 || kk == nz) {
+            stage_sum(a, i, j, kk, x);
 #pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        stage_sum(a, i, j, e - 1, lo[e]);
-        stage_sum(a, i, j, nz - 1 + e, hi[e]);
-    }
-    for (int k = 2; k <= nz - 2; ++k) {
-        float r[P2G_NF];
-        stage_sum(a, i, j, k, r);
-        {
-            const float v = r[0];
-            F(F_VOLG, k) = v;
-            F(F_VORTG + 0, k) = r[1] / v;
-            F(F_VORTG + 1, k) = r[2] / v;
-            F(F_VORTG + 2, k) = r[3] / v;
-            F(F_TBUOYG, k) = r[4] / v;
-            F(F_DBUOYG, k) = r[5] / v;
-            F(F_HUMG, k) = r[6] / v;
+            for (int f = 0; f < P2G_NF; ++f) r[f] = 2.0f * x[f];
+        } else if (kk == 1 || kk == nz - 1) {
+            stage_sum(a, i, j, kk, x);
+            stage_sum(a, i, j, kk == 1 ? -1 : nz + 1, y);
+#pragma unroll
+            for (int f = 0; f < P2G_NF; ++f) r[f] = x[f] + y[f];
+        } else {
+            stage_sum(a, i, j, kk, r);
         }
-    }
-    // free-slip folds (parcel_interpl.f90:258-287)
-    float p0[P2G_NF], p1[P2G_NF], pn[P2G_NF], pn1[P2G_NF];
-#pragma unroll
-    for (int f = 0; f < P2G_NF; ++f) {
-        p0[f] = 2.0f * lo[1][f];
-        p1[f] = lo[2][f] + lo[0][f];
-        pn[f] = 2.0f * hi[1][f];
-        pn1[f] = hi[0][f] + hi[2][f];
-    }
-    auto store_norm = [&](int k, const float *r, bool zero_zeta) {
+    };
+    auto zero_zeta = [&](int kk) { return (kk == 0 && g.zeta_zero[0]) || (kk == nz && g.zeta_zero[1]); };
+
+    if (k >= 0 && k <= nz) {
+        float r[P2G_NF];
+        folded(k, r);
         const float v = r[0];
         F(F_VOLG, k) = v;
         F(F_VORTG + 0, k) = r[1] / v;
         F(F_VORTG + 1, k) = r[2] / v;
-        F(F_VORTG + 2, k) = zero_zeta ? 0.0f : r[3] / v;
+        F(F_VORTG + 2, k) = zero_zeta(k) ? 0.0f : r[3] / v;
         F(F_TBUOYG, k) = r[4] / v;
         F(F_DBUOYG, k) = r[5] / v;
         F(F_HUMG, k) = r[6] / v;
-    };
-    store_norm(0, p0, g.zeta_zero[0] != 0);
-    store_norm(1, p1, false);
-    store_norm(nz - 1, pn1, false);
-    store_norm(nz, pn, g.zeta_zero[1] != 0);
-    // halo planes keep the raw sums for vol/dbuoy/hum; vortg and tbuoyg are
-    // extrapolated (:306-318)
-    const int halo[2] = {-1, nz + 1};
-    const float *raw[2] = {lo[0], hi[2]};
-    for (int h = 0; h < 2; ++h) {
-        const int k = halo[h];
-        F(F_VOLG, k) = raw[h][0];
-        F(F_DBUOYG, k) = raw[h][5];
-        F(F_HUMG, k) = raw[h][6];
-        const int kb = (h == 0) ? 0 : nz, kn = (h == 0) ? 1 : nz - 1;
-        for (int c = 0; c < 3; ++c) F(F_VORTG + c, k) = 2.0f * F(F_VORTG + c, kb) - F(F_VORTG + c, kn);
-        F(F_TBUOYG, k) = 2.0f * F(F_TBUOYG, kb) - F(F_TBUOYG, kn);
+    } else {
+        // halo planes keep the raw sums for vol/dbuoy/hum; vortg and tbuoyg
+        // are extrapolated from the normalised boundary planes (:306-318)
+        float raw[P2G_NF], rb[P2G_NF], rn[P2G_NF];
+        stage_sum(a, i, j, k, raw);
+        const int kb = (k < 0) ? 0 : nz, kn = (k < 0) ? 1 : nz - 1;
+        folded(kb, rb);
+        folded(kn, rn);
+        F(F_VOLG, k) = raw[0];
+        F(F_DBUOYG, k) = raw[5];
+        F(F_HUMG, k) = raw[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float vb = (c == 2 && zero_zeta(kb)) ? 0.0f : rb[1 + c] / rb[0];
+            const float vn = rn[1 + c] / rn[0];
+            F(F_VORTG + c, k) = 2.0f * vb - vn;
+        }
+        F(F_TBUOYG, k) = 2.0f * (rb[4] / rb[0]) - rn[4] / rn[0];
     }
     // parcel counts (:201-205, folds :239-243)
-    auto cnt = [&](const int *c, int k) -> int {
-        return (k >= -1 && k <= nz) ? c[((int64_t)(k + 1) * g.ny + j) * g.nx + i] : 0;
+    auto cnt = [&](const int *c, int kk) -> int {
+        return (kk >= -1 && kk <= nz) ? c[((int64_t)(kk + 1) * g.ny + j) * g.nx + i] : 0;
     };
-    for (int k = -1; k <= nz + 1; ++k) {
-        int np = cnt(a.count, k), ns = cnt(a.scount, k);
-        if (k == 0) {
-            np += cnt(a.count, -1);
-            ns += cnt(a.scount, -1);
-        }
-        if (k == nz - 1) {
-            np += cnt(a.count, nz);
-            ns += cnt(a.scount, nz);
-        }
-        a.nparg[(int64_t)(k + 1) * PS + base] = np;
-        a.nsparg[(int64_t)(k + 1) * PS + base] = ns;
+    int np = cnt(a.count, k), ns = cnt(a.scount, k);
+    if (k == 0) {
+        np += cnt(a.count, -1);
+        ns += cnt(a.scount, -1);
     }
-    // clear this column of the histogram the next push fills (it may be the
-    // one just read: every element is read above by this thread only)
-    if (a.clear_count)
-        for (int k = -1; k <= nz; ++k) {
-            const int64_t c = ((int64_t)(k + 1) * g.ny + j) * g.nx + i;
-            a.clear_count[c] = 0;
-            a.clear_scount[c] = 0;
-        }
+    if (k == nz - 1) {
+        np += cnt(a.count, nz);
+        ns += cnt(a.scount, nz);
+    }
+    a.nparg[(int64_t)(k + 1) * PS + base] = np;
+    a.nsparg[(int64_t)(k + 1) * PS + base] = ns;
+    // clear the histogram the next push fills (never the one read here)
+    if (a.clear_count && k <= nz) {
+        const int64_t c = ((int64_t)(k + 1) * g.ny + j) * g.nx + i;
+        a.clear_count[c] = 0;
+        a.clear_scount[c] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -694,9 +688,9 @@ void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::v
 {
     std::vector<int> nb, nf;
     for (const P2GArgs &a : tiles) nb.push_back(a.ntx * a.nty * a.nchz);
-    for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny + 127) / 128));
+    for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny * (f.g.nz + 3) + 255) / 256));
     P.add((const void *)k_p2g_tiles, P2G_THREADS, 0, phase, tiles, nb);
-    P.add((const void *)k_p2g_finalize, 128, 0, phase, fin, nf);
+    P.add((const void *)k_p2g_finalize, 256, 0, phase, fin, nf);
 }
 
 }  // namespace epic
