@@ -86,6 +86,7 @@ def load_library(path=LIB_PATH):
         "epic_set_profiling": ([h, i], i),
         "epic_get_timers": ([h, dp, ip, i], i),
         "epic_member_bytes": ([P(Grid), i64], i64),
+        "epic_parcel_split": ([h, d, d, P(i64)], i),
         "epic_autotune": ([h, ip, ip], i),
         "epic_set_tiling": ([h, i, i], i),
         "epic_batch_create": ([P(h), i, P(h)], i),
@@ -237,6 +238,14 @@ class Member:
         ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))   # noqa: E731
         self._chk(L.epic_cell_sort(self._h, ip(loca), ip(nppc), ip(kc1), ip(node)))
         return loca, nppc, kc1, node
+
+    # ---- parcel operations ----
+    def parcel_split(self, lambda_max=4.0, amax=0.0):
+        """parcel_split (parcel_split.f90:37-197); returns the number of splits."""
+        ns = C.c_int64(0)
+        self._chk(load_library().epic_parcel_split(self._h, lambda_max, amax, C.byref(ns)))
+        self.n = int(load_library().epic_num_parcels(self._h))
+        return ns.value
 
     # ---- tiling ----
     def autotune(self):

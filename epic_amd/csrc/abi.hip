@@ -707,11 +707,12 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     if (bad(hipStreamCreate(&m->s))) return EPIC_E_HIP;
     const int64_t c = m->cap, nc = ncells(g);
     if (bad(dalloc(&m->pos, 3 * c)) || bad(dalloc(&m->attr, (int64_t)NATTR * c)) || bad(dalloc(&m->orig, c)) ||
-        bad(dalloc(&m->axbuf, 12 * c)) || bad(dalloc(&m->keys, c)) || bad(dalloc(&m->perm, c)) ||
+        bad(dalloc(&m->axbuf, 12 * c)) || bad(dalloc(&m->keys, c)) || bad(dalloc(&m->perm, c + 1)) ||
         bad(dalloc(&m->cnt[0], nc)) || bad(dalloc(&m->scnt[0], nc)) || bad(dalloc(&m->cnt[1], nc)) ||
         bad(dalloc(&m->scnt[1], nc)) || bad(dalloc(&m->cnt[2], nc)) || bad(dalloc(&m->scnt[2], nc)) ||
         bad(dalloc(&m->cell_start, nc + 1)) || bad(dalloc(&m->cursor, nc)) ||
-        bad(dalloc(&m->bsum, scan_scratch_ints(std::max<int64_t>(nc, (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2))))))
+        bad(dalloc(&m->bsum, scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nc, c),
+                                                                  (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2))))))
         return EPIC_E_NOMEM;
     const int64_t FS = field_stride(g);
     if (bad(dalloc(&m->fld, (int64_t)F_COUNT * FS)) || bad(dalloc(&m->nparg, FS)) || bad(dalloc(&m->nsparg, FS)) ||
@@ -1087,9 +1088,9 @@ int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels)
     g.nzp = ((g.nz + 1 + ZB - 1) / ZB) * ZB;
     const int64_t c = std::max<int64_t>(max_parcels, 1), nc = ncells(g), FS = field_stride(g);
     int64_t b = 0;
-    b += c * (3 * 8 + (int64_t)NATTR * 4 + 4 + 12 * 4 + 4 + 4);               // parcels, axes, orig, keys, perm
+    b += c * (3 * 8 + (int64_t)NATTR * 4 + 4 + 12 * 4 + 4 + 4) + 4;           // parcels, axes, orig, keys, perm
     b += 4 * (6 * nc + (nc + 1) + nc);                                        // histograms, cell_start, cursor
-    b += 4 * scan_scratch_ints(std::max<int64_t>(nc, (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2)));
+    b += 4 * scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nc, c), (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2)));
     b += 4 * ((int64_t)F_COUNT * FS + 2 * FS) + 4 * staging_floats_max(g);   // fields, counts, staging
     b += (int64_t)NSPEC * 8 * spec_size(g);                                  // spectra
     return b;
@@ -1278,6 +1279,57 @@ int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const doub
         }
     }
     return first;
+}
+
+int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_split)
+{
+    if (!m) return EPIC_E_INVALID;
+    if (n_split) *n_split = 0;
+    const Grid &g = m->g;
+    if (!(lambda_max > 0.0)) lambda_max = 4.0;   // options.f90:89
+    if (!(amax > 0.0))                           // parameters.f90:150
+        amax = std::cbrt(0.75 / 3.14159265358979323846) * std::min(g.dx[0], std::min(g.dx[1], g.dx[2]));
+    const int64_t n0 = m->n;
+    if (n0 == 0) return EPIC_OK;
+    // flags in keys, child slots (exclusive prefix, total at [n0]) in perm
+    SplitArgs a{};
+    a.g = g;
+    a.p = m->pd;
+    a.lambda_max = lambda_max;
+    a.amax = amax;
+    a.flag = m->keys;
+    a.rank = m->perm;
+    a.err = m->err_d;
+    {
+        Plan P;
+        add_split_flag(P, PH_OTHER, {a});
+        add_exclusive_scan(P, PH_OTHER, {ScanArgs{m->keys, n0, m->perm, m->bsum, nullptr}});
+        HIPOK(m, P.run(m->s));
+    }
+    m->keys_valid = false;   // the keys array now holds the flags
+    int total = 0;
+    HIPOK(m, hipMemcpyAsync(&total, m->perm + n0, sizeof(int), hipMemcpyDeviceToHost, m->s));
+    int rc = check(m, "epic_parcel_split");
+    if (rc) return rc;
+    if (n0 + total > m->cap)   // the reference grows its container (parcel_split.f90:87-96)
+        return fail(m, EPIC_E_CAPACITY, "epic_parcel_split: more parcels than max_parcels after the split");
+    if (total > 0) {
+        Plan P;
+        add_split_apply(P, PH_OTHER, {a});
+        HIPOK(m, P.run(m->s));
+        m->n = n0 + total;
+        bind_parcels(m);
+        m->invalidate_plans();
+        // the cached support-point axes of every parcel from its current B
+        // (what the next correction's vol2grid stores, parcel_interpl.f90:92-106)
+        Plan Q;
+        build_axes(Q, {m}, {m->ax_stale});
+        HIPOK(m, Q.run(m->s));
+        rc = check(m, "epic_parcel_split");
+        if (rc) return rc;
+    }
+    if (n_split) *n_split = total;
+    return EPIC_OK;
 }
 
 int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)

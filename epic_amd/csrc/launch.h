@@ -140,6 +140,18 @@ void add_vor2vel_columns(Plan &P, int phase, const std::vector<ColArgs> &items);
 void add_vor2vel_fixup(Plan &P, int phase, const std::vector<FixArgs> &items);
 void add_vtend_fixup(Plan &P, int phase, const std::vector<FixArgs> &items);
 
+// ---- parcel splitting (split.hip) ----
+struct SplitArgs {
+    Grid g;
+    ParcelsDev p;          // p.n: parcel count before the split
+    double lambda_max, amax;
+    int *flag;             // n entries
+    const int *rank;       // exclusive prefix of flag
+    int *err;
+};
+void add_split_flag(Plan &P, int phase, const std::vector<SplitArgs> &items);
+void add_split_apply(Plan &P, int phase, const std::vector<SplitArgs> &items);
+
 // ---- utilities (abi.hip) ----
 struct FillArgs {
     int *p;

@@ -123,6 +123,18 @@ int epic_grid2par(epic_handle h, int add);
  * t > t_initial. */
 int epic_ls_rk_step(epic_handle h, double *t, double dt, double t_initial);
 
+/* parcel_split (parcel_split.f90:37-197): every parcel whose aspect ratio
+ * sqrt(D1/D3) >= lambda_max or whose largest eigenvalue D1 >= amax^2 splits
+ * along its major axis into two parcels of half the volume, with B reduced by
+ * 3/4 D1 v1 v1^T, at x -/+ sqrt(3/5)/2 sqrt(D1) v1 (reflective, then periodic
+ * BC).  Children are appended after the existing parcels in ascending order
+ * of their parent's index.  lambda_max <= 0 selects 4 (options.f90:89),
+ * amax <= 0 selects (3/(4 pi))^(1/3) min(dx) (parameters.f90:150).  If the
+ * split would exceed max_parcels, fails with EPIC_E_CAPACITY and changes
+ * nothing (the reference grows its container instead).  The cached
+ * support-point axes are recomputed from the new shapes. */
+int epic_parcel_split(epic_handle h, double lambda_max, double amax, int64_t *n_split);
+
 /* Cell counting sort of the parcels' current positions with the reference's
  * key ijk = 1 + ix + hx iy + hx hy iz on the halo box
  * (parcel_nearest.f90:196-257, 411-424, 531-541): loca(n) 1-based cell of

@@ -1587,3 +1587,96 @@ int orc_nearest_cell_sort(const orc_dom *d, int64_t n, const double *position, i
     free(kc2);
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* parcel_split.f90:37-197, single rank in serial order                     */
+/* ------------------------------------------------------------------------ */
+static int full_matrix(const orc_parcels *p, int64_t n, double A[9])
+{
+    const double *B = p->B + 5 * n;
+    double b33;
+    int rc = orc_get_B33(B, p->volume[n], &b33);
+    if (rc) return rc;
+    A[0] = B[0]; A[1] = B[1]; A[2] = B[2];
+    A[3] = B[1]; A[4] = B[3]; A[5] = B[4];
+    A[6] = B[2]; A[7] = B[4]; A[8] = b33;
+    return ORC_OK;
+}
+
+double orc_default_amax(const orc_dom *d)
+{
+    /* parameters.f90:150: amax = (f34 * fpi) ** f13 * minval(dx) */
+    double mdx = d->dx[0];
+    if (d->dx[1] < mdx) mdx = d->dx[1];
+    if (d->dx[2] < mdx) mdx = d->dx[2];
+    return pow(f34 * (one / ORC_PI), f13) * mdx;
+}
+
+int orc_parcel_split(const orc_dom *d, orc_parcels *p, int64_t cap, double lambda_max, double amax,
+                     int64_t *n_split)
+{
+    const double dh = f12 * sqrt(three / 5.0);
+    if (amax <= zero) amax = orc_default_amax(d);
+    const int64_t n0 = p->n;
+    /* which parcels split (:56-76): lam = sqrt(D1/D3) (get_aspect_ratio :423-429) */
+    int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n0 > 0 ? n0 : 1));
+    int64_t ni = 0;
+    for (int64_t n = 0; n < n0; ++n) {
+        double A[9], D[3];
+        int rc = full_matrix(p, n, A);
+        if (rc) {
+            free(idx);
+            return rc;
+        }
+        orc_scherzinger_eigenvalues(A, D);
+        const double lam = sqrt(D[0] / D[2]);
+        if (lam < lambda_max && D[0] < amax * amax) continue;
+        idx[ni++] = n;
+    }
+    if (n0 + ni > cap) {   /* the reference grows its container (:87-96) */
+        free(idx);
+        return ORC_E_ALLOC;
+    }
+    for (int64_t i = 0; i < ni; ++i) {
+        const int64_t n = idx[i];
+        double A[9], D[3], V[9];
+        full_matrix(p, n, A);
+        orc_scherzinger_diagonalise(A, D, V);
+        double *B = p->B + 5 * n;
+        const double vol = p->volume[n];
+        /* B - 3/4 D1 v1 v1^T (:118-122) */
+        B[0] = B[0] - f34 * D[0] * (V[0] * V[0]);
+        B[1] = B[1] - f34 * D[0] * V[0] * V[1];
+        B[2] = B[2] - f34 * D[0] * V[0] * V[2];
+        B[3] = B[3] - f34 * D[0] * (V[1] * V[1]);
+        B[4] = B[4] - f34 * D[0] * V[1] * V[2];
+        p->volume[n] = f12 * vol;
+        const int64_t c = p->n++;
+        memcpy(p->B + 5 * c, B, 5 * sizeof(double));
+        memcpy(p->vorticity + 3 * c, p->vorticity + 3 * n, 3 * sizeof(double));
+        p->volume[c] = p->volume[n];
+        p->buoyancy[c] = p->buoyancy[n];
+        p->humidity[c] = p->humidity[n];
+        for (int k = 0; k < 3; ++k) {
+            p->delta_pos[3 * c + k] = zero;
+            p->delta_vor[3 * c + k] = zero;
+        }
+        for (int k = 0; k < 5; ++k) p->delta_b[5 * c + k] = zero;
+        /* offsets along the major axis (:147-149) */
+        const double sd = sqrt(D[0]);
+        for (int k = 0; k < 3; ++k) {
+            const double v = V[k] * dh * sd;
+            p->position[3 * c + k] = p->position[3 * n + k] - v;
+            p->position[3 * n + k] = p->position[3 * n + k] + v;
+        }
+        orc_apply_reflective_bc(d, p->position + 3 * c, p->B + 5 * c);
+        orc_apply_reflective_bc(d, p->position + 3 * n, p->B + 5 * n);
+        /* parcel_communicate -> locate_parcels applies the periodic BC to
+         * both (parcel_mpi.f90:279-300) */
+        orc_apply_periodic_bc(d, p->position + 3 * c);
+        orc_apply_periodic_bc(d, p->position + 3 * n);
+    }
+    free(idx);
+    if (n_split) *n_split = ni;
+    return ORC_OK;
+}

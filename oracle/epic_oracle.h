@@ -126,6 +126,12 @@ void orc_reset_timers(void);
 /* ---- cell counting sort (parcel_nearest.f90:196-257, 411-424, 531-541) ---- */
 /* loca[n] = ijk (1-based halo cell index), node[k] = parcel index (0-based)
  * for k in [kc1[ijk]-1, kc1[ijk]+nppc[ijk]-1), kc1 1-based offsets. */
+/* ---- parcel split (parcel_split.f90:37-197) ---- */
+double orc_default_amax(const orc_dom *d);   /* (3/(4 pi))^(1/3) min(dx) (parameters.f90:150) */
+/* cap: capacity of the arrays of p; amax <= 0 selects the default */
+int orc_parcel_split(const orc_dom *d, orc_parcels *p, int64_t cap, double lambda_max, double amax,
+                     int64_t *n_split);
+
 int64_t orc_nearest_ncell(const orc_dom *d);
 int orc_nearest_cell_sort(const orc_dom *d, int64_t n, const double *position,
                           int *loca, int *nppc, int *kc1, int *node);

@@ -122,6 +122,9 @@ def _declare(L):
     L.orc_get_timers.restype = None
     L.orc_reset_timers.argtypes = []
     L.orc_reset_timers.restype = None
+    L.orc_default_amax.argtypes = [P(Dom)]
+    L.orc_default_amax.restype = d
+    L.orc_parcel_split.argtypes = [P(Dom), P(Parcels), C.c_int64, d, d, P(C.c_int64)]
     L.orc_nearest_ncell.argtypes = [P(Dom)]
     L.orc_nearest_ncell.restype = C.c_int64
     L.orc_nearest_cell_sort.argtypes = [P(Dom), C.c_int64, P(d), P(C.c_int), P(C.c_int), P(C.c_int),
@@ -274,6 +277,35 @@ class State:
                      "delta_vor", "delta_b", "strain", "Vetas", "Vtaus"):
             setattr(p, name, ptr(getattr(self, name)))
         return p
+
+    _PARCEL_ARRAYS = {"position": 3, "vorticity": 3, "B": 5, "volume": 0, "buoyancy": 0, "humidity": 0,
+                      "delta_pos": 3, "delta_vor": 3, "delta_b": 5, "strain": 8, "Vetas": 3, "Vtaus": 3}
+
+    def _grow(self, cap):
+        for name, k in self._PARCEL_ARRAYS.items():
+            a = getattr(self, name)
+            b = np.zeros((cap, k) if k else cap)
+            b[:self.n] = a[:self.n]
+            setattr(self, name, b)
+
+    def _trim(self):
+        for name in self._PARCEL_ARRAYS:
+            setattr(self, name, np.ascontiguousarray(getattr(self, name)[:self.n]))
+
+    def parcel_split(self, lambda_max=4.0, amax=0.0):
+        """parcel_split (parcel_split.f90:37-197); returns the number of splits."""
+        cap = 2 * self.n
+        self._grow(cap)
+        ns = C.c_int64(0)
+        p = self._parcels()
+        rc = lib().orc_parcel_split(C.byref(self.dom), C.byref(p), cap, lambda_max, amax, C.byref(ns))
+        self.n = int(p.n)
+        self._trim()
+        _chk(rc)
+        return ns.value
+
+    def default_amax(self):
+        return lib().orc_default_amax(C.byref(self.dom))
 
     def _fields(self):
         f = Fields()

@@ -1,0 +1,57 @@
+"""parcel_split on the GPU against the FP64 oracle, through the C-ABI.
+
+Same FP32-rounded inputs on both sides; the split decision and the child
+slots are integer work (bit-exact count and order), the new positions are
+FP64 on both sides, B and volume are stored in FP32 on the GPU (max-rel
+1e-6 against FP64)."""
+import numpy as np
+import pytest
+
+from epic_amd import workloads as W
+from epic_amd.core import EpicError, Member
+from tests._util import maxrel, oracle_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("lam", [4.0, 2.5, 1.5])
+def test_split_matches_oracle(lam):
+    mem = W.rising_thermal(16, seed=51, humidity_plume=True, n_blobs=2)
+    print(f"split: thermal16 lambda_max={lam}", flush=True)
+    gm = Member.from_workload(mem, max_parcels=2 * mem.n)
+    ns = gm.parcel_split(lambda_max=lam)
+    st = oracle_state(mem)
+    ns_ref = st.parcel_split(lambda_max=lam)
+    assert ns == ns_ref and gm.n == st.n
+    p = gm.get_parcels()
+    assert np.abs(p["position"] - st.position).max() <= 1e-9 * max(mem.dx)
+    assert maxrel(p["B"], st.B) <= 1e-6
+    assert np.array_equal(p["volume"], st.volume)
+    assert np.array_equal(p["vorticity"], st.vorticity)
+    assert np.array_equal(p["buoyancy"], st.buoyancy) and np.array_equal(p["humidity"], st.humidity)
+
+
+def test_split_capacity_error_leaves_state():
+    mem = W.rising_thermal(16, seed=52)
+    gm = Member.from_workload(mem)   # max_parcels = n
+    before = gm.get_parcels()
+    with pytest.raises(EpicError) as e:
+        gm.parcel_split(lambda_max=1.5)
+    assert e.value.code == -9
+    after = gm.get_parcels()
+    for k in before:
+        assert np.array_equal(before[k], after[k])
+
+
+def test_step_after_split_matches_oracle():
+    mem = W.rising_thermal(16, seed=53)
+    gm = Member.from_workload(mem, max_parcels=2 * mem.n)
+    gm.parcel_split(lambda_max=2.0)
+    gm.ls_rk_step(0.0, 0.01)
+    st = oracle_state(mem)
+    st.parcel_split(lambda_max=2.0)
+    st.ls_rk_step(0.0, 0.01)
+    p = gm.get_parcels()
+    assert np.abs(p["position"] - st.position).max() / max(mem.dx) <= 1e-5
+    assert maxrel(p["vorticity"], st.vorticity) <= 3e-5
+    assert maxrel(p["B"], st.B) <= 3e-5
