@@ -241,7 +241,7 @@ constexpr int NEXT = 300;   // extended cells of a layer: 10 x 10 x 3
 
 __device__ inline int box_planes(int tz) { return tz + 3; }
 
-__global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+__global__ __launch_bounds__(P2G_THREADS, 3) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const P2GArgs &a = tab[ib.item];
@@ -264,7 +264,10 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(const P2GArgs *__rest
     __shared__ __attribute__((aligned(8))) unsigned short extc[P2G_MAXP * 4];   // extended cell of every support point (511: none)
     __shared__ unsigned short ord[P2G_MAXP];       // batch entries of each cell in caller order
     __shared__ unsigned char cell_of[P2G_MAXP];    // own cell of each batch entry
-    __shared__ int origk[P2G_MAXP];                // caller index of each batch entry
+    __shared__ __attribute__((aligned(16))) unsigned short items[P2G_MAXP * 4];   // support points binned by extended cell
+    int *const origk = (int *)items;               // caller index of each batch entry (until the binning)
+    __shared__ int bin_start[NEXT];
+    __shared__ int wave_tot[P2G_THREADS / 64];
     __shared__ int cell_off[NCELL + 1];            // concatenated offsets of the layer's cells
     __shared__ int cell_cnt[NCELL];
     __shared__ int row_slot[P2G_TY];
@@ -407,13 +410,10 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(const P2GArgs *__rest
             // cells of the layer around it in row-major order, their parcels in
             // caller order, the points of a parcel in order: a fixed summation
             // order that depends on the parcel set alone ----
-            float acc[8][P2G_NF];
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-#pragma unroll
-                for (int f = 0; f < P2G_NF; ++f) acc[c][f] = 0.0f;
-            bool any = false;
-            if (ecell) {
+            // scan of the 3x3 neighbourhood in that order, twice: count, then
+            // fill this extended cell's bin (cheap passes); the accumulation
+            // then runs over the bin only
+            auto scan_nbh = [&](auto &&visit) {
                 for (int r = ey - 1; r <= ey + 1; ++r) {
                     if (r < 0 || r >= P2G_TY) continue;
                     for (int cx = ex - 1; cx <= ex + 1; ++cx) {
@@ -424,27 +424,58 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g_tiles(const P2GArgs *__rest
                         for (int x = lo; x < hi; ++x) {
                             const int q = ord[x];
                             const ushort4 e4 = *(const ushort4 *)(extc + q * 4);
-                            const unsigned short ev[4] = {e4.x, e4.y, e4.z, e4.w};
-#pragma unroll
-                            for (int jp = 0; jp < 4; ++jp) {
-                                if (ev[jp] != tid) continue;
-                                any = true;
-                                const float fx = frac[q * 12 + jp * 3 + 0];
-                                const float fy = frac[q * 12 + jp * 3 + 1];
-                                const float fz = frac[q * 12 + jp * 3 + 2];
-                                const float *v = vals + q * 7;
-                                const float wxy[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx),
-                                                      fy * fx};
-#pragma unroll
-                                for (int cc = 0; cc < 8; ++cc) {
-                                    const float wz = (cc & 4) ? fz : 1.0f - fz;
-                                    const float wt = v[0] * (wz * wxy[cc & 3]);
-                                    acc[cc][0] += wt;
-#pragma unroll
-                                    for (int f = 1; f < P2G_NF; ++f) acc[cc][f] += wt * v[f];
-                                }
-                            }
+                            if (e4.x == tid) visit(q * 4 + 0);
+                            if (e4.y == tid) visit(q * 4 + 1);
+                            if (e4.z == tid) visit(q * 4 + 2);
+                            if (e4.w == tid) visit(q * 4 + 3);
                         }
+                    }
+                }
+            };
+            int nbin = 0;
+            if (ecell) scan_nbh([&](int) { ++nbin; });
+            {   // exclusive prefix of the bin sizes over the work-group
+                const int lane = tid & 63, w = tid >> 6;
+                int incl = nbin;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(incl, o, 64);
+                    if (lane >= o) incl += t;
+                }
+                if (lane == 63) wave_tot[w] = incl;
+                __syncthreads();
+                int base = 0;
+                for (int u = 0; u < w; ++u) base += wave_tot[u];
+                if (ecell) bin_start[tid] = base + incl - nbin;
+            }
+            __syncthreads();   // origk is no longer read: items may be written
+            if (ecell) {
+                int k2 = bin_start[tid];
+                scan_nbh([&](int it) { items[k2++] = (unsigned short)it; });
+            }
+            float acc[8][P2G_NF];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int f = 0; f < P2G_NF; ++f) acc[c][f] = 0.0f;
+            const bool any = nbin > 0;
+            if (ecell) {
+                const int b0 = bin_start[tid];
+                for (int x = b0; x < b0 + nbin; ++x) {
+                    const int it = items[x];
+                    const int q = it >> 2, jp = it & 3;
+                    const float fx = frac[q * 12 + jp * 3 + 0];
+                    const float fy = frac[q * 12 + jp * 3 + 1];
+                    const float fz = frac[q * 12 + jp * 3 + 2];
+                    const float *v = vals + q * 7;
+                    const float wxy[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx), fy * fx};
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) {
+                        const float wz = (cc & 4) ? fz : 1.0f - fz;
+                        const float wt = v[0] * (wz * wxy[cc & 3]);
+                        acc[cc][0] += wt;
+#pragma unroll
+                        for (int f = 1; f < P2G_NF; ++f) acc[cc][f] += wt * v[f];
                     }
                 }
             }
