@@ -8,6 +8,7 @@
 !   vor2vel()               inversion.f90:25
 !   vorticity_tendency()    inversion.f90:321
 !   ls_rk_step(t)           ls_rk.f90:88   (fixed time step)
+!   parcel_split()          parcel_split.f90:37
 ! Parcel and field arrays keep the reference's layouts: position(3, n),
 ! vorticity(3, n), B(5, n), volume(n), ... and f(-1:nz+1, -1:ny+1, -1:nx+1).
 ! A non-zero status from the library stops the run with its message, as the
@@ -98,6 +99,16 @@ module epic_gpu_mod
             real(c_double), intent(inout) :: t
             real(c_double), value       :: dt, t_initial
         end function
+        integer(c_int) function c_epic_parcel_split(h, lambda_max, amax, n_split) bind(C, name="epic_parcel_split")
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value              :: h
+            real(c_double), value           :: lambda_max, amax
+            integer(c_int64_t), intent(out) :: n_split
+        end function
+        integer(c_int64_t) function c_epic_num_parcels(h) bind(C, name="epic_num_parcels")
+            import :: c_int64_t, c_ptr
+            type(c_ptr), value :: h
+        end function
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -110,7 +121,8 @@ module epic_gpu_mod
     double precision :: fixed_dt = 0.0d0, time_initial = 0.0d0
 
     public :: epic_gpu_init, epic_gpu_finalise, epic_gpu_upload_parcels, epic_gpu_download_parcels, &
-              epic_gpu_get_field, par2grid, grid2par, vor2vel, vorticity_tendency, ls_rk_step
+              epic_gpu_get_field, epic_gpu_num_parcels, par2grid, grid2par, vor2vel, vorticity_tendency, &
+              ls_rk_step, parcel_split
 
 contains
 
@@ -214,5 +226,21 @@ contains
         call check(c_epic_ls_rk_step(handle, tt, fixed_dt, time_initial), "ls_rk_step")
         t = tt
     end subroutine ls_rk_step
+
+    ! parcel_split with the reference's defaults (parcel%lambda_max = 4,
+    ! amax from the grid); the parcel count grows by the number of splits
+    subroutine parcel_split(lambda_max)
+        double precision, optional, intent(in) :: lambda_max
+        integer(c_int64_t) :: n_split
+        real(c_double) :: lam
+        lam = 4.0d0
+        if (present(lambda_max)) lam = lambda_max
+        call check(c_epic_parcel_split(handle, lam, 0.0d0, n_split), "parcel_split")
+        num_parcels = c_epic_num_parcels(handle)
+    end subroutine parcel_split
+
+    integer function epic_gpu_num_parcels()
+        epic_gpu_num_parcels = int(c_epic_num_parcels(handle))
+    end function epic_gpu_num_parcels
 
 end module epic_gpu_mod
