@@ -1680,3 +1680,379 @@ int orc_parcel_split(const orc_dom *d, orc_parcels *p, int64_t cap, double lambd
     if (n_split) *n_split = ni;
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* parcel_merge.f90 + parcel_nearest.f90:375-627, 937-1377 on one rank       */
+/*                                                                          */
+/* On one rank the eight MPI neighbours are the rank itself: every small    */
+/* parcel whose nearest grid index is 0 or n in x or y is sent to itself as */
+/* a periodic image (apply_periodic_shift, :564-598), appended after the    */
+/* parcels in the receive order of the directions (mpi_tags.f90) and binned */
+/* into the halo cells.  Images search like any small parcel; their result  */
+/* returns to the source, which keeps it when strictly closer               */
+/* (find_closest_parcel_globally, :794-932).                                */
+/* ------------------------------------------------------------------------ */
+static double fnint(double v) { return (double)lround(v); }   /* Fortran nint: half away from zero */
+
+int orc_parcel_merge(const orc_dom *d, orc_parcels *p, int64_t *n_merge_out)
+{
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const int hx = nx + 3, hy = ny + 3;
+    const double vmin = d->vmin > zero ? d->vmin : d->vcell / 20.0;
+    const double hlx = d->lower[0] - d->dx[0], hly = d->lower[1] - d->dx[1];
+    const int64_t n = p->n;
+    if (n_merge_out) *n_merge_out = 0;
+
+    /* handle_periodic_edge_parcels (:172-188) and the local small list */
+    int64_t ns_loc = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double *x = p->position + 3 * i;
+        int g0 = (int)floor((x[0] - d->lower[0]) * d->dxi[0]);
+        int g1 = (int)floor((x[1] - d->lower[1]) * d->dxi[1]);
+        if (g0 < -1) x[0] += d->extent[0];
+        if (g0 > nx) x[0] -= d->extent[0];
+        if (g1 < -1) x[1] += d->extent[1];
+        if (g1 > ny) x[1] -= d->extent[1];
+        if (p->volume[i] < vmin) ++ns_loc;
+    }
+    if (ns_loc == 0) return ORC_OK;
+    int64_t *sml = (int64_t *)malloc(sizeof(int64_t) * ns_loc);
+    {
+        int64_t j = 0;
+        for (int64_t i = 0; i < n; ++i)
+            if (p->volume[i] < vmin) sml[j++] = i;
+    }
+    /* images in receive order n = NORTH..SOUTHEAST; list and shift of each:
+     * list (boundary test on the nearest grid index) and shift (dx, dy) */
+    static const int lst_x[8] = {-1, -1, 1, 0, 1, 0, 1, 0};     /* ix == nx (1), == 0 (0), any (-1) */
+    static const int lst_y[8] = {0, 1, -1, -1, 0, 0, 1, 1};     /* iy == 0 (0), == ny (1), any (-1) */
+    static const int sh_x[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+    static const int sh_y[8] = {1, -1, 0, 0, 1, 1, -1, -1};
+    int64_t cap_img = 8 * ns_loc, nimg = 0;
+    double *ipos = (double *)malloc(sizeof(double) * 3 * cap_img);
+    int64_t *ipid = (int64_t *)malloc(sizeof(int64_t) * cap_img);
+    int64_t *imid = (int64_t *)malloc(sizeof(int64_t) * cap_img);
+    int img_dir_start[9];
+    for (int dir = 0; dir < 8; ++dir) {
+        img_dir_start[dir] = (int)nimg;
+        for (int64_t m = 0; m < ns_loc; ++m) {
+            const double *x = p->position + 3 * sml[m];
+            const int ix = (int)fnint(d->dxi[0] * (x[0] - d->lower[0]));
+            const int iy = (int)fnint(d->dxi[1] * (x[1] - d->lower[1]));
+            const int in_x = (lst_x[dir] < 0) || (lst_x[dir] == 0 && ix == 0) || (lst_x[dir] == 1 && ix == nx);
+            const int in_y = (lst_y[dir] < 0) || (lst_y[dir] == 0 && iy == 0) || (lst_y[dir] == 1 && iy == ny);
+            if (!(in_x && in_y)) continue;
+            /* apply_periodic_shift: edges tested on the unshifted position */
+            const int ex_ = (ix <= 0) || (ix >= nx), ey_ = (iy <= 0) || (iy >= ny);
+            double *q = ipos + 3 * nimg;
+            q[0] = x[0] + ((sh_x[dir] != 0 && ex_) ? sh_x[dir] * d->extent[0] : zero);
+            q[1] = x[1] + ((sh_y[dir] != 0 && ey_) ? sh_y[dir] * d->extent[1] : zero);
+            q[2] = x[2];
+            ipid[nimg] = sml[m];
+            imid[nimg] = m;
+            ++nimg;
+        }
+    }
+    img_dir_start[8] = (int)nimg;
+    /* bin parcels then images into the halo cells (:196-257) */
+    const int64_t ntot = n + nimg;
+    const int64_t ncell = orc_nearest_ncell(d);
+    int *nppc = (int *)calloc((size_t)ncell, sizeof(int));
+    int *kc1 = (int *)malloc(sizeof(int) * ncell), *kc2 = (int *)malloc(sizeof(int) * ncell);
+    int64_t *loca = (int64_t *)malloc(sizeof(int64_t) * ntot);
+    int64_t *node = (int64_t *)malloc(sizeof(int64_t) * ntot);
+    int rc = ORC_OK;
+#define PPOS(i) ((i) < n ? p->position + 3 * (i) : ipos + 3 * ((i) - n))
+    for (int64_t i = 0; i < ntot && rc == ORC_OK; ++i) {
+        double *x = PPOS(i);
+        if (i >= n) {   /* handle_periodic_edge_parcels for the images */
+            int g0 = (int)floor((x[0] - d->lower[0]) * d->dxi[0]);
+            int g1 = (int)floor((x[1] - d->lower[1]) * d->dxi[1]);
+            if (g0 < -1) x[0] += d->extent[0];
+            if (g0 > nx) x[0] -= d->extent[0];
+            if (g1 < -1) x[1] += d->extent[1];
+            if (g1 > ny) x[1] -= d->extent[1];
+        }
+        int jx = (int)(d->dxi[0] * (x[0] - hlx));
+        int jy = (int)(d->dxi[1] * (x[1] - hly));
+        int jz = (int)(d->dxi[2] * (x[2] - d->lower[2]));
+        if (jz > nz - 1) jz = nz - 1;
+        int64_t ijk = (int64_t)jx + (int64_t)hx * jy + (int64_t)hx * hy * jz;
+        if (ijk < 0 || ijk >= ncell) rc = ORC_E_OUT_OF_GRID;
+        else {
+            nppc[ijk] += 1;
+            loca[i] = ijk;
+        }
+    }
+    kc1[0] = 0;
+    for (int64_t c = 1; c < ncell; ++c) kc1[c] = kc1[c - 1] + nppc[c - 1];
+    for (int64_t c = 0; c < ncell; ++c) kc2[c] = kc1[c];
+    for (int64_t i = 0; i < ntot && rc == ORC_OK; ++i) node[kc2[loca[i]]++] = i;
+    /* closest other parcel of every small parcel, local then images (:608-662) */
+    const int64_t nsm = ns_loc + nimg;
+    int64_t *iclo = (int64_t *)malloc(sizeof(int64_t) * nsm);
+    double *dclo = (double *)malloc(sizeof(double) * nsm);
+    const double dsq0 = d->extent[0] * d->extent[0] + d->extent[1] * d->extent[1] + d->extent[2] * d->extent[2];
+    for (int64_t m = 0; m < nsm && rc == ORC_OK; ++m) {
+        const int64_t is = (m < ns_loc) ? sml[m] : n + (m - ns_loc);
+        const double *xs = PPOS(is);
+        const int ix0 = (int)fnint(d->dxi[0] * (xs[0] - hlx));
+        const int iy0 = (int)fnint(d->dxi[1] * (xs[1] - hly));
+        const int iz0 = (int)fnint(d->dxi[2] * (xs[2] - d->lower[2]));
+        double dmin = dsq0;
+        int64_t ic = -1;
+        for (int iz = (iz0 - 1 > 0 ? iz0 - 1 : 0); iz <= (iz0 < nz - 1 ? iz0 : nz - 1); ++iz)
+            for (int iy = (iy0 - 1 > 0 ? iy0 - 1 : 0); iy <= (iy0 < ny + 1 ? iy0 : ny + 1); ++iy)
+                for (int ix = (ix0 - 1 > 0 ? ix0 - 1 : 0); ix <= (ix0 < nx + 1 ? ix0 : nx + 1); ++ix) {
+                    const int64_t ijk = (int64_t)ix + (int64_t)hx * iy + (int64_t)hx * hy * iz;
+                    for (int k = kc1[ijk]; k < kc2[ijk]; ++k) {
+                        const int64_t j = node[k];
+                        if (j == is) continue;
+                        const double *xj = PPOS(j);
+                        const double delz = xj[2] - xs[2];
+                        if (delz * delz < dmin) {
+                            const double delx = xj[0] - xs[0], dely = xj[1] - xs[1];
+                            const double dsq = delx * delx + dely * dely + delz * delz;
+                            if (dsq < dmin) {
+                                dmin = dsq;
+                                ic = j;
+                            }
+                        }
+                    }
+                }
+        /* update_remote_indices (:746-788) */
+        if (is >= n && ic >= n) dmin = HUGE_VAL;
+        else if (ic >= n) ic = ipid[ic - n];
+        iclo[m] = ic;
+        dclo[m] = dmin;
+    }
+    /* find_closest_parcel_globally: the image results of the directions in
+     * the order SOUTH, NORTH, EAST, WEST, SOUTHEAST, SOUTHWEST, NORTHEAST,
+     * NORTHWEST (receive tag n' = SEND_NEIGHBOUR_TAG of the image direction) */
+    static const int back_order[8] = {1, 0, 3, 2, 7, 6, 5, 4};
+    for (int o = 0; o < 8 && rc == ORC_OK; ++o) {
+        const int dir = back_order[o];
+        for (int k = img_dir_start[dir]; k < img_dir_start[dir + 1]; ++k) {
+            const int64_t m = ns_loc + k, ms = imid[k];
+            if (dclo[ms] > dclo[m]) {
+                dclo[ms] = dclo[m];
+                iclo[ms] = iclo[m];
+            }
+        }
+    }
+    for (int64_t m = 0; m < ns_loc && rc == ORC_OK; ++m)
+        if (iclo[m] < 0) rc = ORC_E_MERGE;   /* 'Merge error: no near enough neighbour found.' */
+    /* resolve_tree (:937-1377), one rank */
+    unsigned char *leaf = NULL, *avail = NULL, *merged = NULL, *dom_ = NULL, *iso = NULL;
+    int64_t nm = 0;
+    int64_t *isma = NULL, *ic_m = NULL;
+    if (rc == ORC_OK) {
+        leaf = (unsigned char *)calloc((size_t)n, 1);
+        avail = (unsigned char *)calloc((size_t)n, 1);
+        merged = (unsigned char *)calloc((size_t)n, 1);
+        dom_ = (unsigned char *)calloc((size_t)ns_loc, 1);
+        iso = (unsigned char *)calloc((size_t)ns_loc, 1);
+        int cont = 1;
+        while (cont) {
+            cont = 0;
+            for (int64_t m = 0; m < ns_loc; ++m)
+                if (!merged[sml[m]]) {
+                    leaf[sml[m]] = 1;
+                    avail[iclo[m]] = 1;
+                }
+            for (int64_t m = 0; m < ns_loc; ++m)
+                if (!merged[sml[m]]) leaf[iclo[m]] = 0;
+            for (int64_t m = 0; m < ns_loc; ++m)
+                if (!merged[sml[m]] && !leaf[sml[m]]) avail[iclo[m]] = 0;
+            for (int64_t m = 0; m < ns_loc; ++m) {
+                const int64_t is = sml[m], ic = iclo[m];
+                if (!merged[is] && leaf[is] && avail[ic]) {
+                    cont = 1;
+                    merged[is] = 1;
+                    merged[ic] = 1;
+                }
+            }
+        }
+        for (int64_t m = 0; m < ns_loc; ++m)
+            if (!merged[sml[m]] && leaf[sml[m]]) avail[iclo[m]] = 1;
+        for (int64_t m = 0; m < ns_loc; ++m) {
+            const int64_t is = sml[m], ic = iclo[m];
+            if (merged[is] && leaf[is]) {
+                dom_[m] = 1;
+            } else if (!merged[is]) {
+                if (leaf[is]) dom_[m] = 1;
+                else if (!avail[is]) {
+                    if (avail[ic]) dom_[m] = 1;
+                    else {
+                        iso[m] = 1;
+                        avail[is] = 1;   /* the lower rank (here: the only one) makes its parcel available */
+                    }
+                }
+            }
+        }
+        for (int64_t m = 0; m < ns_loc; ++m)
+            if (!dom_[m] && iso[m] && avail[iclo[m]]) dom_[m] = 1;
+        isma = (int64_t *)malloc(sizeof(int64_t) * ns_loc);
+        ic_m = (int64_t *)malloc(sizeof(int64_t) * ns_loc);
+        for (int64_t m = 0; m < ns_loc; ++m)
+            if (dom_[m]) {
+                isma[nm] = sml[m];
+                ic_m[nm] = iclo[m];
+                ++nm;
+            }
+    }
+    /* geometric_merge / do_group_merge (parcel_merge.f90:129-317) */
+    if (rc == ORC_OK && nm > 0) {
+        int64_t *grp = (int64_t *)malloc(sizeof(int64_t) * n);
+        for (int64_t i = 0; i < n; ++i) grp[i] = -1;
+        double *vm = (double *)malloc(sizeof(double) * nm), *x0 = (double *)malloc(sizeof(double) * nm);
+        double *y0 = (double *)malloc(sizeof(double) * nm), *posm = (double *)malloc(sizeof(double) * 3 * nm);
+        double *bm = (double *)malloc(sizeof(double) * nm), *hm = (double *)malloc(sizeof(double) * nm);
+        double *wm = (double *)malloc(sizeof(double) * 3 * nm), *Bm = (double *)malloc(sizeof(double) * 6 * nm);
+        int64_t ng = 0;
+        for (int64_t m = 0; m < nm; ++m) {
+            const int64_t ic = ic_m[m], is = isma[m];
+            orc_apply_periodic_bc(d, p->position + 3 * is);
+            if (grp[ic] < 0) {
+                const int64_t l = ng++;
+                grp[ic] = l;
+                const double v = p->volume[ic];
+                vm[l] = v;
+                x0[l] = p->position[3 * ic + 0];
+                y0[l] = p->position[3 * ic + 1];
+                posm[3 * l + 0] = zero;
+                posm[3 * l + 1] = zero;
+                posm[3 * l + 2] = v * p->position[3 * ic + 2];
+                bm[l] = v * p->buoyancy[ic];
+                hm[l] = v * p->humidity[ic];
+                for (int k = 0; k < 3; ++k) wm[3 * l + k] = v * p->vorticity[3 * ic + k];
+                for (int k = 0; k < 6; ++k) Bm[6 * l + k] = zero;
+            }
+            const int64_t l = grp[ic];
+            const double vs = p->volume[is];
+            vm[l] = vm[l] + vs;
+            double delx = p->position[3 * is + 0] - x0[l];
+            delx = delx - d->extent[0] * fnint(delx * (one / d->extent[0]));
+            double dely = p->position[3 * is + 1] - y0[l];
+            dely = dely - d->extent[1] * fnint(dely * (one / d->extent[1]));
+            posm[3 * l + 0] = posm[3 * l + 0] + vs * delx;
+            posm[3 * l + 1] = posm[3 * l + 1] + vs * dely;
+            posm[3 * l + 2] = posm[3 * l + 2] + vs * p->position[3 * is + 2];
+            bm[l] = bm[l] + vs * p->buoyancy[is];
+            hm[l] = hm[l] + vs * p->humidity[is];
+            for (int k = 0; k < 3; ++k) wm[3 * l + k] = wm[3 * l + k] + vs * p->vorticity[3 * is + k];
+        }
+        for (int64_t l = 0; l < ng; ++l) {
+            const double vi = one / vm[l];
+            posm[3 * l + 0] = vi * posm[3 * l + 0];
+            posm[3 * l + 1] = vi * posm[3 * l + 1];
+            posm[3 * l + 0] = x0[l] + posm[3 * l + 0];
+            posm[3 * l + 1] = y0[l] + posm[3 * l + 1];
+            posm[3 * l + 2] = vi * posm[3 * l + 2];
+            orc_apply_periodic_bc(d, posm + 3 * l);
+            bm[l] = vi * bm[l];
+            hm[l] = vi * hm[l];
+            for (int k = 0; k < 3; ++k) wm[3 * l + k] = vi * wm[3 * l + k];
+        }
+        for (int64_t i = 0; i < n; ++i) grp[i] = -1;
+        int64_t l2 = 0;
+        for (int64_t m = 0; m < nm && rc == ORC_OK; ++m) {
+            const int64_t ic = ic_m[m], is = isma[m];
+            double b33;
+            if (grp[ic] < 0) {
+                const int64_t l = l2++;
+                grp[ic] = l;
+                const double vi = one / vm[l];
+                rc = orc_get_B33(p->B + 5 * ic, p->volume[ic], &b33);
+                if (rc) break;
+                double delx = p->position[3 * ic + 0] - posm[3 * l + 0];
+                delx = delx - d->extent[0] * fnint(delx * (one / d->extent[0]));
+                double dely = p->position[3 * ic + 1] - posm[3 * l + 1];
+                dely = dely - d->extent[1] * fnint(dely * (one / d->extent[1]));
+                const double delz = p->position[3 * ic + 2] - posm[3 * l + 2];
+                const double mu = p->volume[ic] * vi;
+                const double *B = p->B + 5 * ic;
+                Bm[6 * l + 0] = mu * (5.0 * (delx * delx) + B[0]);
+                Bm[6 * l + 1] = mu * (5.0 * delx * dely + B[1]);
+                Bm[6 * l + 2] = mu * (5.0 * delx * delz + B[2]);
+                Bm[6 * l + 3] = mu * (5.0 * (dely * dely) + B[3]);
+                Bm[6 * l + 4] = mu * (5.0 * dely * delz + B[4]);
+                Bm[6 * l + 5] = mu * (5.0 * (delz * delz) + b33);
+                p->volume[ic] = vm[l];
+                for (int k = 0; k < 3; ++k) p->position[3 * ic + k] = posm[3 * l + k];
+                p->buoyancy[ic] = bm[l];
+                p->humidity[ic] = hm[l];
+                for (int k = 0; k < 3; ++k) p->vorticity[3 * ic + k] = wm[3 * l + k];
+            }
+            const int64_t l = grp[ic];
+            const double vi = one / vm[l];
+            double delx = p->position[3 * is + 0] - posm[3 * l + 0];
+            delx = delx - d->extent[0] * fnint(delx * (one / d->extent[0]));
+            double dely = p->position[3 * is + 1] - posm[3 * l + 1];
+            dely = dely - d->extent[1] * fnint(dely * (one / d->extent[1]));
+            const double delz = p->position[3 * is + 2] - posm[3 * l + 2];
+            rc = orc_get_B33(p->B + 5 * is, p->volume[is], &b33);
+            if (rc) break;
+            const double mu = vi * p->volume[is];
+            const double *B = p->B + 5 * is;
+            Bm[6 * l + 0] = Bm[6 * l + 0] + mu * (5.0 * (delx * delx) + B[0]);
+            Bm[6 * l + 1] = Bm[6 * l + 1] + mu * (5.0 * delx * dely + B[1]);
+            Bm[6 * l + 2] = Bm[6 * l + 2] + mu * (5.0 * delx * delz + B[2]);
+            Bm[6 * l + 3] = Bm[6 * l + 3] + mu * (5.0 * (dely * dely) + B[3]);
+            Bm[6 * l + 4] = Bm[6 * l + 4] + mu * (5.0 * dely * delz + B[4]);
+            Bm[6 * l + 5] = Bm[6 * l + 5] + mu * (5.0 * (delz * delz) + b33);
+        }
+        /* geometric_merge normalisation (:319-352) */
+        for (int64_t i = 0; i < n; ++i) grp[i] = -1;
+        l2 = 0;
+        for (int64_t m = 0; m < nm && rc == ORC_OK; ++m) {
+            const int64_t ic = ic_m[m];
+            if (grp[ic] >= 0) continue;
+            const int64_t l = l2++;
+            grp[ic] = l;
+            const double *b = Bm + 6 * l;
+            const double detB = b[0] * (b[3] * b[5] - b[4] * b[4]) - b[1] * (b[1] * b[5] - b[2] * b[4]) +
+                                b[2] * (b[1] * b[4] - b[2] * b[3]);
+            const double abc = f34 * vm[l] * (one / ORC_PI);
+            const double factor = pow(abc * abc / detB, f13);
+            for (int k = 0; k < 5; ++k) p->B[5 * ic + k] = b[k] * factor;
+        }
+        free(grp); free(vm); free(x0); free(y0); free(posm); free(bm); free(hm); free(wm); free(Bm);
+    }
+    /* parcels%delete (parcel_container.f90:413-452): holes filled from the end */
+    if (rc == ORC_OK && nm > 0) {
+        int64_t l = n - 1, k = nm - 1;
+        while (k >= 0 && l == isma[k]) {
+            --l;
+            --k;
+        }
+        for (int64_t m = 0; m <= k; ++m) {
+            const int64_t dst = isma[m], src = l;
+            memcpy(p->position + 3 * dst, p->position + 3 * src, 3 * sizeof(double));
+            memcpy(p->vorticity + 3 * dst, p->vorticity + 3 * src, 3 * sizeof(double));
+            memcpy(p->B + 5 * dst, p->B + 5 * src, 5 * sizeof(double));
+            p->volume[dst] = p->volume[src];
+            p->buoyancy[dst] = p->buoyancy[src];
+            p->humidity[dst] = p->humidity[src];
+            memcpy(p->delta_pos + 3 * dst, p->delta_pos + 3 * src, 3 * sizeof(double));
+            memcpy(p->delta_vor + 3 * dst, p->delta_vor + 3 * src, 3 * sizeof(double));
+            memcpy(p->delta_b + 5 * dst, p->delta_b + 5 * src, 5 * sizeof(double));
+            memcpy(p->strain + 8 * dst, p->strain + 8 * src, 8 * sizeof(double));
+            memcpy(p->Vetas + 3 * dst, p->Vetas + 3 * src, 3 * sizeof(double));
+            memcpy(p->Vtaus + 3 * dst, p->Vtaus + 3 * src, 3 * sizeof(double));
+            --l;
+            while (k >= 0 && l == isma[k]) {
+                --l;
+                --k;
+            }
+        }
+        p->n = n - nm;
+        /* parcel_communicate -> locate_parcels: periodic BC on every parcel */
+        for (int64_t i = 0; i < p->n; ++i) orc_apply_periodic_bc(d, p->position + 3 * i);
+    }
+#undef PPOS
+    if (n_merge_out && rc == ORC_OK) *n_merge_out = nm;
+    free(sml); free(ipos); free(ipid); free(imid); free(nppc); free(kc1); free(kc2); free(loca); free(node);
+    free(iclo); free(dclo); free(leaf); free(avail); free(merged); free(dom_); free(iso); free(isma); free(ic_m);
+    return rc;
+}

@@ -33,7 +33,8 @@ enum {
     ORC_E_RK_ORDER = -4,       /* ls_rk_setup: only 3rd/4th order (ls_rk.f90:76-78) */
     ORC_E_DT = -5,             /* dt <= 0 (rk_utils.f90:179-182) */
     ORC_E_FFT_LEN = -6,        /* length not factorisable by 2,3,5 (stafft.f90:87-96) */
-    ORC_E_ALLOC = -7
+    ORC_E_ALLOC = -7,
+    ORC_E_MERGE = -8           /* Merge error: no near enough neighbour found (parcel_nearest.f90:551-556) */
 };
 
 typedef struct {
@@ -131,6 +132,9 @@ double orc_default_amax(const orc_dom *d);   /* (3/(4 pi))^(1/3) min(dx) (parame
 /* cap: capacity of the arrays of p; amax <= 0 selects the default */
 int orc_parcel_split(const orc_dom *d, orc_parcels *p, int64_t cap, double lambda_max, double amax,
                      int64_t *n_split);
+
+/* ---- parcel merge (parcel_merge.f90, parcel_nearest.f90), one rank ---- */
+int orc_parcel_merge(const orc_dom *d, orc_parcels *p, int64_t *n_merge);
 
 int64_t orc_nearest_ncell(const orc_dom *d);
 int orc_nearest_cell_sort(const orc_dom *d, int64_t n, const double *position,

@@ -125,6 +125,7 @@ def _declare(L):
     L.orc_default_amax.argtypes = [P(Dom)]
     L.orc_default_amax.restype = d
     L.orc_parcel_split.argtypes = [P(Dom), P(Parcels), C.c_int64, d, d, P(C.c_int64)]
+    L.orc_parcel_merge.argtypes = [P(Dom), P(Parcels), P(C.c_int64)]
     L.orc_nearest_ncell.argtypes = [P(Dom)]
     L.orc_nearest_ncell.restype = C.c_int64
     L.orc_nearest_cell_sort.argtypes = [P(Dom), C.c_int64, P(d), P(C.c_int), P(C.c_int), P(C.c_int),
@@ -139,6 +140,7 @@ ERRORS = {
     -5: "time step must be positive",
     -6: "FFT length not factorisable",
     -7: "allocation failed",
+    -8: "Merge error: no near enough neighbour found.",
 }
 
 
@@ -303,6 +305,17 @@ class State:
         self._trim()
         _chk(rc)
         return ns.value
+
+    def parcel_merge(self):
+        """parcel_merge (parcel_merge.f90, parcel_nearest.f90) on one rank;
+        returns the number of merged small parcels."""
+        nm = C.c_int64(0)
+        p = self._parcels()
+        rc = lib().orc_parcel_merge(C.byref(self.dom), C.byref(p), C.byref(nm))
+        self.n = int(p.n)
+        self._trim()
+        _chk(rc)
+        return nm.value
 
     def default_amax(self):
         return lib().orc_default_amax(C.byref(self.dom))

@@ -1,0 +1,183 @@
+"""parcel_merge of the oracle (parcel_merge.f90, parcel_nearest.f90 on one
+rank) against what the definition fixes:
+
+* volume, volume-weighted vorticity, buoyancy and humidity, and the
+  volume-weighted centroid (modulo the periodic wrap) are conserved;
+* a merged shape is the summed second moments of its parts about the new
+  centroid (parallel-axis theorem, V/5 (B + 5 d d^T) per part), rescaled so
+  that det B = (3V/4pi)^2;
+* the nearest search is a brute-force minimum over the small parcel's
+  neighbourhood, periodic in x and y;
+* isolated dual links merge once, the later index into the earlier;
+* deleted parcels are replaced from the end of the container in order.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from epic_amd import workloads as W
+from tests._util import oracle_state
+
+
+def _shrink(st, idx, s):
+    """Scale volume by s and the shape by s^(2/3): consistent small parcels."""
+    st.volume[idx] *= s
+    st.B[idx] *= s ** (2.0 / 3.0)
+
+
+def _full(B, vol):
+    b33 = O.get_B33(B, vol)
+    return np.array([[B[0], B[1], B[2]], [B[1], B[3], B[4]], [B[2], B[4], b33]])
+
+
+def test_merge_conserves_volume_weighted_quantities():
+    mem = W.rising_thermal(16, seed=61, humidity_plume=True, n_blobs=2)
+    st = oracle_state(mem)
+    rng = np.random.default_rng(1)
+    idx = np.nonzero(rng.random(mem.n) < 0.08)[0]
+    _shrink(st, idx, 0.03)
+    V0 = st.volume.sum()
+    W0 = (st.volume[:, None] * st.vorticity).sum(0)
+    b0 = (st.volume * st.buoyancy).sum()
+    h0 = (st.volume * st.humidity).sum()
+    z0 = (st.volume * st.position[:, 2]).sum()
+    nm = st.parcel_merge()
+    assert nm > 0.8 * len(idx)
+    assert st.n == mem.n - nm
+    assert abs(st.volume.sum() - V0) <= 1e-13 * V0
+    assert np.allclose((st.volume[:, None] * st.vorticity).sum(0), W0, rtol=1e-12, atol=1e-12 * np.abs(W0).max())
+    assert abs((st.volume * st.buoyancy).sum() - b0) <= 1e-12 * max(1.0, abs(b0))
+    assert abs((st.volume * st.humidity).sum() - h0) <= 1e-12 * max(1.0, abs(h0))
+    assert abs((st.volume * st.position[:, 2]).sum() - z0) <= 1e-12 * V0
+    for n in range(st.n):
+        A = _full(st.B[n], st.volume[n])
+        assert np.linalg.eigvalsh(A).min() > 0
+
+
+def _lattice(n=8, seed=0):
+    """Large spheres on a jittered 8^3 lattice, one per cell."""
+    lower = (-0.5 * math.pi,) * 3
+    extent = (math.pi,) * 3
+    dx = extent[0] / n
+    rng = np.random.default_rng(seed)
+    pos = W.lattice_positions(n, n, n, lower, (dx,) * 3, 1) + rng.uniform(-0.2, 0.2, (n ** 3, 3)) * dx
+    vol = np.full(n ** 3, dx ** 3)
+    a2 = (0.75 * vol / math.pi) ** (2 / 3)
+    B = np.zeros((n ** 3, 5))
+    B[:, 0] = a2
+    B[:, 3] = a2
+    return lower, extent, dx, pos, vol, B
+
+
+def _state(lower, extent, pos, vol, B, vor=None, n=8):
+    dom = O.make_dom(n, n, n, lower, extent)
+    st = O.State(dom, len(vol))
+    st.position[:] = pos
+    st.volume[:] = vol
+    st.B[:] = B
+    st.vorticity[:] = 0.0 if vor is None else vor
+    return st
+
+
+def test_two_body_merge_is_the_parallel_axis_second_moment():
+    lower, extent, dx, pos, vol, B = _lattice()
+    # one small sphere next to parcel 300, well inside the domain
+    ic = 300
+    p_small = pos[ic] + np.array([0.3, -0.2, 0.1]) * dx
+    v_small = vol[0] / 40
+    a2s = (0.75 * v_small / math.pi) ** (2 / 3)
+    pos = np.vstack([pos, p_small])
+    vol = np.append(vol, v_small)
+    B = np.vstack([B, [a2s, 0, 0, a2s, 0]])
+    st = _state(lower, extent, pos, vol, B)
+    big_B, big_x, big_v = B[ic].copy(), pos[ic].copy(), vol[ic]
+    assert st.parcel_merge() == 1
+    assert st.n == len(vol) - 1   # the small parcel was the last one
+    V = big_v + v_small
+    xc = (big_v * big_x + v_small * p_small) / V
+    assert np.allclose(st.position[ic], xc, atol=1e-15)
+    assert st.volume[ic] == pytest.approx(V, rel=1e-15)
+    # summed second moments about the merged centroid (parallel-axis theorem)
+    S = np.zeros((3, 3))
+    for x, v, Bp in ((big_x, big_v, big_B), (p_small, v_small, np.array([a2s, 0, 0, a2s, 0]))):
+        d = x - xc
+        S += v / V * (_full(Bp, v) + 5.0 * np.outer(d, d))
+    abc = 0.75 * V / math.pi
+    S *= (abc ** 2 / np.linalg.det(S)) ** (1 / 3)
+    got = _full(st.B[ic], st.volume[ic])
+    assert np.allclose(got, S, rtol=1e-12, atol=1e-15)
+
+
+def test_small_parcel_merges_with_brute_force_nearest():
+    lower, extent, dx, pos, vol, B = _lattice(seed=3)
+    rng = np.random.default_rng(4)
+    n0 = len(vol)
+    # small parcels at random places (also next to the periodic edges)
+    ks = rng.choice(n0, 12, replace=False)
+    extra = pos[ks] + rng.uniform(-0.45, 0.45, (12, 3)) * dx
+    extra[:, 2] = np.clip(extra[:, 2], lower[2] + 0.01, lower[2] + extent[2] - 0.01)
+    for d in range(2):
+        extra[:, d] = lower[d] + np.mod(extra[:, d] - lower[d], extent[d])
+    v_small = vol[0] / 50
+    a2s = (0.75 * v_small / math.pi) ** (2 / 3)
+    pos2 = np.vstack([pos, extra])
+    vol2 = np.append(vol, np.full(12, v_small))
+    B2 = np.vstack([B, np.tile([a2s, 0, 0, a2s, 0], (12, 1))])
+    # expected partner: nearest parcel by periodic distance (the smalls are far
+    # apart, so every small one merges with its nearest large neighbour)
+    want = []
+    for s in range(n0, n0 + 12):
+        d = pos2 - pos2[s]
+        for k in range(2):
+            d[:, k] -= extent[k] * np.round(d[:, k] / extent[k])
+        r2 = (d ** 2).sum(1)
+        r2[s] = np.inf
+        want.append(int(np.argmin(r2)))
+    if len(set(want)) < 12 or any(w >= n0 for w in want):
+        pytest.skip("random layout produced a shared or small partner")
+    st = _state(lower, extent, pos2, vol2, B2)
+    vols_before = vol2.copy()
+    assert st.parcel_merge() == 12
+    assert st.n == n0
+    for s, w in zip(range(n0, n0 + 12), want):
+        assert st.volume[w] == pytest.approx(vols_before[w] + v_small, rel=1e-14), (s, w)
+
+
+def test_isolated_dual_link_merges_once_into_the_earlier_parcel():
+    lower, extent, dx, pos, vol, B = _lattice(seed=5)
+    n0 = len(vol)
+    # two small parcels close to each other, midway between lattice parcels
+    c = pos[200] + np.array([0.5, 0.5, 0.5]) * dx
+    pa, pb = c - np.array([0.02, 0, 0]) * dx, c + np.array([0.02, 0, 0]) * dx
+    v_small = vol[0] / 50
+    a2s = (0.75 * v_small / math.pi) ** (2 / 3)
+    pos2 = np.vstack([pos, pa, pb])
+    vol2 = np.append(vol, [v_small, v_small])
+    B2 = np.vstack([B, [a2s, 0, 0, a2s, 0], [a2s, 0, 0, a2s, 0]])
+    st = _state(lower, extent, pos2, vol2, B2)
+    assert st.parcel_merge() == 1
+    assert st.n == n0 + 1
+    # parcel n0 (the earlier one) survives with both volumes
+    assert st.volume[n0] == pytest.approx(2 * v_small, rel=1e-15)
+    assert np.allclose(st.position[n0], c, atol=1e-14)
+
+
+def test_delete_fills_holes_from_the_end():
+    lower, extent, dx, pos, vol, B = _lattice(seed=6)
+    n0 = len(vol)
+    # small parcels inserted at indices 10 and 20 (next to their neighbours)
+    v_small = vol[0] / 50
+    a2s = (0.75 * v_small / math.pi) ** (2 / 3)
+    pos2, vol2, B2 = pos.copy(), vol.copy(), B.copy()
+    for k in (10, 20):
+        pos2[k] = pos[k + 1] + np.array([0.1, 0.05, 0.0]) * dx
+        vol2[k] = v_small
+        B2[k] = [a2s, 0, 0, a2s, 0]
+    st = _state(lower, extent, pos2, vol2, B2)
+    assert st.parcel_merge() == 2
+    # holes 10, 20 get the last two parcels, the last one into the first hole
+    assert np.array_equal(st.position[10], pos2[n0 - 1])
+    assert np.array_equal(st.position[20], pos2[n0 - 2])
+    assert st.n == n0 - 2
