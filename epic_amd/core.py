@@ -27,7 +27,7 @@ FIELDS = {"volg": (0, 1), "vortg": (1, 3), "tbuoyg": (2, 1), "dbuoyg": (3, 1), "
 
 ERRORS = {-1: "EPIC_E_INVALID", -2: "EPIC_E_HIP", -3: "EPIC_E_PARCEL_COUNT", -4: "EPIC_E_B33",
           -5: "EPIC_E_OUT_OF_GRID", -6: "EPIC_E_RK_ORDER", -7: "EPIC_E_DT", -8: "EPIC_E_NOMEM",
-          -9: "EPIC_E_CAPACITY"}
+          -9: "EPIC_E_CAPACITY", -10: "EPIC_E_MERGE"}
 
 
 class EpicError(RuntimeError):
@@ -87,6 +87,7 @@ def load_library(path=LIB_PATH):
         "epic_get_timers": ([h, dp, ip, i], i),
         "epic_member_bytes": ([P(Grid), i64], i64),
         "epic_parcel_split": ([h, d, d, P(i64)], i),
+        "epic_parcel_merge": ([h, P(i64)], i),
         "epic_autotune": ([h, ip, ip], i),
         "epic_set_tiling": ([h, i, i], i),
         "epic_batch_create": ([P(h), i, P(h)], i),
@@ -246,6 +247,14 @@ class Member:
         self._chk(load_library().epic_parcel_split(self._h, lambda_max, amax, C.byref(ns)))
         self.n = int(load_library().epic_num_parcels(self._h))
         return ns.value
+
+    def parcel_merge(self):
+        """parcel_merge (parcel_merge.f90) on one rank; returns the number of
+        parcels merged away."""
+        nm = C.c_int64(0)
+        self._chk(load_library().epic_parcel_merge(self._h, C.byref(nm)))
+        self.n = int(load_library().epic_num_parcels(self._h))
+        return nm.value
 
     # ---- tiling ----
     def autotune(self):

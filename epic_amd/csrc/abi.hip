@@ -162,6 +162,7 @@ int error_code(epic_member *m, int e, const char *where)
     if (e & ERR_PARCEL_COUNT) return fail(m, EPIC_E_PARCEL_COUNT, w + ": par2grid: Wrong total number of parcels!");
     if (e & ERR_B33) return fail(m, EPIC_E_B33, w + ": get_B33: Division by small number!");
     if (e & ERR_OUT_OF_GRID) return fail(m, EPIC_E_OUT_OF_GRID, w + ": support point outside the node grid");
+    if (e & ERR_MERGE) return fail(m, EPIC_E_MERGE, w + ": Merge error: no near enough neighbour found.");
     return fail(m, EPIC_E_INVALID, w + ": internal consistency error");
 }
 
@@ -1329,6 +1330,40 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
         if (rc) return rc;
     }
     if (n_split) *n_split = total;
+    return EPIC_OK;
+}
+
+int epic_parcel_merge(epic_handle m, int64_t *n_merge)
+{
+    if (!m) return EPIC_E_INVALID;
+    if (n_merge) *n_merge = 0;
+    MergeCtx c{};
+    c.g = m->g;
+    c.p = m->pd;
+    c.vmin = m->g.vmin;
+    c.s = m->s;
+    c.ncell = (int64_t)(m->g.nx + 3) * (m->g.ny + 3) * (m->g.nz + 2);
+    c.err = m->err_d;
+    c.attr = m->attr;
+    c.cap = m->cap;
+    c.nattr = NATTR;
+    int64_t nm = 0;
+    const int r = run_merge(c, &nm);
+    if (r < 0) return fail(m, EPIC_E_HIP, "epic_parcel_merge: HIP failure");
+    int rc = check(m, "epic_parcel_merge");
+    if (rc) return rc;
+    if (nm > 0) {
+        m->n -= nm;
+        bind_parcels(m);
+        m->invalidate_plans();
+        m->keys_valid = false;
+        Plan Q;
+        build_axes(Q, {m}, {m->ax_stale});
+        HIPOK(m, Q.run(m->s));
+        rc = check(m, "epic_parcel_merge");
+        if (rc) return rc;
+    }
+    if (n_merge) *n_merge = nm;
     return EPIC_OK;
 }
 

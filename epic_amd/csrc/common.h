@@ -29,6 +29,7 @@ enum : int {
     ERR_B33 = 2,            // |B11 B22 - B12^2| <= eps
     ERR_OUT_OF_GRID = 4,    // support point outside the node grid / neighbour cells
     ERR_CELL_MISMATCH = 8,  // a sorted parcel is not in its sorted cell
+    ERR_MERGE = 16,         // a small parcel found no neighbour to merge with
 };
 
 // field slots inside Member::fld (FP32 node fields)

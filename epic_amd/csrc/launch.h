@@ -152,6 +152,21 @@ struct SplitArgs {
 void add_split_flag(Plan &P, int phase, const std::vector<SplitArgs> &items);
 void add_split_apply(Plan &P, int phase, const std::vector<SplitArgs> &items);
 
+// ---- parcel merging (merge.hip); direct launches on ctx.s ----
+struct MergeCtx {
+    Grid g;
+    ParcelsDev p;          // p.n: parcel count before the merge
+    double vmin;
+    hipStream_t s;
+    int64_t ncell;         // (nx+3)(ny+3)(nz+2) halo cells
+    int *err;
+    float *attr;           // the parcel attribute slab (nattr arrays of cap)
+    int64_t cap;
+    int nattr;
+};
+// 0: done (*n_merge merged), 1: stopped on a device error bit, -1: HIP failure
+int run_merge(const MergeCtx &c, int64_t *n_merge);
+
 // ---- utilities (abi.hip) ----
 struct FillArgs {
     int *p;

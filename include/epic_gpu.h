@@ -66,7 +66,8 @@ enum {
     EPIC_E_RK_ORDER = -6,      /* "Only third and fourth order RK supported." (ls_rk.f90:76-78) */
     EPIC_E_DT = -7,            /* time step must be positive (rk_utils.f90:179-182) */
     EPIC_E_NOMEM = -8,
-    EPIC_E_CAPACITY = -9       /* more parcels than max_parcels */
+    EPIC_E_CAPACITY = -9,      /* more parcels than max_parcels */
+    EPIC_E_MERGE = -10         /* "Merge error: no near enough neighbour found." (parcel_nearest.f90:551-556) */
 };
 
 /* field selectors for epic_get_field / epic_set_field (fields.f90:21-51) */
@@ -134,6 +135,17 @@ int epic_ls_rk_step(epic_handle h, double *t, double dt, double t_initial);
  * nothing (the reference grows its container instead).  The cached
  * support-point axes are recomputed from the new shapes. */
 int epic_parcel_split(epic_handle h, double lambda_max, double amax, int64_t *n_split);
+
+/* parcel_merge (parcel_merge.f90:48-109 with find_nearest,
+ * parcel_nearest.f90:375-627, 937-1377) with the semantics of the reference
+ * on one MPI rank: every parcel with volume < vmin merges into its nearest
+ * neighbour (periodic images of boundary small parcels included) as the
+ * tree resolution allows; mergers conserve volume, volume-weighted
+ * vorticity, buoyancy and humidity, take the volume-weighted centre and the
+ * summed second moments rescaled to det B = (3V/4pi)^2.  Merged parcels are
+ * deleted, their slots filled from the end of the container.  The cached
+ * support-point axes are recomputed.  *n_merge: number of parcels removed. */
+int epic_parcel_merge(epic_handle h, int64_t *n_merge);
 
 /* Cell counting sort of the parcels' current positions with the reference's
  * key ijk = 1 + ix + hx iy + hx hy iz on the halo box
