@@ -2056,3 +2056,175 @@ int orc_parcel_merge(const orc_dom *d, orc_parcels *p, int64_t *n_merge_out)
     free(iclo); free(dclo); free(leaf); free(avail); free(merged); free(dom_); free(iso); free(isma); free(ic_m);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* parcel_correction.f90: apply_laplace (:137-172) with diverge             */
+/* (inversion.f90:410-455) and lapinv1 (inversion_utils.f90:565-589, tables */
+/* :240-271); apply_gradient (:175-258); apply_vortcor (:91-133)            */
+/* ------------------------------------------------------------------------ */
+/* Lap(phi) = rhs semi-spectrally: 2nd-order differences in z, FP64
+ * tridiagonal elimination per horizontal wavenumber; the (0,0) mode is zero */
+static void lapinv1(const orc_dom *d, const orc_inv *v, double *fs)
+{
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const double dzisq = d->dxi[2] * d->dxi[2], ap = dzisq;
+#pragma omp parallel for schedule(static) num_threads(nthreads_of(d))
+    for (int kx = 0; kx < nx; ++kx) {
+        double *etdv = (double *)malloc(sizeof(double) * (nz + 1));
+        double *htdv = (double *)malloc(sizeof(double) * (nz + 1));
+        double *rs = (double *)malloc(sizeof(double) * (nz + 1));
+        for (int ky = 0; ky < ny; ++ky) {
+            double *f = fs + SIDX(d, 0, ky, kx);
+            if (kx == 0 && ky == 0) {
+                for (int iz = 0; iz <= nz; ++iz) f[iz] = zero;
+                continue;
+            }
+            const double a0 = -two * dzisq - v->k2l2[ky + ny * kx];
+            htdv[0] = one / a0;
+            etdv[0] = -two * ap * htdv[0];
+            for (int iz = 1; iz <= nz - 1; ++iz) {
+                htdv[iz] = one / (a0 + ap * etdv[iz - 1]);
+                etdv[iz] = -ap * htdv[iz];
+            }
+            htdv[nz] = one / (a0 + two * ap * etdv[nz - 1]);
+            for (int iz = 0; iz <= nz; ++iz) rs[iz] = f[iz];
+            f[0] = rs[0] * htdv[0];
+            for (int iz = 1; iz <= nz - 1; ++iz) f[iz] = (rs[iz] - ap * f[iz - 1]) * htdv[iz];
+            f[nz] = (rs[nz] - two * ap * f[nz - 1]) * htdv[nz];
+            for (int iz = nz - 1; iz >= 0; --iz) f[iz] = etdv[iz] * f[iz + 1] + f[iz];
+        }
+        free(etdv);
+        free(htdv);
+        free(rs);
+    }
+}
+
+/* parcel correction field: grad phi with Lap phi = volg/vcell - 1 */
+static int laplace_grad(const orc_dom *d, orc_inv *v, const double *volg, double *grad)
+{
+    const int64_t H = orc_halo_size(d), G = orc_spec_size(d);
+    double *phi = (double *)malloc(sizeof(double) * H);
+    double *ds = (double *)malloc(sizeof(double) * G), *ws = (double *)malloc(sizeof(double) * G);
+    for (int64_t i = 0; i < H; ++i) phi[i] = volg[i] * (one / d->vcell) - one;
+    int rc = orc_fftxyp2s(d, phi, ds);
+    if (rc == ORC_OK) {
+        for (int iz = 0; iz <= d->nz; ++iz) ds[SIDX(d, iz, 0, 0)] = zero;
+        lapinv1(d, v, ds);
+        orc_diffx(d, ds, ws);
+        orc_fftxys2p(d, ws, grad + 0 * H);
+        orc_diffy(d, ds, ws);
+        orc_fftxys2p(d, ws, grad + 1 * H);
+        central_diffz_ss(d, ds, ws);
+        for (int kx = 0; kx < d->nx; ++kx)
+            for (int ky = 0; ky < d->ny; ++ky) {
+                ws[SIDX(d, 0, ky, kx)] = zero;
+                ws[SIDX(d, d->nz, ky, kx)] = zero;
+            }
+        orc_fftxys2p(d, ws, grad + 2 * H);
+        for (int c = 0; c < 3; ++c) {
+            halo_fill(d, grad + c * H);
+            extrap_lin(d, grad + c * H);   /* grad(-1) = 2 grad(0) - grad(1), likewise at nz+1 */
+        }
+    }
+    free(phi);
+    free(ds);
+    free(ws);
+    return rc;
+}
+
+int orc_apply_laplace(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, int l_reuse)
+{
+    int rc = orc_par2grid(d, p, f, l_reuse);   /* volg as vol2grid forms it (parcel_interpl.f90:92-136) */
+    if (rc) return rc;
+    const int64_t H = orc_halo_size(d);
+    double *grad = (double *)calloc((size_t)(3 * H), sizeof(double));
+    rc = laplace_grad(d, v, f->volg, grad);
+    if (rc == ORC_OK) {
+        static const int ord[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+        for (int64_t n = 0; n < p->n; ++n) {
+            double *x = p->position + 3 * n;
+            int ijk[3];
+            double w[8];
+            orc_trilinear(d, x, ijk, w);
+            int64_t hs[8];
+            for (int kk = 0; kk < 2; ++kk)
+                for (int jj = 0; jj < 2; ++jj)
+                    for (int ii = 0; ii < 2; ++ii) hs[kk * 4 + jj * 2 + ii] = HIDX(d, ijk[2] + kk, ijk[1] + jj, ijk[0] + ii);
+            double s[3];
+            for (int l = 0; l < 3; ++l) {
+                s[l] = 0.0;
+                for (int m = 0; m < 8; ++m) s[l] += w[ord[m]] * grad[l * H + hs[ord[m]]];
+            }
+            for (int l = 0; l < 3; ++l) x[l] = x[l] + s[l];
+            orc_apply_periodic_bc(d, x);   /* parcel_communicate */
+        }
+    }
+    free(grad);
+    return rc;
+}
+
+int orc_apply_gradient(const orc_dom *d, orc_parcels *p, orc_fields *f, double prefactor, double max_compression,
+                       int l_reuse)
+{
+    int rc = orc_par2grid(d, p, f, l_reuse);
+    if (rc) return rc;
+    const double vci = one / d->vcell;
+#define PHI(k, j, i) (f->volg[HIDX(d, k, j, i)] * vci - one)
+    for (int64_t n = 0; n < p->n; ++n) {
+        double *x = p->position + 3 * n;
+        int ijk[3];
+        double w[8];
+        orc_trilinear(d, x, ijk, w);
+        const int is = ijk[0], js = ijk[1], ks = ijk[2];
+        const double xf = w[1] + w[3] + w[5] + w[7];
+        const double yf = w[2] + w[3] + w[6] + w[7];
+        const double zf = w[4] + w[5] + w[6] + w[7];
+        const double xfc = one - xf, yfc = one - yf, zfc = one - zf;
+        double lim_x = d->dx[0] * xf * xfc;
+        double xs = -prefactor * lim_x *
+                    (zfc * (yfc * (PHI(ks, js, is + 1) - PHI(ks, js, is)) + yf * (PHI(ks, js + 1, is + 1) - PHI(ks, js + 1, is))) +
+                     zf * (yfc * (PHI(ks + 1, js, is + 1) - PHI(ks + 1, js, is)) +
+                           yf * (PHI(ks + 1, js + 1, is + 1) - PHI(ks + 1, js + 1, is))));
+        lim_x = lim_x * max_compression;
+        xs = fmax(-lim_x, fmin(xs, lim_x));
+        double lim_y = d->dx[1] * yf * yfc;
+        double ys = -prefactor * lim_y *
+                    (zfc * (xfc * (PHI(ks, js + 1, is) - PHI(ks, js, is)) + xf * (PHI(ks, js + 1, is + 1) - PHI(ks, js, is + 1))) +
+                     zf * (xfc * (PHI(ks + 1, js + 1, is) - PHI(ks + 1, js, is)) +
+                           xf * (PHI(ks + 1, js + 1, is + 1) - PHI(ks + 1, js, is + 1))));
+        lim_y = lim_y * max_compression;
+        ys = fmax(-lim_y, fmin(ys, lim_y));
+        double lim_z = d->dx[2] * zf * zfc;
+        double zs = -prefactor * lim_z *
+                    (xfc * (yfc * (PHI(ks + 1, js, is) - PHI(ks, js, is)) + yf * (PHI(ks + 1, js + 1, is) - PHI(ks, js + 1, is))) +
+                     xf * (yfc * (PHI(ks + 1, js, is + 1) - PHI(ks, js, is + 1)) +
+                           yf * (PHI(ks + 1, js + 1, is + 1) - PHI(ks, js + 1, is + 1))));
+        lim_z = lim_z * max_compression;
+        zs = fmax(-lim_z, fmin(zs, lim_z));
+        x[0] = x[0] + xs;
+        x[1] = x[1] + ys;
+        x[2] = x[2] + zs;
+        orc_apply_periodic_bc(d, x);
+    }
+#undef PHI
+    return ORC_OK;
+}
+
+void orc_volume_weighted_vorticity(const orc_parcels *p, double vor_bar[3])
+{
+    double vs = zero, w[3] = {zero, zero, zero};
+    for (int64_t n = 0; n < p->n; ++n) {
+        vs = vs + p->volume[n];
+        for (int c = 0; c < 3; ++c) w[c] = w[c] + p->vorticity[3 * n + c] * p->volume[n];
+    }
+    for (int c = 0; c < 3; ++c) vor_bar[c] = w[c] / vs;
+}
+
+void orc_apply_vortcor(orc_parcels *p, const double vor_bar[3])
+{
+    double now[3];
+    orc_volume_weighted_vorticity(p, now);
+    for (int c = 0; c < 3; ++c) now[c] = now[c] - vor_bar[c];
+    for (int64_t n = 0; n < p->n; ++n)
+        for (int c = 0; c < 3; ++c) p->vorticity[3 * n + c] = p->vorticity[3 * n + c] - now[c];
+}

@@ -136,6 +136,13 @@ int orc_parcel_split(const orc_dom *d, orc_parcels *p, int64_t cap, double lambd
 /* ---- parcel merge (parcel_merge.f90, parcel_nearest.f90), one rank ---- */
 int orc_parcel_merge(const orc_dom *d, orc_parcels *p, int64_t *n_merge);
 
+/* ---- parcel correction (parcel_correction.f90) ---- */
+int orc_apply_laplace(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, int l_reuse);
+int orc_apply_gradient(const orc_dom *d, orc_parcels *p, orc_fields *f, double prefactor, double max_compression,
+                       int l_reuse);
+void orc_volume_weighted_vorticity(const orc_parcels *p, double vor_bar[3]);
+void orc_apply_vortcor(orc_parcels *p, const double vor_bar[3]);
+
 int64_t orc_nearest_ncell(const orc_dom *d);
 int orc_nearest_cell_sort(const orc_dom *d, int64_t n, const double *position,
                           int *loca, int *nppc, int *kc1, int *node);

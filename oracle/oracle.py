@@ -126,6 +126,12 @@ def _declare(L):
     L.orc_default_amax.restype = d
     L.orc_parcel_split.argtypes = [P(Dom), P(Parcels), C.c_int64, d, d, P(C.c_int64)]
     L.orc_parcel_merge.argtypes = [P(Dom), P(Parcels), P(C.c_int64)]
+    L.orc_apply_laplace.argtypes = [P(Dom), C.c_void_p, P(Parcels), P(Fields), C.c_int]
+    L.orc_apply_gradient.argtypes = [P(Dom), P(Parcels), P(Fields), d, d, C.c_int]
+    L.orc_volume_weighted_vorticity.argtypes = [P(Parcels), P(d)]
+    L.orc_volume_weighted_vorticity.restype = None
+    L.orc_apply_vortcor.argtypes = [P(Parcels), P(d)]
+    L.orc_apply_vortcor.restype = None
     L.orc_nearest_ncell.argtypes = [P(Dom)]
     L.orc_nearest_ncell.restype = C.c_int64
     L.orc_nearest_cell_sort.argtypes = [P(Dom), C.c_int64, P(d), P(C.c_int), P(C.c_int), P(C.c_int),
@@ -316,6 +322,27 @@ class State:
         self._trim()
         _chk(rc)
         return nm.value
+
+    def apply_laplace(self, l_reuse=None):
+        """apply_laplace (parcel_correction.f90:137-172)."""
+        flag = -1 if l_reuse is None else int(bool(l_reuse))
+        _chk(lib().orc_apply_laplace(C.byref(self.dom), self.inv(), C.byref(self._parcels()),
+                                     C.byref(self._fields()), flag))
+
+    def apply_gradient(self, prefactor=1.8, max_compression=0.5, l_reuse=None):
+        """apply_gradient (parcel_correction.f90:175-258)."""
+        flag = -1 if l_reuse is None else int(bool(l_reuse))
+        _chk(lib().orc_apply_gradient(C.byref(self.dom), C.byref(self._parcels()), C.byref(self._fields()),
+                                      prefactor, max_compression, flag))
+
+    def volume_weighted_vorticity(self):
+        out = np.zeros(3)
+        lib().orc_volume_weighted_vorticity(C.byref(self._parcels()), ptr(out))
+        return out
+
+    def apply_vortcor(self, vor_bar):
+        vb = np.ascontiguousarray(vor_bar, dtype=np.float64)
+        lib().orc_apply_vortcor(C.byref(self._parcels()), ptr(vb))
 
     def default_amax(self):
         return lib().orc_default_amax(C.byref(self.dom))
