@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libepic_gpu.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["abi.hip", "plan.hip", "p2g.hip", "g2p.hip", "fft.hip", "inversion.hip", "split.hip", "merge.hip"]
+SOURCES = ["abi.hip", "plan.hip", "p2g.hip", "g2p.hip", "fft.hip", "inversion.hip", "split.hip", "merge.hip",
+           "correction.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -88,6 +88,10 @@ def load_library(path=LIB_PATH):
         "epic_member_bytes": ([P(Grid), i64], i64),
         "epic_parcel_split": ([h, d, d, P(i64)], i),
         "epic_parcel_merge": ([h, P(i64)], i),
+        "epic_apply_laplace": ([h, i], i),
+        "epic_apply_gradient": ([h, d, d, i], i),
+        "epic_init_parcel_correction": ([h, dp], i),
+        "epic_apply_vortcor": ([h], i),
         "epic_autotune": ([h, ip, ip], i),
         "epic_set_tiling": ([h, i, i], i),
         "epic_batch_create": ([P(h), i, P(h)], i),
@@ -255,6 +259,27 @@ class Member:
         self._chk(load_library().epic_parcel_merge(self._h, C.byref(nm)))
         self.n = int(load_library().epic_num_parcels(self._h))
         return nm.value
+
+    # ---- parcel correction (parcel_correction.f90) ----
+    def apply_laplace(self, l_reuse=None):
+        """apply_laplace (parcel_correction.f90:142-181)."""
+        self._chk(load_library().epic_apply_laplace(self._h, _opt(l_reuse)))
+
+    def apply_gradient(self, prefactor=1.8, max_compression=0.5, l_reuse=None):
+        """apply_gradient (parcel_correction.f90:183-259); defaults of
+        options.f90:92-93."""
+        self._chk(load_library().epic_apply_gradient(self._h, prefactor, max_compression, _opt(l_reuse)))
+
+    def init_parcel_correction(self):
+        """init_parcel_correction (parcel_correction.f90:42-83); returns the
+        stored volume-weighted mean vorticity."""
+        out = np.zeros(3)
+        self._chk(load_library().epic_init_parcel_correction(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def apply_vortcor(self):
+        """apply_vortcor (parcel_correction.f90:89-138)."""
+        self._chk(load_library().epic_apply_vortcor(self._h))
 
     # ---- tiling ----
     def autotune(self):

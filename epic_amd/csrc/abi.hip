@@ -74,7 +74,13 @@ struct epic_member {
     std::string msg;
     // launch plans of the call surface (built on first use)
     Plan pl_keys, pl_p2g_fresh, pl_p2g_reuse, pl_vor2vel, pl_vtend, pl_g2p[2], pl_step;
+    Plan pl_lap[2];   // parcel correction: forward / inverse transforms
     double step_dt = -1.0;
+    // parcel correction: the target volume-weighted vorticity and the FP64
+    // reduction scratch (allocated on first use)
+    double vor_bar[3] = {0.0, 0.0, 0.0};
+    double *vw_d = nullptr;
+    double *vw_h = nullptr;
     uint64_t gen = 0;   // bumped whenever plans built over this member go stale
     void invalidate_plans()
     {
@@ -87,6 +93,8 @@ struct epic_member {
         pl_g2p[0].release();
         pl_g2p[1].release();
         pl_step.release();
+        pl_lap[0].release();
+        pl_lap[1].release();
         step_dt = -1.0;
     }
 };
@@ -568,6 +576,37 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
 }
 
 // enqueue a member plan on the member stream (event timers when profiling)
+// parcel correction transforms: phi = volg / vcell - 1 to its spectrum
+// (forward), the three gradient spectra back to F_GRAD (inverse)
+void build_laplace(epic_member *m)
+{
+    FftXArgs x = xargs(m);
+    x.mode = 3;
+    x.out = m->spec[0];
+    add_fft_xf(m->pl_lap[0], PH_OTHER, {x});
+    FftYArgs y = yargs(m);
+    y.mode = 0;
+    y.in = m->spec[0];
+    y.out = m->spec[0];
+    add_fft_y(m->pl_lap[0], PH_OTHER, {y});
+    std::vector<FftYArgs> yi;
+    std::vector<FftXArgs> xi;
+    for (int c = 0; c < 3; ++c) {
+        FftYArgs b = yargs(m);
+        b.mode = 1;
+        b.in = m->spec[1 + c];
+        b.out = m->spec[1 + c];
+        yi.push_back(b);
+        FftXArgs a = xargs(m);
+        a.mode = 0;
+        a.slot = F_GRAD + c;
+        a.in = m->spec[1 + c];
+        xi.push_back(a);
+    }
+    add_fft_y(m->pl_lap[1], PH_OTHER, yi);
+    add_fft_xi(m->pl_lap[1], PH_OTHER, xi);
+}
+
 int run_plan(epic_member *m, Plan &P)
 {
     hipError_t e = m->profiling ? P.run(m->s, &m->ev, &m->ev_phase) : P.run(m->s);
@@ -754,12 +793,13 @@ int epic_destroy(epic_handle m)
     m->invalidate_plans();
     void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
                     m->cnt[2], m->scnt[2], m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
-                    m->twx, m->twy, m->twz, m->err_d, m->flags_d};
+                    m->twx, m->twy, m->twz, m->err_d, m->flags_d, m->vw_d};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < NSPEC; ++i)
         if (m->spec[i]) (void)hipFree(m->spec[i]);
     if (m->err_h) (void)hipHostFree(m->err_h);
+    if (m->vw_h) (void)hipHostFree(m->vw_h);
     if (m->s) (void)hipStreamDestroy(m->s);
     delete m;
     return EPIC_OK;
@@ -1365,6 +1405,86 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
     }
     if (n_merge) *n_merge = nm;
     return EPIC_OK;
+}
+
+int epic_apply_laplace(epic_handle m, int l_reuse)
+{
+    if (!m) return EPIC_E_INVALID;
+    int rc = epic_par2grid(m, l_reuse);   // vol2grid (parcel_correction.f90:151)
+    if (rc) return rc;
+    if (m->n == 0) return EPIC_OK;
+    const Grid &g = m->g;
+    if (m->pl_lap[0].empty()) build_laplace(m);
+    rc = run_plan(m, m->pl_lap[0]);
+    if (rc) return rc;
+    // spectra 4..6 are free between calls: the FP64 tridiagonal scratch
+    launch_lapinv_columns(g, m->spec[0], (double *)m->spec[4], (double *)m->spec[5], (double *)m->spec[6], m->spec[1],
+                          m->spec[2], m->spec[3], m->s);
+    HIPOK(m, hipGetLastError());
+    rc = run_plan(m, m->pl_lap[1]);
+    if (rc) return rc;
+    launch_grad_halo(g, m->fld, m->s);
+    launch_laplace_push(g, m->pd, m->fld, m->err_d, m->s);
+    HIPOK(m, hipGetLastError());
+    m->keys_valid = false;
+    return check(m, "epic_apply_laplace");
+}
+
+int epic_apply_gradient(epic_handle m, double prefactor, double max_compression, int l_reuse)
+{
+    if (!m) return EPIC_E_INVALID;
+    int rc = epic_par2grid(m, l_reuse);   // vol2grid (parcel_correction.f90:194)
+    if (rc) return rc;
+    if (m->n == 0) return EPIC_OK;
+    launch_gradient_push(m->g, m->pd, m->fld, prefactor, max_compression, m->err_d, m->s);
+    HIPOK(m, hipGetLastError());
+    m->keys_valid = false;
+    return check(m, "epic_apply_gradient");
+}
+
+// sum vol and sum vol * vorticity over the parcels, FP64 in a fixed order
+static int volume_weighted_sums(epic_member *m, double out[4])
+{
+    if (!m->vw_d) {
+        HIPOK(m, dalloc(&m->vw_d, vw_partials(m->cap) + 4));
+        HIPOK(m, hipHostMalloc((void **)&m->vw_h, sizeof(double) * 4, hipHostMallocDefault));
+    }
+    double *tot = m->vw_d + vw_partials(m->cap);
+    launch_volume_weighted_vorticity(m->pd, m->vw_d, tot, m->s);
+    HIPOK(m, hipGetLastError());
+    HIPOK(m, hipMemcpyAsync(m->vw_h, tot, sizeof(double) * 4, hipMemcpyDeviceToHost, m->s));
+    HIPOK(m, hipStreamSynchronize(m->s));
+    for (int c = 0; c < 4; ++c) out[c] = m->vw_h[c];
+    return EPIC_OK;
+}
+
+int epic_init_parcel_correction(epic_handle m, double *vor_bar)
+{
+    if (!m) return EPIC_E_INVALID;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    if (m->n > 0) {
+        const int rc = volume_weighted_sums(m, s);
+        if (rc) return rc;
+    }
+    for (int c = 0; c < 3; ++c) m->vor_bar[c] = (s[0] > 0.0) ? s[1 + c] / s[0] : 0.0;
+    if (vor_bar)
+        for (int c = 0; c < 3; ++c) vor_bar[c] = m->vor_bar[c];
+    return EPIC_OK;
+}
+
+int epic_apply_vortcor(epic_handle m)
+{
+    if (!m) return EPIC_E_INVALID;
+    if (m->n == 0) return EPIC_OK;
+    double s[4];
+    const int rc = volume_weighted_sums(m, s);
+    if (rc) return rc;
+    if (!(s[0] > 0.0)) return fail(m, EPIC_E_INVALID, "epic_apply_vortcor: total parcel volume is not positive");
+    double d[3];
+    for (int c = 0; c < 3; ++c) d[c] = s[1 + c] / s[0] - m->vor_bar[c];
+    launch_vortcor(m->pd, d, m->s);
+    HIPOK(m, hipGetLastError());
+    return check(m, "epic_apply_vortcor");
 }
 
 int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)

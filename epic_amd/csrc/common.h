@@ -42,7 +42,8 @@ enum FieldSlot : int {
     F_VELOG = 7,    // 3
     F_VELGRADG = 10, // 8: dudx dudy dudz dvdx dvdy dvdz dwdx dwdy
     F_VTEND = 18,   // 3
-    F_COUNT = 21
+    F_GRAD = 21,    // 3: the parcel-correction displacement field (grad phi)
+    F_COUNT = 24
 };
 
 enum { DUDX = 0, DUDY, DUDZ, DVDX, DVDY, DVDZ, DWDX, DWDY };

@@ -27,6 +27,8 @@ __device__ inline float load_phys(const FftXArgs &a, int k, int j, int i)
     const int64_t FS = field_stride(g), PS = plane_stride(g);
     const int64_t o = (int64_t)(k + 1) * PS + (int64_t)j * g.nx + i;
     if (a.mode == 0) return a.fld[a.slot * FS + o];
+    // parcel-correction source: volg / vcell - 1
+    if (a.mode == 3) return a.fld[F_VOLG * FS + o] * (float)(1.0 / g.vcell) - 1.0f;
     // vorticity-tendency flux: (omega_a + f_a) * vel_b + sgn * tbuoy
     float v = (a.fld[(F_VORTG + a.comp_a) * FS + o] + (float)g.f_cor[a.comp_a]) * a.fld[(F_VELOG + a.comp_b) * FS + o];
     if (a.sgn != 0) v += (float)a.sgn * a.fld[F_TBUOYG * FS + o];

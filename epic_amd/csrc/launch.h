@@ -103,7 +103,8 @@ struct FftXArgs {
     Grid g;
     FftPlan plan;          // complex length nx/2
     const float2 *tw;      // e^{-2 pi i t / nx}, t < nx
-    int mode;              // forward: 0 field, 1 tendency flux; inverse: 0 store, 2 store + d/dz flux
+    int mode;              // forward: 0 field, 1 tendency flux, 3 volg / vcell - 1;
+                           // inverse: 0 store, 2 store + d/dz flux
     int slot;              // field slot read (forward, mode 0) or written (inverse)
     int comp_a, comp_b, sgn;
     const float *fld;      // node fields read
@@ -166,6 +167,20 @@ struct MergeCtx {
 };
 // 0: done (*n_merge merged), 1: stopped on a device error bit, -1: HIP failure
 int run_merge(const MergeCtx &c, int64_t *n_merge);
+
+// ---- parcel correction (correction.hip); direct launches on s ----
+// the FP64 tridiagonal solve of every spectral column and its three gradient
+// spectra; fr, fi, et hold (nz + 1) ny (nx/2 + 1) doubles each
+void launch_lapinv_columns(const Grid &g, const float2 *in, double *fr, double *fi, double *et, float2 *ox, float2 *oy,
+                           float2 *oz, hipStream_t s);
+void launch_grad_halo(const Grid &g, float *fld, hipStream_t s);
+void launch_laplace_push(const Grid &g, const ParcelsDev &p, const float *fld, int *err, hipStream_t s);
+void launch_gradient_push(const Grid &g, const ParcelsDev &p, const float *fld, double prefactor,
+                          double max_compression, int *err, hipStream_t s);
+int64_t vw_partials(int64_t n);   // doubles of scratch launch_volume_weighted_vorticity needs
+// out[0] = sum vol, out[1..3] = sum vol * vorticity (FP64, fixed order)
+void launch_volume_weighted_vorticity(const ParcelsDev &p, double *part, double *out, hipStream_t s);
+void launch_vortcor(const ParcelsDev &p, const double d[3], hipStream_t s);
 
 // ---- utilities (abi.hip) ----
 struct FillArgs {

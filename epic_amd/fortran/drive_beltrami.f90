@@ -2,10 +2,11 @@
 ! loop (epic3d.f90:128-161) would: a 16^3 Beltrami member on
 ! [-pi/2, pi/2]^3 with a 2x2x2 lattice of spherical parcels
 ! (parcel_init.f90:31-152), vorticity omega = alpha u sampled at the parcel
-! centres (beltrami_3d.f90:103-122), then two fixed-dt RK steps.  Prints the
-! final time, the parcel count and checksums of the final parcel state and of
-! the gridded volume, which a caller compares with the C-ABI run of the same
-! member.
+! centres (beltrami_3d.f90:103-122), then two iterations of the run loop:
+! apply_vortcor, a fixed-dt RK step, parcel_merge, parcel_split and two
+! rounds of apply_laplace / apply_gradient.  Prints the final time, the
+! parcel count and checksums of the final parcel state and of the gridded
+! volume, which a caller compares with the C-ABI run of the same member.
 program drive_beltrami
     use iso_c_binding, only : c_int64_t
     use epic_gpu_mod
@@ -16,7 +17,7 @@ program drive_beltrami
     double precision :: lower(3), extent(3), dx(3), vcell, abc, l23, x, y, z, alpha, fk2l2, t
     double precision, allocatable :: pos(:, :), vor(:, :), B(:, :), vol(:), buoy(:), hum(:)
     double precision :: volg(-1:nz+1, -1:ny+1, -1:nx+1)
-    integer :: ix, iy, iz, i, j, k, l, step
+    integer :: ix, iy, iz, i, j, k, l, step, cor_iter, m
 
     lower = -0.5d0 * pi
     extent = pi
@@ -60,12 +61,25 @@ program drive_beltrami
     hum = 0.0d0
 
     call epic_gpu_init(nx, ny, nz, lower, extent, 0.0d0, 0.0d0, 0.0d0, (/0.0d0, 0.0d0, 0.0d0/), &
-                       (/.true., .true./), 4, int(n, c_int64_t), 0.01d0, 0.0d0)
+                       (/.true., .true./), 4, int(2 * n, c_int64_t), 0.01d0, 0.0d0)
     call epic_gpu_upload_parcels(n, pos, vor, B, vol, buoy, hum)
+    call init_parcel_correction
     t = 0.0d0
     do step = 1, 2
+        call apply_vortcor
         call ls_rk_step(t)
+        call parcel_merge
+        call parcel_split
+        do cor_iter = 1, 2
+            call apply_laplace((cor_iter > 1))
+            call apply_gradient(1.8d0, 0.5d0, .true.)
+        enddo
     enddo
+    m = epic_gpu_num_parcels()
+    if (m /= n) then
+        write(*, *) "parcel count changed: ", m
+        error stop 1
+    endif
     call epic_gpu_download_parcels(n, pos, vor, B, vol, buoy, hum)
     call par2grid
     call epic_gpu_get_field(EPIC_FIELD_VOLG, volg)

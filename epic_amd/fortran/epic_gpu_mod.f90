@@ -9,6 +9,12 @@
 !   vorticity_tendency()    inversion.f90:321
 !   ls_rk_step(t)           ls_rk.f90:88   (fixed time step)
 !   parcel_split()          parcel_split.f90:37
+!   parcel_merge()          parcel_merge.f90:48
+!   init_parcel_correction  parcel_correction.f90:42
+!   apply_vortcor           parcel_correction.f90:89
+!   apply_laplace(l_reuse)  parcel_correction.f90:142
+!   apply_gradient(prefactor, max_compression, l_reuse)
+!                           parcel_correction.f90:183
 ! Parcel and field arrays keep the reference's layouts: position(3, n),
 ! vorticity(3, n), B(5, n), volume(n), ... and f(-1:nz+1, -1:ny+1, -1:nx+1).
 ! A non-zero status from the library stops the run with its message, as the
@@ -105,6 +111,32 @@ module epic_gpu_mod
             real(c_double), value           :: lambda_max, amax
             integer(c_int64_t), intent(out) :: n_split
         end function
+        integer(c_int) function c_epic_parcel_merge(h, n_merge) bind(C, name="epic_parcel_merge")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value              :: h
+            integer(c_int64_t), intent(out) :: n_merge
+        end function
+        integer(c_int) function c_epic_apply_laplace(h, l_reuse) bind(C, name="epic_apply_laplace")
+            import :: c_int, c_ptr
+            type(c_ptr), value    :: h
+            integer(c_int), value :: l_reuse
+        end function
+        integer(c_int) function c_epic_apply_gradient(h, prefactor, max_compression, l_reuse) &
+                bind(C, name="epic_apply_gradient")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value    :: h
+            real(c_double), value :: prefactor, max_compression
+            integer(c_int), value :: l_reuse
+        end function
+        integer(c_int) function c_epic_init_parcel_correction(h, vor_bar) bind(C, name="epic_init_parcel_correction")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value          :: h
+            real(c_double), intent(out) :: vor_bar(3)
+        end function
+        integer(c_int) function c_epic_apply_vortcor(h) bind(C, name="epic_apply_vortcor")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
         integer(c_int64_t) function c_epic_num_parcels(h) bind(C, name="epic_num_parcels")
             import :: c_int64_t, c_ptr
             type(c_ptr), value :: h
@@ -122,7 +154,8 @@ module epic_gpu_mod
 
     public :: epic_gpu_init, epic_gpu_finalise, epic_gpu_upload_parcels, epic_gpu_download_parcels, &
               epic_gpu_get_field, epic_gpu_num_parcels, par2grid, grid2par, vor2vel, vorticity_tendency, &
-              ls_rk_step, parcel_split
+              ls_rk_step, parcel_split, parcel_merge, init_parcel_correction, apply_vortcor, apply_laplace, &
+              apply_gradient
 
 contains
 
@@ -238,6 +271,38 @@ contains
         call check(c_epic_parcel_split(handle, lam, 0.0d0, n_split), "parcel_split")
         num_parcels = c_epic_num_parcels(handle)
     end subroutine parcel_split
+
+    subroutine parcel_merge
+        integer(c_int64_t) :: n_merge
+        call check(c_epic_parcel_merge(handle, n_merge), "parcel_merge")
+        num_parcels = c_epic_num_parcels(handle)
+    end subroutine parcel_merge
+
+    subroutine init_parcel_correction
+        real(c_double) :: vor_bar(3)
+        call check(c_epic_init_parcel_correction(handle, vor_bar), "init_parcel_correction")
+    end subroutine init_parcel_correction
+
+    subroutine apply_vortcor
+        call check(c_epic_apply_vortcor(handle), "apply_vortcor")
+    end subroutine apply_vortcor
+
+    subroutine apply_laplace(l_reuse)
+        logical, optional, intent(in) :: l_reuse
+        integer(c_int) :: flag
+        flag = -1
+        if (present(l_reuse)) flag = merge(1, 0, l_reuse)
+        call check(c_epic_apply_laplace(handle, flag), "apply_laplace")
+    end subroutine apply_laplace
+
+    subroutine apply_gradient(prefactor, max_compression, l_reuse)
+        double precision, intent(in)  :: prefactor, max_compression
+        logical, optional, intent(in) :: l_reuse
+        integer(c_int) :: flag
+        flag = -1
+        if (present(l_reuse)) flag = merge(1, 0, l_reuse)
+        call check(c_epic_apply_gradient(handle, prefactor, max_compression, flag), "apply_gradient")
+    end subroutine apply_gradient
 
     integer function epic_gpu_num_parcels()
         epic_gpu_num_parcels = int(c_epic_num_parcels(handle))

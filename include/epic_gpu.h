@@ -147,6 +147,28 @@ int epic_parcel_split(epic_handle h, double lambda_max, double amax, int64_t *n_
  * support-point axes are recomputed.  *n_merge: number of parcels removed. */
 int epic_parcel_merge(epic_handle h, int64_t *n_merge);
 
+/* apply_laplace(l_reuse) (parcel_correction.f90:142-181): vol2grid, then
+ * the parcel displacement grad phi with Lap phi = volg/vcell - 1 (diverge,
+ * inversion.f90:410-455; horizontal FFTs, FP64 tridiagonal z solve lapinv1,
+ * inversion_utils.f90:565-589; the (0,0) mode is zero), extrapolated to the
+ * z halo, interpolated trilinearly at each parcel centre and added to its
+ * position; periodic BC in x and y.  l_reuse as for epic_par2grid. */
+int epic_apply_laplace(epic_handle h, int l_reuse);
+/* apply_gradient(prefactor, max_compression, l_reuse)
+ * (parcel_correction.f90:183-259): the limited displacement
+ * -prefactor lim grad phi from phi = volg/vcell - 1 on the 8 nodes around
+ * each parcel centre, clipped to +/- max_compression lim per direction with
+ * lim = dx w (1 - w); periodic BC.  EPIC's defaults are prefactor 1.8
+ * (gradient_pref) and max_compression 0.5 (options.f90:92-93). */
+int epic_apply_gradient(epic_handle h, double prefactor, double max_compression, int l_reuse);
+/* init_parcel_correction (parcel_correction.f90:42-83): stores the volume-
+ * weighted mean vorticity of the parcels as the target of
+ * epic_apply_vortcor; vor_bar (3 doubles, may be NULL) receives it. */
+int epic_init_parcel_correction(epic_handle h, double *vor_bar);
+/* apply_vortcor (parcel_correction.f90:89-138): shifts every parcel's
+ * vorticity so that the volume-weighted mean equals the stored target. */
+int epic_apply_vortcor(epic_handle h);
+
 /* Cell counting sort of the parcels' current positions with the reference's
  * key ijk = 1 + ix + hx iy + hx hy iz on the halo box
  * (parcel_nearest.f90:196-257, 411-424, 531-541): loca(n) 1-based cell of

@@ -2058,9 +2058,9 @@ int orc_parcel_merge(const orc_dom *d, orc_parcels *p, int64_t *n_merge_out)
 }
 
 /* ------------------------------------------------------------------------ */
-/* parcel_correction.f90: apply_laplace (:137-172) with diverge             */
+/* parcel_correction.f90: apply_laplace (:142-181) with diverge             */
 /* (inversion.f90:410-455) and lapinv1 (inversion_utils.f90:565-589, tables */
-/* :240-271); apply_gradient (:175-258); apply_vortcor (:91-133)            */
+/* :240-271); apply_gradient (:183-259); apply_vortcor (:89-138)            */
 /* ------------------------------------------------------------------------ */
 /* Lap(phi) = rhs semi-spectrally: 2nd-order differences in z, FP64
  * tridiagonal elimination per horizontal wavenumber; the (0,0) mode is zero */

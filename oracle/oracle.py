@@ -324,13 +324,13 @@ class State:
         return nm.value
 
     def apply_laplace(self, l_reuse=None):
-        """apply_laplace (parcel_correction.f90:137-172)."""
+        """apply_laplace (parcel_correction.f90:142-181)."""
         flag = -1 if l_reuse is None else int(bool(l_reuse))
         _chk(lib().orc_apply_laplace(C.byref(self.dom), self.inv(), C.byref(self._parcels()),
                                      C.byref(self._fields()), flag))
 
     def apply_gradient(self, prefactor=1.8, max_compression=0.5, l_reuse=None):
-        """apply_gradient (parcel_correction.f90:175-258)."""
+        """apply_gradient (parcel_correction.f90:183-259)."""
         flag = -1 if l_reuse is None else int(bool(l_reuse))
         _chk(lib().orc_apply_gradient(C.byref(self.dom), C.byref(self._parcels()), C.byref(self._fields()),
                                       prefactor, max_compression, flag))

@@ -1,9 +1,11 @@
 """The Fortran ISO_C_BINDING module drives the same GPU core as the C-ABI.
 
 The example driver (epic_amd/fortran/drive_beltrami.f90) builds a 16^3
-Beltrami member with analytic vorticity at the parcel centres and takes two
-fixed-dt RK4 steps through ``ls_rk_step``, then prints checksums of the final
-parcel state and of par2grid's volume field.  The same member run through the
+Beltrami member with analytic vorticity at the parcel centres and runs two
+iterations of EPIC's run loop (apply_vortcor, a fixed-dt RK4 ``ls_rk_step``,
+parcel_merge, parcel_split, two rounds of apply_laplace / apply_gradient),
+then prints checksums of the final parcel state and of par2grid's volume
+field.  The same member run through the
 C-ABI from Python must give the same parcel state (the device computation is
 deterministic and identical), so the checksums agree to summation-order
 rounding; a transposed or misread argument in the Fortran interfaces would not.
@@ -42,7 +44,7 @@ def _member_inputs(n=16):
 
 @pytest.mark.skipif(not os.path.exists(B.FORTRAN_DRIVER), reason="Fortran driver not built (no amdflang)")
 def test_fortran_driver_matches_c_abi():
-    print("fortran driver: beltrami 16^3, two RK4 steps", flush=True)
+    print("fortran driver: beltrami 16^3, two run-loop iterations", flush=True)
     res = subprocess.run([B.FORTRAN_DRIVER], capture_output=True, text=True, timeout=120)
     print(res.stdout, res.stderr, flush=True)
     assert res.returncode == 0
@@ -51,11 +53,18 @@ def test_fortran_driver_matches_c_abi():
 
     n = 16
     lower, extent, pos, vor, Bm, vol = _member_inputs(n)
-    m = Member(n, n, n, lower, extent, zeta_zero=(True, True), max_parcels=pos.shape[0])
+    m = Member(n, n, n, lower, extent, zeta_zero=(True, True), max_parcels=2 * pos.shape[0])
     m.set_parcels(pos, vor, Bm, vol, np.zeros(len(vol)), np.zeros(len(vol)))
+    m.init_parcel_correction()
     t = 0.0
     for _ in range(2):
+        m.apply_vortcor()
         t = m.ls_rk_step(t, 0.01, 0.0)
+        assert m.parcel_merge() == 0
+        assert m.parcel_split() == 0
+        for it in (1, 2):
+            m.apply_laplace(l_reuse=it > 1)
+            m.apply_gradient(1.8, 0.5, l_reuse=True)
     p = m.get_parcels()
     m.par2grid()
     volg = m.get_field("volg")
