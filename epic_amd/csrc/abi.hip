@@ -51,7 +51,7 @@ struct epic_member {
     int *keys = nullptr, *cnt[3] = {}, *scnt[3] = {};
     int *cell_start = nullptr, *cursor = nullptr, *perm = nullptr, *bsum = nullptr;
     bool keys_valid = false;
-    int tz_p2g = 16, tz_g2p = 16;   // z chunk of the tile kernels (autotuned)
+    int tz_p2g = 4, tz_g2p = 4;     // z chunk of the tile kernels (autotuned)
     // fields
     float *fld = nullptr;
     int *nparg = nullptr, *nsparg = nullptr;
@@ -284,13 +284,16 @@ P2GLayout layout_tz(const Grid &g, int tz)
     return L;
 }
 
-// z chunks the autotuner may choose; the staging buffer fits all of them
-const int TZ_CANDIDATES[4] = {4, 8, 16, 32};
+// z chunks the autotuner may choose (par2grid: 64 tz threads per work-group;
+// both stage a node box of tz + 3 planes in LDS); the staging buffer fits
+// every par2grid candidate
+const int TZ_P2G[3] = {2, 4, 8};
+const int TZ_G2P[3] = {2, 4, 8};
 
 int64_t staging_floats_max(const Grid &g)
 {
     int64_t mx = 0;
-    for (int tz : TZ_CANDIDATES) mx = std::max(mx, layout_tz(g, tz).staging_floats);
+    for (int tz : TZ_P2G) mx = std::max(mx, layout_tz(g, tz).staging_floats);
     return mx;
 }
 
@@ -1149,7 +1152,7 @@ int epic_autotune(epic_handle m, int *tz_p2g, int *tz_g2p)
     }
     float best_p = 1e30f, best_g = 1e30f;
     int tp = m->tz_p2g, tg = m->tz_g2p;
-    for (int tz : TZ_CANDIDATES) {
+    for (int tz : TZ_P2G) {
         if (tz > m->g.nz + 2) continue;
         m->tz_p2g = tz;
         Plan P;
@@ -1166,7 +1169,7 @@ int epic_autotune(epic_handle m, int *tz_p2g, int *tz_g2p)
         build_p2g(P, {m}, {P2GSel{m->ax_cur, nullptr, nullptr, 0, -1}});
         HIPOK(m, P.run(m->s));
     }
-    for (int tz : TZ_CANDIDATES) {
+    for (int tz : TZ_G2P) {
         if (tz > m->g.nz + 2) continue;
         // a first stage with cb dt = 0 leaves position, vorticity and B
         // unchanged; it rewrites the stage deltas (reset by every step's first
@@ -1211,11 +1214,10 @@ int epic_set_tiling(epic_handle m, int tz_p2g, int tz_g2p)
 {
     if (!m) return EPIC_E_INVALID;
     bool ok_p = false, ok_g = false;
-    for (int tz : TZ_CANDIDATES) {
-        ok_p |= (tz == tz_p2g);
-        ok_g |= (tz == tz_g2p);
-    }
-    if (!ok_p || !ok_g) return fail(m, EPIC_E_INVALID, "epic_set_tiling: z chunk must be 4, 8, 16 or 32");
+    for (int tz : TZ_P2G) ok_p |= (tz == tz_p2g);
+    for (int tz : TZ_G2P) ok_g |= (tz == tz_g2p);
+    if (!ok_p || !ok_g)
+        return fail(m, EPIC_E_INVALID, "epic_set_tiling: z chunks must be 2, 4 or 8");
     m->tz_p2g = tz_p2g;
     m->tz_g2p = tz_g2p;
     m->invalidate_plans();

@@ -19,8 +19,6 @@ constexpr int ZB = 16;           // z planes per FFT work-group
 constexpr int P2G_TX = 8;        // p2g tile cells in x
 constexpr int P2G_TY = 8;        // p2g tile cells in y
 constexpr int P2G_BOX = 11;      // node box per tile side: 1 lower + 8 + 2 upper
-constexpr int P2G_THREADS = 320; // 300 extended cells (10 x 10 x 3) + spare
-constexpr int P2G_MAXP = 640;    // parcels per LDS batch
 constexpr int P2G_NF = 7;        // vol, xi, eta, zeta, tbuoy, dbuoy, hum
 
 // error bits reported through the member's device error word

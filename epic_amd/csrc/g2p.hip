@@ -14,6 +14,8 @@
 #include "ellipsoid.h"
 #include "launch.h"
 
+#include <algorithm>
+
 namespace epic {
 
 // Stand-alone grid2par(add) (parcel_interpl.f90:405-470): support points
@@ -119,76 +121,89 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
 }
 
 // ---------------------------------------------------------------------------
-// Tiled form used by ls_rk_step: one work-group per (8x8 column tile, z
-// chunk) marching up the chunk one cell layer at a time, like par2grid.
-// The 14 gathered fields of the node planes k-1..k+2 are staged in an LDS
-// ring (node-major, 16 floats per node) so every corner is four 16-byte LDS
-// reads; parcels are read through the cell sort, support points come from
-// the half axes cached for this stage.
+// Tiled form used by ls_rk_step: one work-group per (8x8 column tile, z chunk
+// of tz cell layers).  The 14 gathered fields of the chunk's node box (node
+// planes k0-1 .. k0+tz+1, x and y from -1 to 9 around the tile) are staged in
+// LDS once, node-major with 16 floats per node, so every corner is three
+// 16-byte and one 8-byte LDS read; then every thread takes parcels of the
+// chunk through the cell sort (the rows of 8 cells are contiguous slot
+// ranges) with no barrier between parcels.  Support points come from the
+// half axes cached for this stage.
 // ---------------------------------------------------------------------------
 constexpr int G2P_THREADS = 256;
-constexpr int NODE_PAD = 20;   // floats per staged node (14 used; 80-byte stride spreads the LDS banks)
+constexpr int NODE_F = 16;   // floats per staged node (14 used)
+
+size_t g2p_lds_bytes(int tz) { return sizeof(float) * NODE_F * (tz + 3) * P2G_BOX * P2G_BOX; }
 
 __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const G2PTileArgs &a = tab[ib.item];
     const Grid &g = a.g;
+    const int tz = a.tz;
     const int tx = ib.blk % a.ntx;
     const int ty = (ib.blk / a.ntx) % a.nty;
     const int cz = ib.blk / (a.ntx * a.nty);
     const int i0 = tx * P2G_TX, j0 = ty * P2G_TY;
-    const int k0 = -1 + cz * a.tz;
-    const int k1 = min(k0 + a.tz, g.nz + 1);
+    const int k0 = -1 + cz * tz;
+    const int nl = min(tz, g.nz + 1 - k0);   // layers k0 .. k0 + nl - 1
     const int tid = threadIdx.x;
     const int64_t PS = plane_stride(g), FS = field_stride(g);
     constexpr int NN = P2G_BOX * P2G_BOX;
+    const int nbox = tz + 3;
 
-    __shared__ float4 ring[4 * NN * (NODE_PAD / 4)];
-    __shared__ int row_slot[P2G_TY], row_off[P2G_TY + 1];
+    extern __shared__ float4 box[];                 // [plane][y][x] x NODE_F floats
+    __shared__ int row_slot[64], row_off[65];       // the chunk's rows of 8 cells
     int e = 0;
 
-    auto load_plane = [&](int P) {
-        float *dst = (float *)(ring + ((P + 8) & 3) * NN * (NODE_PAD / 4));
-        const bool inside = (P >= -1 && P <= g.nz + 1);
-        for (int u = tid; u < 14 * NN; u += G2P_THREADS) {
-            const int f = u / NN, node = u - f * NN;
+    // ---- stage the node box (plane P = k0 - 1 + lp) ----
+    {
+        float *dst = (float *)box;
+        for (int u = tid; u < 14 * NN * nbox; u += G2P_THREADS) {
+            const int lp = u / (14 * NN), rem = u - lp * (14 * NN);
+            const int f = rem / NN, node = rem - f * NN;
             const int ly = node / P2G_BOX, lx = node - ly * P2G_BOX;
+            const int P = k0 - 1 + lp;
             float v = 0.0f;
-            if (inside) {
+            if (P >= -1 && P <= g.nz + 1) {
                 const int gi = wrap(i0 - 1 + lx, g.nx), gj = wrap(j0 - 1 + ly, g.ny);
                 v = a.fld[(F_VELOG + f) * FS + (int64_t)(P + 1) * PS + (int64_t)gj * g.nx + gi];
             }
-            dst[node * NODE_PAD + f] = v;
+            dst[(lp * NN + node) * NODE_F + f] = v;
         }
-    };
+    }
+    const int nrow = nl * P2G_TY;
+    if (tid < nrow) {
+        const int k = k0 + tid / P2G_TY, r = tid % P2G_TY;
+        const int c0 = ((k + 1) * g.ny + (j0 + r)) * g.nx + i0;
+        row_slot[tid] = a.cell_start[c0];
+        row_off[tid + 1] = a.cell_start[c0 + P2G_TX] - a.cell_start[c0];
+    }
+    __syncthreads();
+    if (tid < 64) {   // inclusive scan of the row sizes in one wave
+        int v = (tid < nrow) ? row_off[tid + 1] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(v, o, 64);
+            if (tid >= o) v += t;
+        }
+        if (tid < nrow) row_off[tid + 1] = v;
+        if (tid == 0) row_off[0] = 0;
+    }
+    __syncthreads();
+    const int total = row_off[nrow];
 
-    load_plane(k0 - 1);
-    load_plane(k0);
-    load_plane(k0 + 1);
-    for (int k = k0; k < k1; ++k) {
-        __syncthreads();   // the previous layer no longer reads plane k-2's slot
-        load_plane(k + 2);
-        if (tid < P2G_TY) {
-            const int c0 = ((k + 1) * g.ny + (j0 + tid)) * g.nx + i0;
-            row_slot[tid] = a.cell_start[c0];
-            row_off[tid + 1] = a.cell_start[c0 + P2G_TX] - a.cell_start[c0];
-        }
-        __syncthreads();
-        if (tid == 0) {
-            row_off[0] = 0;
-            for (int r = 0; r < P2G_TY; ++r) row_off[r + 1] += row_off[r];
-        }
-        __syncthreads();
-        const int total = row_off[P2G_TY];
-        for (int base = 0; base < total; base += G2P_THREADS) {   // uniform rounds: the ballot needs every lane
-            const int q = base + tid;
-            int key = -1;
-            bool small = false;
-            auto process = [&]() {
+    for (int base = 0; base < total; base += G2P_THREADS) {   // uniform rounds: the ballot needs every lane
+        const int q = base + tid;
+        int key = -1;
+        bool small = false;
+        auto process = [&]() {
             if (q >= total) return;
+            // the row holding entry q: largest r with row_off[r] <= q
             int r = 0;
-            while (r + 1 < P2G_TY && row_off[r + 1] <= q) ++r;
+            for (int step = 32; step > 0; step >>= 1)
+                if (r + step < nrow && row_off[r + step] <= q) r += step;
+            const int k = k0 + r / P2G_TY;
             const int pi = a.perm[row_slot[r] + (q - row_off[r])];
             double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
             double B[5];
@@ -232,7 +247,8 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__
                     e |= ERR_OUT_OF_GRID;
                     continue;
                 }
-                const int lx = cx + off[0] + 1, ly = cy + off[1] + 1, P = k + off[2];
+                // box coordinates of the point's lower corner
+                const int lx = cx + off[0] + 1, ly = cy + off[1] + 1, lp = k + off[2] - (k0 - 1);
                 const float pxc = 1.0f - w[0], pyc = 1.0f - w[1], pzc = 1.0f - w[2];
                 const float w00 = pyc * pxc, w10 = pyc * w[0], w01 = w[1] * pxc, w11 = w[1] * w[0];
                 // corners in Fortran element order of ww(0:1,0:1,0:1): k fastest
@@ -244,8 +260,9 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__
 #pragma unroll
                 for (int m = 0; m < 8; ++m) {
                     const int dz = m & 1, dy = (m >> 1) & 1, dx = (m >> 2) & 1;
-                    const float4 *nd = ring + (((P + dz + 8) & 3) * NN + (ly + dy) * P2G_BOX + (lx + dx)) * (NODE_PAD / 4);
-                    const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+                    const float4 *nd = box + ((lp + dz) * NN + (ly + dy) * P2G_BOX + (lx + dx)) * (NODE_F / 4);
+                    const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
+                    const float2 q3 = *(const float2 *)(nd + 3);
                     const float vv[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
                                           q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
 #pragma unroll
@@ -312,12 +329,11 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__
             else key = ((nk + 1) * g.ny + wrap(nj, g.ny)) * g.nx + wrap(ni, g.nx);
             a.keys[pi] = key;
             small = (vol <= g.vmin);
-            };
-            process();
-            // next stage's cell histogram, one atomic per run of equal keys
-            wave_run_add(key, a.count);
-            wave_run_add(small ? key : -1, a.scount);
-        }
+        };
+        process();
+        // next stage's cell histogram, one atomic per run of equal keys
+        wave_run_add(key, a.count);
+        wave_run_add(small ? key : -1, a.scount);
     }
     if (e) atomicOr(a.err, e);
 }
@@ -325,8 +341,12 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__
 void add_rk_stage_tiles(Plan &P, int phase, const std::vector<G2PTileArgs> &items)
 {
     std::vector<int> nb;
-    for (const G2PTileArgs &a : items) nb.push_back(a.p.n > 0 ? a.ntx * a.nty * a.nchz : 0);
-    P.add((const void *)k_g2p_tiles, G2P_THREADS, 0, phase, items, nb);
+    size_t lds = 0;
+    for (const G2PTileArgs &a : items) {
+        nb.push_back(a.p.n > 0 ? a.ntx * a.nty * a.nchz : 0);
+        lds = std::max(lds, g2p_lds_bytes(a.tz));
+    }
+    P.add((const void *)k_g2p_tiles, G2P_THREADS, lds, phase, items, nb);
 }
 
 void add_grid2par(Plan &P, int phase, const std::vector<G2PArgs> &items)

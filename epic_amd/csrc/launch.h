@@ -61,6 +61,7 @@ struct P2GArgs {
     float *staging;
     int *err;
     int tz, ntx, nty, nchz;
+    int lpitch;                 // LDS row pitch of the node box (set by add_p2g)
 };
 struct FinalizeArgs {
     Grid g;

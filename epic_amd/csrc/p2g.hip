@@ -6,24 +6,22 @@
 //                      small parcels per cell (nparg / nsparg)
 //   2. scan            exclusive prefix sum of the histogram -> cell_start
 //   3. k_place         slot = cell_start + integer ticket (order not fixed yet)
-//   4. k_cell_order    order every cell's slots by caller parcel index, so the
-//                      sorted order is a function of the parcel set alone
-//   5. k_p2g_tiles     one work-group per (8x8 column tile, z chunk): marches
-//                      up the chunk one cell layer at a time; the layer's
-//                      parcels are staged in LDS (support points computed in
-//                      FP64); every extended cell gathers the points that
-//                      fall in it from its 3x3 neighbours in a fixed order,
-//                      accumulates its 8 corners x 7 fields in registers and
-//                      adds them to an LDS ring of node planes in 8
-//                      parity phases (cells of equal parity share no node);
-//                      finished planes are written to a per-tile staging box
+//   4. k_cell_order    order every cell's slots by caller parcel index
+//                      (epic_cell_sort only: the reference's cell order)
+//   5. k_p2g_tiles     one work-group per (8x8 column tile, z chunk of tz
+//                      layers), one thread per cell: support points in FP64,
+//                      each of the 4 x 8 x 7 contributions of a parcel scaled
+//                      by a per-tile power of two, rounded and added to a
+//                      64-bit integer LDS accumulator of its node; the box is
+//                      converted back and written to a per-tile staging box
 //   6. k_p2g_finalize  per node, sums the staging boxes that cover it in a
 //                      fixed tile order (the periodic halo accumulation of
 //                      field_mpi.f90), applies the free-slip folds,
 //                      normalises by volume, the zeta wall condition and the
 //                      z extrapolation (parcel_interpl.f90:235-319)
-// The sum at every node is formed in an order that depends only on the
-// parcel set, never on scheduling: results are bitwise reproducible.
+// Integer node sums are exact and the staging boxes are added in a fixed
+// tile order, so the result depends on the parcel set alone, never on
+// scheduling or parcel order: results are bitwise reproducible.
 #include "common.h"
 #include "ellipsoid.h"
 #include "launch.h"
@@ -235,279 +233,172 @@ __global__ void k_cell_order(int64_t nc, const int *__restrict__ cell_start, con
 }
 
 // ---------------------------------------------------------------------------
-// 5. tile scatter
+// 5. tile scatter into fixed-point LDS accumulators
 // ---------------------------------------------------------------------------
-constexpr int NEXT = 300;   // extended cells of a layer: 10 x 10 x 3
+// One work-group per (8 x 8 column tile, tz cell layers), one thread per cell.
+// Every contribution w * (vol/4) * value is scaled by a power of two chosen
+// per tile and field (so that no single contribution exceeds 2^30), rounded
+// to an integer and added to a 64-bit LDS accumulator of its node.  Integer
+// sums do not depend on the order of the additions, so the node sums are
+// bitwise reproducible however the waves are scheduled, with no ordering
+// pass and no barriers between parcels; the only rounding is that of each
+// contribution (2^-31 of the tile's largest) and of the final conversion.
+// Node box of a tile: x, y in [-1, 9] around the tile, node planes
+// [k0 - 1, k0 + tz + 1]; rows padded to lpitch 64-bit words (12 spreads the
+// 8 x 8 lanes of a wave evenly over the 32 LDS banks).
 
-__device__ inline int box_planes(int tz) { return tz + 3; }
+__host__ __device__ inline int box_planes(int tz) { return tz + 3; }
 
-__global__ __launch_bounds__(P2G_THREADS, 3) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+// power-of-two scale s with max_abs * s < 2^30 (0 when max_abs is 0)
+__device__ inline float fixed_scale(float max_abs, int &e)
+{
+    if (!(max_abs > 0.0f)) {
+        e = 0;
+        return 0.0f;
+    }
+    frexpf(max_abs, &e);   // max_abs < 2^e
+    return ldexpf(1.0f, 30 - e);
+}
+
+__global__ __launch_bounds__(512) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const P2GArgs &a = tab[ib.item];
     const Grid &g = a.g;
     // stage 1 of a step after the first takes the cached axes (l_reuse)
     const float *const *axes = (a.use_alt && *a.use_alt) ? a.axes_alt : a.axes;
+    const int tz = a.tz, lp = a.lpitch;
     const int tx = ib.blk % a.ntx;
     const int ty = (ib.blk / a.ntx) % a.nty;
     const int cz = ib.blk / (a.ntx * a.nty);
     const int i0 = tx * P2G_TX, j0 = ty * P2G_TY;
-    const int k0 = -1 + cz * a.tz;
-    const int k1 = min(k0 + a.tz, g.nz + 1);   // layers [k0, k1)
+    const int k0 = -1 + cz * tz;
     const int tid = threadIdx.x;
+    const int nbox = box_planes(tz);
+    const int lplane = P2G_BOX * lp;              // words per LDS node plane
+    const int lfield = nbox * lplane;             // words per field
+    extern __shared__ unsigned long long acc[];   // [field][plane][y][x]
+    __shared__ float wmax[512 / 64][P2G_NF];
+    __shared__ int s_err;
 
-    constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
-    constexpr int NCELL = P2G_TX * P2G_TY;
-    __shared__ float ring[4 * PLANE];
-    __shared__ float frac[P2G_MAXP * 12];
-    __shared__ float vals[P2G_MAXP * 7];
-    __shared__ __attribute__((aligned(8))) unsigned short extc[P2G_MAXP * 4];   // extended cell of every support point (511: none)
-    __shared__ unsigned short ord[P2G_MAXP];       // batch entries of each cell in caller order
-    __shared__ unsigned char cell_of[P2G_MAXP];    // own cell of each batch entry
-    __shared__ __attribute__((aligned(16))) unsigned short items[P2G_MAXP * 4];   // support points binned by extended cell
-    int *const origk = (int *)items;               // caller index of each batch entry (until the binning)
-    __shared__ int bin_start[NEXT];
-    __shared__ int wave_tot[P2G_THREADS / 64];
-    __shared__ int cell_off[NCELL + 1];            // concatenated offsets of the layer's cells
-    __shared__ int cell_cnt[NCELL];
-    __shared__ int row_slot[P2G_TY];
-    __shared__ int row_off[P2G_TY + 1];
-    __shared__ int batch_cell[NCELL + 1];          // cell-aligned batch boundaries
-    __shared__ int s_nbatch, s_err;
-
-    for (int u = tid; u < 4 * PLANE; u += P2G_THREADS) ring[u] = 0.0f;
+    for (int u = tid; u < P2G_NF * lfield; u += blockDim.x) acc[u] = 0ull;
     if (tid == 0) s_err = 0;
 
-    const int nbox = box_planes(a.tz);
-    float *stage = a.staging + (int64_t)ib.blk * nbox * PLANE;
-    // extended cell owned by this thread in the accumulation (300 used)
-    const bool ecell = tid < NEXT;
-    const int ez = tid / 100 - 1;
-    const int ey = (tid % 100) / 10 - 1;
-    const int ex = tid % 10 - 1;
-
-    auto flush_plane = [&](int P) {   // node plane P -> staging, then clear
-        float *dst = stage + (int64_t)(P - (k0 - 1)) * PLANE;
-        float *src = ring + ((P + 8) & 3) * PLANE;
-        for (int u = tid; u < PLANE; u += P2G_THREADS) {
-            dst[u] = src[u];
-            src[u] = 0.0f;
-        }
+    // this thread's cell
+    const int cx = tid & 7, cy = (tid >> 3) & 7, lz = tid >> 6;
+    const int k = k0 + lz;
+    int s0 = 0, s1 = 0;
+    if (k <= g.nz) {
+        const int c = ((k + 1) * g.ny + j0 + cy) * g.nx + i0 + cx;
+        s0 = a.cell_start[c];
+        s1 = a.cell_start[c + 1];
+    }
+    // the gridded quantities of a parcel: vol/4 and vol/4 times vorticity,
+    // total buoyancy (with condensation), buoyancy and humidity
+    auto values = [&](int pi, double z, float v[P2G_NF]) {
+        const double qc = fmax(0.0, (double)a.p.hum[pi] - g.q0 * exp(g.lambda_c * (g.lower[2] - z)));
+        const float v0 = (float)(0.25 * (double)a.p.vol[pi]);
+        v[0] = v0;
+        v[1] = v0 * a.p.vor[0][pi];
+        v[2] = v0 * a.p.vor[1][pi];
+        v[3] = v0 * a.p.vor[2][pi];
+        v[4] = v0 * (float)((double)a.p.buoy[pi] + g.glat * qc);
+        v[5] = v0 * a.p.buoy[pi];
+        v[6] = v0 * a.p.hum[pi];
     };
 
-    for (int k = k0; k < k1; ++k) {
-        __syncthreads();
-        // ---- the layer's cells: rows of 8 consecutive cells in the sort ----
-        if (tid < P2G_TY) {
-            const int c0 = ((k + 1) * g.ny + (j0 + tid)) * g.nx + i0;
-            row_slot[tid] = a.cell_start[c0];
-            row_off[tid + 1] = a.cell_start[c0 + P2G_TX] - a.cell_start[c0];
-        }
-        __syncthreads();
-        if (tid == 0) {
-            row_off[0] = 0;
-            for (int r = 0; r < P2G_TY; ++r) row_off[r + 1] += row_off[r];
-        }
-        __syncthreads();
-        if (tid < NCELL) {   // one lane per cell of the layer
-            const int r = tid / P2G_TX, cx = tid % P2G_TX;
-            const int base = ((k + 1) * g.ny + (j0 + r)) * g.nx + i0 + cx;
-            const int s0 = a.cell_start[base];
-            cell_cnt[tid] = a.cell_start[base + 1] - s0;
-            cell_off[tid] = row_off[r] + (s0 - row_slot[r]);
-            if (tid == 0) cell_off[NCELL] = row_off[P2G_TY];
-        }
-        __syncthreads();
-        if (tid == 0) {
-            // cell-aligned batches of at most P2G_MAXP parcels (one batch unless crowded)
-            int nb = 0;
-            batch_cell[0] = 0;
-            if (row_off[P2G_TY] > P2G_MAXP) {
-                int acc = 0;
-                for (int c = 0; c < NCELL; ++c) {
-                    const int m = cell_cnt[c];
-                    if (m > P2G_MAXP) s_err |= ERR_OUT_OF_GRID;   // one cell beyond a batch: not supported
-                    if (acc + m > P2G_MAXP) {
-                        batch_cell[++nb] = c;
-                        acc = 0;
-                    }
-                    acc += m;
-                }
-            }
-            batch_cell[++nb] = NCELL;
-            s_nbatch = nb;
-        }
-        __syncthreads();
-        const int nbatch = s_nbatch;
-
-        for (int bt = 0; bt < nbatch; ++bt) {
-            const int cb0 = batch_cell[bt], cb1 = batch_cell[bt + 1];
-            const int q0 = cell_off[cb0];
-            const int nb = cell_off[cb1] - q0;
-            if (nb > P2G_MAXP) continue;   // flagged above
-            // owner cell of every entry
-            for (int c = cb0 + tid; c < cb1; c += P2G_THREADS)
-                for (int q = cell_off[c]; q < cell_off[c + 1]; ++q) cell_of[q - q0] = (unsigned char)c;
-            __syncthreads();
-            // ---- load: support points (FP64 positions, cached FP32 half axes) ----
-            for (int q = tid; q < nb; q += P2G_THREADS) {
-                const int gq = q0 + q;
-                const int c = cell_of[q];
-                const int r = c / P2G_TX, cx = c % P2G_TX;
-                const int slot = row_slot[r] + (gq - row_off[r]);
-                const int pi = a.perm[slot];
-                origk[q] = a.p.orig[pi];
-                const double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
-                const double ve[3] = {axes[0][pi], axes[1][pi], axes[2][pi]};
-                const double vt[3] = {axes[3][pi], axes[4][pi], axes[5][pi]};
-                int cc[3];
+    // ---- pass 1: the tile's largest contribution of every field ----
+    float mx[P2G_NF];
 #pragma unroll
-                for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
+    for (int f = 0; f < P2G_NF; ++f) mx[f] = 0.0f;
+    for (int s = s0; s < s1; ++s) {
+        const int pi = a.perm[s];
+        float v[P2G_NF];
+        values(pi, a.p.z[pi], v);   // the same values pass 2 scales
 #pragma unroll
-                for (int jp = 0; jp < 4; ++jp) {
-                    const double cs = support_cos(jp), sn = support_sin(jp);
-                    int off[3];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        const double pt = pos[d] + ve[d] * cs + vt[d] * sn;
-                        const double xyz = (pt - g.lower[d]) * g.dxi[d];
-                        const double fl = floor(xyz);
-                        off[d] = (int)fl - cc[d];
-                        frac[q * 12 + jp * 3 + d] = (float)(xyz - fl);
-                    }
-                    int e = 511;
-                    if (off[0] >= -1 && off[0] <= 1 && off[1] >= -1 && off[1] <= 1 && off[2] >= -1 && off[2] <= 1 &&
-                        cc[2] + off[2] >= -1 && cc[2] + off[2] <= g.nz)
-                        e = (off[2] + 1) * 100 + (r + off[1] + 1) * 10 + (cx + off[0] + 1);
-                    else
-                        atomicOr(&s_err, ERR_OUT_OF_GRID);
-                    extc[q * 4 + jp] = (unsigned short)e;
-                }
-                const double qc = fmax(0.0, (double)a.p.hum[pi] - g.q0 * exp(g.lambda_c * (g.lower[2] - pos[2])));
-                const double btot = (double)a.p.buoy[pi] + g.glat * qc;
-                float *v = vals + q * 7;
-                v[0] = (float)(0.25 * (double)a.p.vol[pi]);
-                v[1] = a.p.vor[0][pi];
-                v[2] = a.p.vor[1][pi];
-                v[3] = a.p.vor[2][pi];
-                v[4] = (float)btot;
-                v[5] = a.p.buoy[pi];
-                v[6] = a.p.hum[pi];
-            }
-            __syncthreads();
-            // ---- order inside each cell by caller index (placement order is arbitrary) ----
-            for (int c = cb0 + tid; c < cb1; c += P2G_THREADS) {
-                const int lo = cell_off[c] - q0, hi = cell_off[c + 1] - q0;
-                for (int q = lo; q < hi; ++q) {
-                    const int oq = origk[q];
-                    int rank = 0;
-                    for (int q2 = lo; q2 < hi; ++q2) rank += (origk[q2] < oq);
-                    ord[lo + rank] = (unsigned short)q;
-                }
-            }
-            __syncthreads();
-            // ---- every extended cell gathers its points (registers): the 3x3
-            // cells of the layer around it in row-major order, their parcels in
-            // caller order, the points of a parcel in order: a fixed summation
-            // order that depends on the parcel set alone ----
-            // scan of the 3x3 neighbourhood in that order, twice: count, then
-            // fill this extended cell's bin (cheap passes); the accumulation
-            // then runs over the bin only
-            auto scan_nbh = [&](auto &&visit) {
-                for (int r = ey - 1; r <= ey + 1; ++r) {
-                    if (r < 0 || r >= P2G_TY) continue;
-                    for (int cx = ex - 1; cx <= ex + 1; ++cx) {
-                        if (cx < 0 || cx >= P2G_TX) continue;
-                        const int c = r * P2G_TX + cx;
-                        if (c < cb0 || c >= cb1) continue;
-                        const int lo = cell_off[c] - q0, hi = cell_off[c + 1] - q0;
-                        for (int x = lo; x < hi; ++x) {
-                            const int q = ord[x];
-                            const ushort4 e4 = *(const ushort4 *)(extc + q * 4);
-                            if (e4.x == tid) visit(q * 4 + 0);
-                            if (e4.y == tid) visit(q * 4 + 1);
-                            if (e4.z == tid) visit(q * 4 + 2);
-                            if (e4.w == tid) visit(q * 4 + 3);
-                        }
-                    }
-                }
-            };
-            int nbin = 0;
-            if (ecell) scan_nbh([&](int) { ++nbin; });
-            {   // exclusive prefix of the bin sizes over the work-group
-                const int lane = tid & 63, w = tid >> 6;
-                int incl = nbin;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int t = __shfl_up(incl, o, 64);
-                    if (lane >= o) incl += t;
-                }
-                if (lane == 63) wave_tot[w] = incl;
-                __syncthreads();
-                int base = 0;
-                for (int u = 0; u < w; ++u) base += wave_tot[u];
-                if (ecell) bin_start[tid] = base + incl - nbin;
-            }
-            __syncthreads();   // origk is no longer read: items may be written
-            if (ecell) {
-                int k2 = bin_start[tid];
-                scan_nbh([&](int it) { items[k2++] = (unsigned short)it; });
-            }
-            float acc[8][P2G_NF];
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-#pragma unroll
-                for (int f = 0; f < P2G_NF; ++f) acc[c][f] = 0.0f;
-            const bool any = nbin > 0;
-            if (ecell) {
-                const int b0 = bin_start[tid];
-                for (int x = b0; x < b0 + nbin; ++x) {
-                    const int it = items[x];
-                    const int q = it >> 2, jp = it & 3;
-                    const float fx = frac[q * 12 + jp * 3 + 0];
-                    const float fy = frac[q * 12 + jp * 3 + 1];
-                    const float fz = frac[q * 12 + jp * 3 + 2];
-                    const float *v = vals + q * 7;
-                    const float wxy[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx), fy * fx};
-#pragma unroll
-                    for (int cc = 0; cc < 8; ++cc) {
-                        const float wz = (cc & 4) ? fz : 1.0f - fz;
-                        const float wt = v[0] * (wz * wxy[cc & 3]);
-                        acc[cc][0] += wt;
-#pragma unroll
-                        for (int f = 1; f < P2G_NF; ++f) acc[cc][f] += wt * v[f];
-                    }
-                }
-            }
-            // ---- parity-phased add into the node ring (equal parity: disjoint nodes) ----
-            const int kz = k + ez;
-            const int par = ((ex + 2) & 1) | (((ey + 2) & 1) << 1) | (((kz + 4) & 1) << 2);
-            for (int ph = 0; ph < 8; ++ph) {
-                __syncthreads();
-                if (ecell && any && par == ph) {
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        const int nxl = ex + 1 + (c & 1);
-                        const int nyl = ey + 1 + ((c >> 1) & 1);
-                        const int P = kz + ((c >> 2) & 1);
-                        float *dst = ring + ((P + 8) & 3) * PLANE + (nyl * P2G_BOX + nxl) * P2G_NF;
-#pragma unroll
-                        for (int f = 0; f < P2G_NF; ++f) dst[f] += acc[c][f];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        __syncthreads();
-        // node plane k-1 is complete
-        flush_plane(k - 1);
+        for (int f = 0; f < P2G_NF; ++f) mx[f] = fmaxf(mx[f], fabsf(v[f]));
     }
+#pragma unroll
+    for (int f = 0; f < P2G_NF; ++f)
+        for (int o = 32; o > 0; o >>= 1) mx[f] = fmaxf(mx[f], __shfl_xor(mx[f], o, 64));
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int f = 0; f < P2G_NF; ++f) wmax[tid >> 6][f] = mx[f];
     __syncthreads();
-    flush_plane(k1 - 1);
+    float sc[P2G_NF];
+#pragma unroll
+    for (int f = 0; f < P2G_NF; ++f) {
+        float m = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, wmax[w][f]);
+        int e;
+        sc[f] = fixed_scale(m, e);
+    }
+
+    // ---- pass 2: support points, trilinear weights, integer accumulation ----
+    int err = 0;
+    for (int s = s0; s < s1; ++s) {
+        const int pi = a.perm[s];
+        const double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
+        const double ve[3] = {axes[0][pi], axes[1][pi], axes[2][pi]};
+        const double vt[3] = {axes[3][pi], axes[4][pi], axes[5][pi]};
+        float v[P2G_NF];
+        values(pi, pos[2], v);
+        float cs[P2G_NF];
+#pragma unroll
+        for (int f = 0; f < P2G_NF; ++f) cs[f] = v[f] * sc[f];   // exact: sc is a power of two
+        int cc[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
+#pragma unroll
+        for (int jp = 0; jp < 4; ++jp) {
+            const double csn = support_cos(jp), snn = support_sin(jp);
+            int off[3];
+            float fr[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double pt = pos[d] + ve[d] * csn + vt[d] * snn;
+                const double xyz = (pt - g.lower[d]) * g.dxi[d];
+                const double fl = floor(xyz);
+                off[d] = (int)fl - cc[d];
+                fr[d] = (float)(xyz - fl);
+            }
+            if (off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1 ||
+                cc[2] + off[2] < -1 || cc[2] + off[2] > g.nz) {
+                err |= ERR_OUT_OF_GRID;
+                continue;
+            }
+            // lower corner of the point's cell in the box
+            const int bx = cx + off[0] + 1, by = cy + off[1] + 1, bz = lz + off[2] + 1;
+            const float wxy[4] = {(1.0f - fr[1]) * (1.0f - fr[0]), (1.0f - fr[1]) * fr[0], fr[1] * (1.0f - fr[0]),
+                                  fr[1] * fr[0]};
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float w = ((c & 4) ? fr[2] : 1.0f - fr[2]) * wxy[c & 3];
+                const int node = (bz + (c >> 2)) * lplane + (by + ((c >> 1) & 1)) * lp + bx + (c & 1);
+#pragma unroll
+                for (int f = 0; f < P2G_NF; ++f) {
+                    const long long q = (long long)(int)rintf(w * cs[f]);
+                    atomicAdd(&acc[f * lfield + node], (unsigned long long)q);
+                }
+            }
+        }
+    }
+    if (err) atomicOr(&s_err, err);
     __syncthreads();
-    flush_plane(k1);
-    __syncthreads();
-    flush_plane(k1 + 1);
+
+    // ---- the box to staging (FP32, node-major, fields fastest) ----
+    float inv[P2G_NF];
+#pragma unroll
+    for (int f = 0; f < P2G_NF; ++f) inv[f] = (sc[f] > 0.0f) ? 1.0f / sc[f] : 0.0f;   // a power of two: exact
+    constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
+    float *stage = a.staging + (int64_t)ib.blk * nbox * PLANE;
+    for (int u = tid; u < nbox * PLANE; u += blockDim.x) {
+        const int f = u % P2G_NF, nd = u / P2G_NF;
+        const int pz = nd / (P2G_BOX * P2G_BOX), yx = nd - pz * (P2G_BOX * P2G_BOX);
+        const int y = yx / P2G_BOX, x = yx - y * P2G_BOX;
+        const long long q = (long long)acc[f * lfield + pz * lplane + y * lp + x];
+        stage[u] = (float)((double)q * (double)inv[f]);
+    }
     if (tid == 0 && s_err) atomicOr(a.err, s_err);
 }
 
@@ -707,7 +598,7 @@ void launch_cell_order(int64_t nc, const int *cell_start, const int *perm_tmp, c
 P2GLayout p2g_layout(const Grid &g)
 {
     P2GLayout L;
-    L.tz = 16;
+    L.tz = 4;
     L.ntx = g.nx / P2G_TX;
     L.nty = g.ny / P2G_TY;
     L.nchz = (g.nz + 2 + L.tz - 1) / L.tz;
@@ -715,12 +606,29 @@ P2GLayout p2g_layout(const Grid &g)
     return L;
 }
 
+int p2g_lpitch(int tz) { return tz >= 8 ? P2G_BOX : 12; }   // 8: two work-groups per CU still fit
+
+size_t p2g_lds_bytes(int tz) { return sizeof(unsigned long long) * P2G_NF * box_planes(tz) * P2G_BOX * p2g_lpitch(tz); }
+
 void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin)
 {
-    std::vector<int> nb, nf;
-    for (const P2GArgs &a : tiles) nb.push_back(a.ntx * a.nty * a.nchz);
+    // one grouped launch per z chunk (the work-group size is 64 tz)
+    std::vector<int> tzs;
+    for (const P2GArgs &a : tiles)
+        if (std::find(tzs.begin(), tzs.end(), a.tz) == tzs.end()) tzs.push_back(a.tz);
+    for (int tz : tzs) {
+        std::vector<P2GArgs> items;
+        std::vector<int> nb;
+        for (P2GArgs a : tiles) {
+            if (a.tz != tz) continue;
+            a.lpitch = p2g_lpitch(tz);
+            items.push_back(a);
+            nb.push_back(a.ntx * a.nty * a.nchz);
+        }
+        P.add((const void *)k_p2g_tiles, 64 * tz, p2g_lds_bytes(tz), phase, items, nb);
+    }
+    std::vector<int> nf;
     for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny * (f.g.nz + 3) + 255) / 256));
-    P.add((const void *)k_p2g_tiles, P2G_THREADS, 0, phase, tiles, nb);
     P.add((const void *)k_p2g_finalize, 256, 0, phase, fin, nf);
 }
 

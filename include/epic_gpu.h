@@ -203,9 +203,9 @@ int epic_get_timers(epic_handle h, double *ms, int *calls, int reset);
 /* Device bytes epic_create allocates for this grid and capacity (the
  * admission figure of a server that keeps members resident in HBM). */
 int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels);
-/* Picks the z chunk (cell layers per work-group) of the par2grid and the
- * fused grid2par/push tile kernels by timing each candidate (4, 8, 16, 32)
- * on this member's parcels; leaves position, vorticity and B unchanged (the
+/* Picks the z chunk (cell layers per work-group: 2, 4 or 8) of the
+ * par2grid and the fused grid2par/push tile kernels by timing each
+ * candidate on this member's parcels; leaves position, vorticity and B unchanged (the
  * RK stage work arrays are overwritten).  Returns the choice. */
 int epic_autotune(epic_handle h, int *tz_p2g, int *tz_g2p);
 int epic_set_tiling(epic_handle h, int tz_p2g, int tz_g2p);

@@ -95,7 +95,7 @@ def test_autotune_leaves_state_and_stays_accurate():
     before = _state(m)
     tz = m.autotune()
     print(f"autotune: tz_p2g={tz[0]} tz_g2p={tz[1]}", flush=True)
-    assert tz[0] in (4, 8, 16, 32) and tz[1] in (4, 8, 16, 32)
+    assert tz[0] in (2, 4, 8) and tz[1] in (2, 4, 8)
     assert np.array_equal(before, _state(m))
     m.ls_rk_step(0.0, 0.01)
     st = oracle_state(w)
@@ -106,7 +106,7 @@ def test_autotune_leaves_state_and_stays_accurate():
     assert maxrel(p["B"], st.B) <= 3 * TOL
 
 
-@pytest.mark.parametrize("tz", [(4, 32), (32, 4), (8, 8)])
+@pytest.mark.parametrize("tz", [(4, 2), (8, 4), (2, 8)])
 def test_every_tiling_matches_oracle(tz):
     w = W.rising_thermal(16, seed=25)
     m = Member.from_workload(w)
