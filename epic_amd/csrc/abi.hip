@@ -51,7 +51,7 @@ struct epic_member {
     int *keys = nullptr, *cnt[3] = {}, *scnt[3] = {};
     int *cell_start = nullptr, *cursor = nullptr, *perm = nullptr, *bsum = nullptr;
     bool keys_valid = false;
-    int tz_p2g = 4, tz_g2p = 4;     // z chunk of the tile kernels (autotuned)
+    int tz_p2g = 2, tz_g2p = 2;     // z chunk of the tile kernels (autotuned)
     // fields
     float *fld = nullptr;
     int *nparg = nullptr, *nsparg = nullptr;
@@ -284,7 +284,7 @@ P2GLayout layout_tz(const Grid &g, int tz)
     return L;
 }
 
-// z chunks the autotuner may choose (par2grid: 64 tz threads per work-group;
+// z chunks the autotuner may choose (par2grid: 64 tz P2G_TPC threads per work-group;
 // both stage a node box of tz + 3 planes in LDS); the staging buffer fits
 // every par2grid candidate
 const int TZ_P2G[3] = {2, 4, 8};

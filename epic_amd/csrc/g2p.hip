@@ -124,18 +124,17 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
 // Tiled form used by ls_rk_step: one work-group per (8x8 column tile, z chunk
 // of tz cell layers).  The 14 gathered fields of the chunk's node box (node
 // planes k0-1 .. k0+tz+1, x and y from -1 to 9 around the tile) are staged in
-// LDS once, node-major with 16 floats per node, so every corner is three
-// 16-byte and one 8-byte LDS read; then every thread takes parcels of the
+// LDS once as four planes of field groups (fields 0-3, 4-7, 8-11 as float4,
+// 12-13 as float2), so every corner is three 16-byte and one 8-byte LDS read
+// and neighbouring nodes sit in neighbouring banks; then every thread takes parcels of the
 // chunk through the cell sort (the rows of 8 cells are contiguous slot
 // ranges) with no barrier between parcels.  Support points come from the
 // half axes cached for this stage.
 // ---------------------------------------------------------------------------
 constexpr int G2P_THREADS = 256;
-constexpr int NODE_F = 16;   // floats per staged node (14 used)
+size_t g2p_lds_bytes(int tz) { return (3 * sizeof(float4) + sizeof(float2)) * (tz + 3) * P2G_BOX * P2G_BOX; }
 
-size_t g2p_lds_bytes(int tz) { return sizeof(float) * NODE_F * (tz + 3) * P2G_BOX * P2G_BOX; }
-
-__global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+__global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const G2PTileArgs &a = tab[ib.item];
@@ -152,13 +151,14 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__
     constexpr int NN = P2G_BOX * P2G_BOX;
     const int nbox = tz + 3;
 
-    extern __shared__ float4 box[];                 // [plane][y][x] x NODE_F floats
+    extern __shared__ float4 box[];                 // [group][plane][y][x]: 3 float4 groups, then a float2 group
+    const int NB = nbox * NN;                       // nodes in the box
+    float2 *box2 = (float2 *)(box + 3 * NB);
     __shared__ int row_slot[64], row_off[65];       // the chunk's rows of 8 cells
     int e = 0;
 
     // ---- stage the node box (plane P = k0 - 1 + lp) ----
     {
-        float *dst = (float *)box;
         for (int u = tid; u < 14 * NN * nbox; u += G2P_THREADS) {
             const int lp = u / (14 * NN), rem = u - lp * (14 * NN);
             const int f = rem / NN, node = rem - f * NN;
@@ -169,7 +169,9 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__
                 const int gi = wrap(i0 - 1 + lx, g.nx), gj = wrap(j0 - 1 + ly, g.ny);
                 v = a.fld[(F_VELOG + f) * FS + (int64_t)(P + 1) * PS + (int64_t)gj * g.nx + gi];
             }
-            dst[(lp * NN + node) * NODE_F + f] = v;
+            const int nd = lp * NN + node;
+            if (f < 12) ((float *)box)[((f >> 2) * NB + nd) * 4 + (f & 3)] = v;
+            else ((float *)box2)[nd * 2 + (f - 12)] = v;
         }
     }
     const int nrow = nl * P2G_TY;
@@ -260,9 +262,9 @@ __global__ __launch_bounds__(G2P_THREADS) void k_g2p_tiles(const G2PTileArgs *__
 #pragma unroll
                 for (int m = 0; m < 8; ++m) {
                     const int dz = m & 1, dy = (m >> 1) & 1, dx = (m >> 2) & 1;
-                    const float4 *nd = box + ((lp + dz) * NN + (ly + dy) * P2G_BOX + (lx + dx)) * (NODE_F / 4);
-                    const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
-                    const float2 q3 = *(const float2 *)(nd + 3);
+                    const int nd = (lp + dz) * NN + (ly + dy) * P2G_BOX + (lx + dx);
+                    const float4 q0 = box[nd], q1 = box[NB + nd], q2 = box[2 * NB + nd];
+                    const float2 q3 = box2[nd];
                     const float vv[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
                                           q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
 #pragma unroll

@@ -235,21 +235,26 @@ __global__ void k_cell_order(int64_t nc, const int *__restrict__ cell_start, con
 // ---------------------------------------------------------------------------
 // 5. tile scatter into fixed-point LDS accumulators
 // ---------------------------------------------------------------------------
-// One work-group per (8 x 8 column tile, tz cell layers), one thread per cell.
+// One work-group per (8 x 8 column tile, tz cell layers), P2G_TPC threads per
+// cell taking every P2G_TPC-th parcel of it (the lanes of a wave are 64
+// distinct cells, so their atomics rarely meet on one node).
 // Every contribution w * (vol/4) * value is scaled by a power of two chosen
-// per tile and field (so that no single contribution exceeds 2^30), rounded
-// to an integer and added to a 64-bit LDS accumulator of its node.  Integer
+// per tile and field (so that no single point's contribution exceeds 2^28),
+// rounded to an integer and added to a 64-bit LDS accumulator of its node;
+// the points of one parcel that fall in the same cell are combined first.  Integer
 // sums do not depend on the order of the additions, so the node sums are
 // bitwise reproducible however the waves are scheduled, with no ordering
 // pass and no barriers between parcels; the only rounding is that of each
-// contribution (2^-31 of the tile's largest) and of the final conversion.
+// contribution (2^-29 of the tile's largest) and of the final conversion.
 // Node box of a tile: x, y in [-1, 9] around the tile, node planes
 // [k0 - 1, k0 + tz + 1]; rows padded to lpitch 64-bit words (12 spreads the
 // 8 x 8 lanes of a wave evenly over the 32 LDS banks).
 
 __host__ __device__ inline int box_planes(int tz) { return tz + 3; }
 
-// power-of-two scale s with max_abs * s < 2^30 (0 when max_abs is 0)
+// power-of-two scale s with max_abs * s < 2^28 (0 when max_abs is 0): the
+// summed weights of up to four points of one parcel stay below 4, so every
+// rounded contribution fits an int32
 __device__ inline float fixed_scale(float max_abs, int &e)
 {
     if (!(max_abs > 0.0f)) {
@@ -257,10 +262,10 @@ __device__ inline float fixed_scale(float max_abs, int &e)
         return 0.0f;
     }
     frexpf(max_abs, &e);   // max_abs < 2^e
-    return ldexpf(1.0f, 30 - e);
+    return ldexpf(1.0f, 28 - e);
 }
 
-__global__ __launch_bounds__(512) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+__global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const P2GArgs &a = tab[ib.item];
@@ -278,14 +283,15 @@ __global__ __launch_bounds__(512) void k_p2g_tiles(const P2GArgs *__restrict__ t
     const int lplane = P2G_BOX * lp;              // words per LDS node plane
     const int lfield = nbox * lplane;             // words per field
     extern __shared__ unsigned long long acc[];   // [field][plane][y][x]
-    __shared__ float wmax[512 / 64][P2G_NF];
+    __shared__ float wmax[1024 / 64][P2G_NF];
     __shared__ int s_err;
 
     for (int u = tid; u < P2G_NF * lfield; u += blockDim.x) acc[u] = 0ull;
     if (tid == 0) s_err = 0;
 
-    // this thread's cell
-    const int cx = tid & 7, cy = (tid >> 3) & 7, lz = tid >> 6;
+    // this thread's cell and its share of the cell's parcels
+    const int ct = tid % (64 * tz), sub = tid / (64 * tz);
+    const int cx = ct & 7, cy = (ct >> 3) & 7, lz = ct >> 6;
     const int k = k0 + lz;
     int s0 = 0, s1 = 0;
     if (k <= g.nz) {
@@ -311,7 +317,7 @@ __global__ __launch_bounds__(512) void k_p2g_tiles(const P2GArgs *__restrict__ t
     float mx[P2G_NF];
 #pragma unroll
     for (int f = 0; f < P2G_NF; ++f) mx[f] = 0.0f;
-    for (int s = s0; s < s1; ++s) {
+    for (int s = s0 + sub; s < s1; s += P2G_TPC) {
         const int pi = a.perm[s];
         float v[P2G_NF];
         values(pi, a.p.z[pi], v);   // the same values pass 2 scales
@@ -336,7 +342,7 @@ __global__ __launch_bounds__(512) void k_p2g_tiles(const P2GArgs *__restrict__ t
 
     // ---- pass 2: support points, trilinear weights, integer accumulation ----
     int err = 0;
-    for (int s = s0; s < s1; ++s) {
+    for (int s = s0 + sub; s < s1; s += P2G_TPC) {
         const int pi = a.perm[s];
         const double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
         const double ve[3] = {axes[0][pi], axes[1][pi], axes[2][pi]};
@@ -349,35 +355,56 @@ __global__ __launch_bounds__(512) void k_p2g_tiles(const P2GArgs *__restrict__ t
         int cc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
+        // the four support points: target cell (box corner index, -1 when
+        // outside the neighbour cells) and cell fractions
+        int tgt[4];
+        float fr[4][3];
 #pragma unroll
         for (int jp = 0; jp < 4; ++jp) {
             const double csn = support_cos(jp), snn = support_sin(jp);
             int off[3];
-            float fr[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double pt = pos[d] + ve[d] * csn + vt[d] * snn;
                 const double xyz = (pt - g.lower[d]) * g.dxi[d];
                 const double fl = floor(xyz);
                 off[d] = (int)fl - cc[d];
-                fr[d] = (float)(xyz - fl);
+                fr[jp][d] = (float)(xyz - fl);
             }
+            tgt[jp] = -1;
             if (off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1 ||
-                cc[2] + off[2] < -1 || cc[2] + off[2] > g.nz) {
+                cc[2] + off[2] < -1 || cc[2] + off[2] > g.nz)
                 err |= ERR_OUT_OF_GRID;
-                continue;
+            else   // lower corner of the point's cell in the box
+                tgt[jp] = (lz + off[2] + 1) * lplane + (cy + off[1] + 1) * lp + cx + off[0] + 1;
+        }
+        // points in one cell share its 8 corners: their weights are summed
+        // first (in point order) and each distinct cell takes 8 x 7 atomics
+#pragma unroll
+        for (int jp = 0; jp < 4; ++jp) {
+            bool lead = tgt[jp] >= 0;
+#pragma unroll
+            for (int jq = 0; jq < jp; ++jq) lead = lead && (tgt[jq] != tgt[jp]);
+            if (!lead) continue;
+            float w[8];
+#pragma unroll
+            for (int jq = jp; jq < 4; ++jq) {
+                if (jq > jp && tgt[jq] != tgt[jp]) continue;
+                const float *f3 = fr[jq];
+                const float wxy[4] = {(1.0f - f3[1]) * (1.0f - f3[0]), (1.0f - f3[1]) * f3[0], f3[1] * (1.0f - f3[0]),
+                                      f3[1] * f3[0]};
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const float wc = ((c & 4) ? f3[2] : 1.0f - f3[2]) * wxy[c & 3];
+                    w[c] = (jq == jp) ? wc : w[c] + wc;
+                }
             }
-            // lower corner of the point's cell in the box
-            const int bx = cx + off[0] + 1, by = cy + off[1] + 1, bz = lz + off[2] + 1;
-            const float wxy[4] = {(1.0f - fr[1]) * (1.0f - fr[0]), (1.0f - fr[1]) * fr[0], fr[1] * (1.0f - fr[0]),
-                                  fr[1] * fr[0]};
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const float w = ((c & 4) ? fr[2] : 1.0f - fr[2]) * wxy[c & 3];
-                const int node = (bz + (c >> 2)) * lplane + (by + ((c >> 1) & 1)) * lp + bx + (c & 1);
+                const int node = tgt[jp] + (c >> 2) * lplane + ((c >> 1) & 1) * lp + (c & 1);
 #pragma unroll
                 for (int f = 0; f < P2G_NF; ++f) {
-                    const long long q = (long long)(int)rintf(w * cs[f]);
+                    const long long q = (long long)(int)rintf(w[c] * cs[f]);
                     atomicAdd(&acc[f * lfield + node], (unsigned long long)q);
                 }
             }
@@ -598,7 +625,7 @@ void launch_cell_order(int64_t nc, const int *cell_start, const int *perm_tmp, c
 P2GLayout p2g_layout(const Grid &g)
 {
     P2GLayout L;
-    L.tz = 4;
+    L.tz = 2;
     L.ntx = g.nx / P2G_TX;
     L.nty = g.ny / P2G_TY;
     L.nchz = (g.nz + 2 + L.tz - 1) / L.tz;
@@ -625,7 +652,7 @@ void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::v
             items.push_back(a);
             nb.push_back(a.ntx * a.nty * a.nchz);
         }
-        P.add((const void *)k_p2g_tiles, 64 * tz, p2g_lds_bytes(tz), phase, items, nb);
+        P.add((const void *)k_p2g_tiles, 64 * tz * P2G_TPC, p2g_lds_bytes(tz), phase, items, nb);
     }
     std::vector<int> nf;
     for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny * (f.g.nz + 3) + 255) / 256));
