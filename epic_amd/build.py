@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "libepic_gpu.so")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["abi.hip", "plan.hip", "p2g.hip", "g2p.hip", "fft.hip", "inversion.hip", "split.hip", "merge.hip",
-           "correction.hip"]
+           "correction.hip", "timestep.hip", "diagnostics.hip",
+           "resort.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

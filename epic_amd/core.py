@@ -49,6 +49,12 @@ class Flags(C.Structure):
     _fields_ = [("zeta_bndry_zero", C.c_int * 2), ("rk_order", C.c_int), ("vmin", C.c_double)]
 
 
+class TimeOpts(C.Structure):
+    """epic_time_t (options.f90:99-107)."""
+    _fields_ = [("fixed", C.c_int), ("fixed_dt", C.c_double), ("alpha", C.c_double), ("limit", C.c_double),
+                ("precise_stop", C.c_int)]
+
+
 _lib = None
 
 
@@ -80,6 +86,12 @@ def load_library(path=LIB_PATH):
         "epic_vorticity_tendency": ([h], i),
         "epic_grid2par": ([h, i], i),
         "epic_ls_rk_step": ([h, dp, d, d], i),
+        "epic_get_time_step": ([h, d, P(TimeOpts), dp, dp, dp], i),
+        "epic_ls_rk_step_adaptive": ([h, dp, P(TimeOpts), d, dp], i),
+        "epic_parcel_diagnostics": ([h, dp, i], i),
+        "epic_resort": ([h], i),
+        "epic_set_resort_interval": ([h, i], i),
+        "epic_field_diagnostics": ([h, dp], i),
         "epic_cell_sort": ([h, ip, ip, ip, ip], i),
         "epic_nearest_ncell": ([h], i64),
         "epic_num_parcels": ([h], i64),
@@ -221,6 +233,39 @@ class Member:
         self._chk(load_library().epic_ls_rk_step(self._h, C.byref(tt), dt, t_initial))
         return tt.value
 
+    def get_time_step(self, t, alpha=0.2, limit=0.0, precise_stop=False, fixed_dt=None):
+        """get_time_step (rk_utils.f90:73-183) from the current fields;
+        returns (dt, gmax, bmax)."""
+        o = TimeOpts(1 if fixed_dt is not None else 0, fixed_dt or 0.0, alpha, limit, int(bool(precise_stop)))
+        dt, g, b = C.c_double(0), C.c_double(0), C.c_double(0)
+        self._chk(load_library().epic_get_time_step(self._h, t, C.byref(o), C.byref(dt), C.byref(g), C.byref(b)))
+        return dt.value, g.value, b.value
+
+    def ls_rk_step_adaptive(self, t, alpha=0.2, limit=0.0, precise_stop=False, t_initial=0.0):
+        """ls_rk_step with get_time_step (ls_rk.f90:88-136); returns (t, dt)."""
+        o = TimeOpts(0, 0.0, alpha, limit, int(bool(precise_stop)))
+        tt, dt = C.c_double(t), C.c_double(0)
+        self._chk(load_library().epic_ls_rk_step_adaptive(self._h, C.byref(tt), C.byref(o), t_initial, C.byref(dt)))
+        return tt.value, dt.value
+
+    PARCEL_STATS = ("ape", "ke", "n_small", "avg_lam", "avg_vol", "std_lam", "std_vol", "sum_vol", "rms_xi",
+                    "rms_eta", "rms_zeta", "ntot_par", "enstrophy", "nsplits", "nbig_iclo", "nmerges",
+                    "min_buoy", "max_buoy")
+    FIELD_STATS = ("rms_v", "avg_npar", "avg_nspar", "keg", "apeg", "eng", "abserr_v", "max_npar", "max_buoy",
+                   "min_npar", "min_buoy")
+
+    def parcel_diagnostics(self, reset_counters=False):
+        """calculate_parcel_diagnostics (parcel_diagnostics.f90:51-159)."""
+        out = np.zeros(18)
+        self._chk(load_library().epic_parcel_diagnostics(self._h, _dp(out), int(bool(reset_counters))))
+        return dict(zip(self.PARCEL_STATS, out))
+
+    def field_diagnostics(self):
+        """calculate_field_diagnostics (field_diagnostics.f90:40-154)."""
+        out = np.zeros(11)
+        self._chk(load_library().epic_field_diagnostics(self._h, _dp(out)))
+        return dict(zip(self.FIELD_STATS, out))
+
     TIMERS = ("par2grid", "cell_sort", "vor2vel", "vorticity_tendency", "push", "grid2par", "axes")
 
     def set_profiling(self, on=True):
@@ -280,6 +325,14 @@ class Member:
     def apply_vortcor(self):
         """apply_vortcor (parcel_correction.f90:89-138)."""
         self._chk(load_library().epic_apply_vortcor(self._h))
+
+    # ---- physical re-sort ----
+    def resort(self):
+        """Reorder the parcel arrays into cell order (not observable)."""
+        self._chk(load_library().epic_resort(self._h))
+
+    def set_resort_interval(self, steps):
+        self._chk(load_library().epic_set_resort_interval(self._h, int(steps)))
 
     # ---- tiling ----
     def autotune(self):

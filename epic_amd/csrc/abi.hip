@@ -71,16 +71,23 @@ struct epic_member {
     int *err_d = nullptr;
     int *err_h = nullptr;
     int *flags_d = nullptr;
+    StepTime *tm_d = nullptr, *tm_h = nullptr;   // the step's time step (device, pinned host copy)
     std::string msg;
     // launch plans of the call surface (built on first use)
-    Plan pl_keys, pl_p2g_fresh, pl_p2g_reuse, pl_vor2vel, pl_vtend, pl_g2p[2], pl_step;
+    Plan pl_keys, pl_p2g_fresh, pl_p2g_reuse, pl_vor2vel, pl_vtend, pl_g2p[2], pl_step, pl_step_adaptive;
     Plan pl_lap[2];   // parcel correction: forward / inverse transforms
-    double step_dt = -1.0;
     // parcel correction: the target volume-weighted vorticity and the FP64
     // reduction scratch (allocated on first use)
     double vor_bar[3] = {0.0, 0.0, 0.0};
     double *vw_d = nullptr;
     double *vw_h = nullptr;
+    // diagnostics: split / merge counts since the last reset, FP64 scratch
+    int64_t n_splits = 0, n_merges = 0;
+    double *diag_d = nullptr, *diag_h = nullptr;
+    // physical re-sort into cell order every resort_every steps (0: never)
+    int resort_every = 10, steps_since_sort = 0;
+    double *sort_tmp = nullptr;
+    Plan pl_resort;
     uint64_t gen = 0;   // bumped whenever plans built over this member go stale
     void invalidate_plans()
     {
@@ -93,9 +100,10 @@ struct epic_member {
         pl_g2p[0].release();
         pl_g2p[1].release();
         pl_step.release();
+        pl_step_adaptive.release();
         pl_lap[0].release();
         pl_lap[1].release();
-        step_dt = -1.0;
+        pl_resort.release();
     }
 };
 
@@ -171,6 +179,7 @@ int error_code(epic_member *m, int e, const char *where)
     if (e & ERR_B33) return fail(m, EPIC_E_B33, w + ": get_B33: Division by small number!");
     if (e & ERR_OUT_OF_GRID) return fail(m, EPIC_E_OUT_OF_GRID, w + ": support point outside the node grid");
     if (e & ERR_MERGE) return fail(m, EPIC_E_MERGE, w + ": Merge error: no near enough neighbour found.");
+    if (e & ERR_DT) return fail(m, EPIC_E_DT, w + ": Time step <= 0!");
     return fail(m, EPIC_E_INVALID, w + ": internal consistency error");
 }
 
@@ -512,8 +521,9 @@ void build_vtend(Plan &P, const Members &ms)
 // then the cache the next step's first par2grid takes when *reuse is set.
 struct StepItem {
     epic_member *m;
-    double dt;
+    StepTime *tm;        // device: dt (written by the host, or by the adaptive time step)
     const int *reuse;
+    bool adaptive;
 };
 
 void build_step(Plan &P, const std::vector<StepItem> &items)
@@ -543,6 +553,12 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
         build_p2g(P, ms, sel);
         build_vor2vel(P, ms);
         build_vtend(P, ms);
+        if (s == 1) {   // get_time_step after the first vorticity_tendency (ls_rk.f90:92-101)
+            std::vector<TsArgs> ts;
+            for (const StepItem *it : act)
+                if (it->adaptive) ts.push_back(TsArgs{it->m->g, it->m->fld, it->tm, it->m->err_d});
+            if (!ts.empty()) add_time_step(P, PH_OTHER, ts);
+        }
         std::vector<G2PTileArgs> gv;
         for (size_t i = 0; i < act.size(); ++i) {
             epic_member *m = act[i]->m;
@@ -562,7 +578,8 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
             a.stage = s;
             a.nstages = ns;
             a.reflect = last ? 1 : 0;
-            a.cbdt = cb[s - 1] * act[i]->dt;
+            a.cb = cb[s - 1];
+            a.dt = &act[i]->tm->dt;
             a.ca = ca[s - 1];
             a.keys = m->keys;
             a.count = m->cnt[wr];
@@ -578,7 +595,6 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
     }
 }
 
-// enqueue a member plan on the member stream (event timers when profiling)
 // parcel correction transforms: phi = volg / vcell - 1 to its spectrum
 // (forward), the three gradient spectra back to F_GRAD (inverse)
 void build_laplace(epic_member *m)
@@ -610,6 +626,7 @@ void build_laplace(epic_member *m)
     add_fft_xi(m->pl_lap[1], PH_OTHER, xi);
 }
 
+// enqueue a member plan on the member stream (event timers when profiling)
 int run_plan(epic_member *m, Plan &P)
 {
     hipError_t e = m->profiling ? P.run(m->s, &m->ev, &m->ev_phase) : P.run(m->s);
@@ -763,8 +780,11 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
         return EPIC_E_NOMEM;
     for (int i = 0; i < NSPEC; ++i)
         if (bad(dalloc(&m->spec[i], spec_size(g)))) return EPIC_E_NOMEM;
-    if (bad(dalloc(&m->err_d, 4)) || bad(dalloc(&m->flags_d, 4))) return EPIC_E_NOMEM;
-    if (bad(hipHostMalloc((void **)&m->err_h, sizeof(int) * 4, hipHostMallocDefault))) return EPIC_E_NOMEM;
+    if (bad(dalloc(&m->err_d, 4)) || bad(dalloc(&m->flags_d, 4)) || bad(dalloc(&m->tm_d, 1))) return EPIC_E_NOMEM;
+    if (bad(hipHostMalloc((void **)&m->err_h, sizeof(int) * 4, hipHostMallocDefault)) ||
+        bad(hipHostMalloc((void **)&m->tm_h, sizeof(StepTime), hipHostMallocDefault)))
+        return EPIC_E_NOMEM;
+    *m->tm_h = StepTime{};
     // twiddles (FP64-accurate roots of unity)
     m->px = make_plan(g.nx / 2);
     m->py = make_plan(g.ny);
@@ -796,13 +816,16 @@ int epic_destroy(epic_handle m)
     m->invalidate_plans();
     void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
                     m->cnt[2], m->scnt[2], m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
-                    m->twx, m->twy, m->twz, m->err_d, m->flags_d, m->vw_d};
+                    m->twx, m->twy, m->twz, m->err_d, m->flags_d, m->vw_d, m->tm_d, m->diag_d,
+                    m->sort_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < NSPEC; ++i)
         if (m->spec[i]) (void)hipFree(m->spec[i]);
     if (m->err_h) (void)hipHostFree(m->err_h);
     if (m->vw_h) (void)hipHostFree(m->vw_h);
+    if (m->tm_h) (void)hipHostFree(m->tm_h);
+    if (m->diag_h) (void)hipHostFree(m->diag_h);
     if (m->s) (void)hipStreamDestroy(m->s);
     delete m;
     return EPIC_OK;
@@ -1019,29 +1042,139 @@ int epic_grid2par(epic_handle m, int add)
     return check(m, "epic_grid2par");
 }
 
+// the arrays that outlive a step: positions, vorticity, B, volume,
+// buoyancy, humidity, caller index, next-stage keys, cached half axes; each
+// is gathered into the cell order through sort_tmp and copied back (stage
+// deltas and strain are rewritten before they are read again)
+static void build_resort(epic_member *m)
+{
+    Plan &P = m->pl_resort;
+    const int64_t n = m->n, c = m->cap, nc = ncells(m->g);
+    add_exclusive_scan(P, PH_SORT, {ScanArgs{m->cnt[0], nc, m->cell_start, m->bsum, m->cursor}});
+    add_place(P, PH_SORT, {PlaceArgs{n, m->keys, m->cell_start, m->cursor, m->perm}});
+    std::vector<std::pair<void *, int>> arrays;
+    for (int d = 0; d < 3; ++d) arrays.push_back({m->pos + d * c, 1});
+    for (int a = 0; a < 11; ++a) arrays.push_back({m->attr + a * c, 0});
+    arrays.push_back({m->orig, 0});
+    arrays.push_back({m->keys, 0});
+    for (int a = 0; a < 6; ++a) arrays.push_back({m->ax_stale[a], 0});
+    for (const auto &ar : arrays) {
+        add_permute(P, PH_SORT, {PermArgs{ar.first, m->sort_tmp, m->perm, n, ar.second}});
+        add_permute(P, PH_SORT, {PermArgs{m->sort_tmp, ar.first, nullptr, n, ar.second}});
+    }
+}
+
+static int resort(epic_member *m)
+{
+    m->steps_since_sort = 0;
+    if (m->n == 0) return EPIC_OK;
+    int rc = ensure_keys(m);   // keys and the histogram cnt[0] of the current positions
+    if (rc) return rc;
+    if (!m->sort_tmp) HIPOK(m, dalloc(&m->sort_tmp, m->cap));
+    if (m->pl_resort.empty()) build_resort(m);
+    rc = run_plan(m, m->pl_resort);
+    if (rc) return rc;
+    return check(m, "epic_resort");
+}
+
+// one RK step of a member: the plan (fixed or adaptive) and its StepTime
+// written before the launch; *dt_taken receives the step's dt
+static int run_step(epic_member *m, double *t, double t_initial, bool adaptive, double *dt_taken, const char *where)
+{
+    int rc = ensure_keys(m);
+    if (rc) return rc;
+    Plan &P = adaptive ? m->pl_step_adaptive : m->pl_step;
+    if (P.empty()) build_step(P, {StepItem{m, m->tm_d, m->flags_d, adaptive}});
+    // the first par2grid of a step reuses the cached axes (ls_rk.f90:93)
+    m->err_h[1] = (*t > t_initial) ? 1 : 0;
+    m->tm_h->t = *t;
+    m->tm_h->adaptive = adaptive ? 1 : 0;
+    m->tm_h->gmax_bits = m->tm_h->b2max_bits = 0ull;
+    HIPOK(m, hipMemcpyAsync(m->flags_d, m->err_h + 1, sizeof(int), hipMemcpyHostToDevice, m->s));
+    HIPOK(m, hipMemcpyAsync(m->tm_d, m->tm_h, sizeof(StepTime), hipMemcpyHostToDevice, m->s));
+    if (m->profiling) {
+        rc = run_plan(m, P);
+        if (rc) return rc;
+    } else {
+        HIPOK(m, P.run_graph(m->s));
+    }
+    HIPOK(m, hipMemcpyAsync(m->tm_h, m->tm_d, sizeof(StepTime), hipMemcpyDeviceToHost, m->s));
+    rc = check(m, where);   // synchronises the stream
+    if (rc) return rc;
+    *dt_taken = m->tm_h->dt;
+    *t = *t + m->tm_h->dt;
+    if (m->resort_every > 0 && ++m->steps_since_sort >= m->resort_every) return resort(m);
+    return EPIC_OK;
+}
+
+int epic_resort(epic_handle m)
+{
+    if (!m) return EPIC_E_INVALID;
+    return resort(m);
+}
+
+int epic_set_resort_interval(epic_handle m, int steps)
+{
+    if (!m || steps < 0) return EPIC_E_INVALID;
+    m->resort_every = steps;
+    return EPIC_OK;
+}
+
 int epic_ls_rk_step(epic_handle m, double *t, double dt, double t_initial)
 {
     if (!m || !t) return EPIC_E_INVALID;
     if (!(dt > 0.0)) return fail(m, EPIC_E_DT, "epic_ls_rk_step: time step must be positive");
-    int rc = ensure_keys(m);
-    if (rc) return rc;
-    if (m->pl_step.empty() || m->step_dt != dt) {
-        m->pl_step.release();
-        build_step(m->pl_step, {StepItem{m, dt, m->flags_d}});
-        m->step_dt = dt;
+    m->tm_h->dt = dt;
+    double taken;
+    return run_step(m, t, t_initial, false, &taken, "epic_ls_rk_step");
+}
+
+int epic_ls_rk_step_adaptive(epic_handle m, double *t, const epic_time_t *opts, double t_initial, double *dt)
+{
+    if (!m || !t || !opts) return EPIC_E_INVALID;
+    if (opts->fixed) {
+        if (!(opts->fixed_dt > 0.0)) return fail(m, EPIC_E_DT, "epic_ls_rk_step_adaptive: time step must be positive");
+        m->tm_h->dt = opts->fixed_dt;
+        double taken;
+        const int rc = run_step(m, t, t_initial, false, &taken, "epic_ls_rk_step_adaptive");
+        if (rc == EPIC_OK && dt) *dt = taken;
+        return rc;
     }
-    // the first par2grid of a step reuses the cached axes (ls_rk.f90:93)
-    m->err_h[1] = (*t > t_initial) ? 1 : 0;
-    HIPOK(m, hipMemcpyAsync(m->flags_d, m->err_h + 1, sizeof(int), hipMemcpyHostToDevice, m->s));
-    if (m->profiling) {
-        rc = run_plan(m, m->pl_step);
-        if (rc) return rc;
-    } else {
-        HIPOK(m, m->pl_step.run_graph(m->s));
-    }
-    rc = check(m, "epic_ls_rk_step");
-    if (rc == EPIC_OK) *t = *t + dt;
+    m->tm_h->alpha = opts->alpha;
+    m->tm_h->limit = opts->limit;
+    m->tm_h->precise_stop = opts->precise_stop ? 1 : 0;
+    double taken = 0.0;
+    const int rc = run_step(m, t, t_initial, true, &taken, "epic_ls_rk_step_adaptive");
+    if (rc == EPIC_OK && dt) *dt = taken;
     return rc;
+}
+
+int epic_get_time_step(epic_handle m, double t, const epic_time_t *opts, double *dt, double *gmax, double *bmax)
+{
+    if (!m || !opts || !dt) return EPIC_E_INVALID;
+    StepTime h{};
+    h.t = t;
+    h.alpha = opts->alpha;
+    h.limit = opts->limit;
+    h.precise_stop = opts->precise_stop ? 1 : 0;
+    h.adaptive = 1;
+    *m->tm_h = h;
+    HIPOK(m, hipMemcpyAsync(m->tm_d, m->tm_h, sizeof(StepTime), hipMemcpyHostToDevice, m->s));
+    {
+        Plan P;
+        add_time_step(P, PH_OTHER, {TsArgs{m->g, m->fld, m->tm_d, m->err_d}});
+        HIPOK(m, P.run(m->s));
+        HIPOK(m, hipMemcpyAsync(m->tm_h, m->tm_d, sizeof(StepTime), hipMemcpyDeviceToHost, m->s));
+        const int rc = check(m, "epic_get_time_step");
+        if (gmax) *gmax = m->tm_h->gmax;
+        if (bmax) *bmax = m->tm_h->bmax;
+        if (rc) return rc;
+    }
+    double h_dt = opts->fixed ? opts->fixed_dt : m->tm_h->dt;
+    if (opts->fixed && opts->precise_stop && (t + h_dt > opts->limit)) h_dt = opts->limit - t;
+    if (!(h_dt > 0.0)) return fail(m, EPIC_E_DT, "epic_get_time_step: Time step <= 0!");
+    *dt = h_dt;
+    return EPIC_OK;
 }
 
 int epic_set_profiling(epic_handle m, int on)
@@ -1075,8 +1208,8 @@ struct epic_batch {
     hipStream_t s = nullptr;
     int *flags_d = nullptr, *errs_d = nullptr;
     int *flags_h = nullptr, *errs_h = nullptr;
+    StepTime *tms_d = nullptr, *tms_h = nullptr;   // one time step per member
     Plan plan;
-    std::vector<double> dts;
     std::vector<uint64_t> gens;
     std::string msg;
 };
@@ -1185,7 +1318,8 @@ int epic_autotune(epic_handle m, int *tz_p2g, int *tz_g2p)
         a.stage = 1;
         a.nstages = 2;
         a.reflect = 0;
-        a.cbdt = 0.0;
+        a.cb = 0.0;
+        a.dt = &m->tm_d->dt;
         a.ca = 0.0;
         a.keys = m->keys;
         a.count = m->cnt[1];
@@ -1249,9 +1383,11 @@ int epic_batch_create(const epic_handle *members, int count, epic_batch_handle *
         return false;
     };
     if (bad(hipStreamCreate(&b->s))) return EPIC_E_HIP;
-    if (bad(dalloc(&b->flags_d, count)) || bad(dalloc(&b->errs_d, count))) return EPIC_E_NOMEM;
+    if (bad(dalloc(&b->flags_d, count)) || bad(dalloc(&b->errs_d, count)) || bad(dalloc(&b->tms_d, count)))
+        return EPIC_E_NOMEM;
     if (bad(hipHostMalloc((void **)&b->flags_h, sizeof(int) * count, hipHostMallocDefault)) ||
-        bad(hipHostMalloc((void **)&b->errs_h, sizeof(int) * count, hipHostMallocDefault)))
+        bad(hipHostMalloc((void **)&b->errs_h, sizeof(int) * count, hipHostMallocDefault)) ||
+        bad(hipHostMalloc((void **)&b->tms_h, sizeof(StepTime) * count, hipHostMallocDefault)))
         return EPIC_E_NOMEM;
     *out = b;
     return EPIC_OK;
@@ -1266,6 +1402,8 @@ int epic_batch_destroy(epic_batch_handle b)
     if (b->errs_d) (void)hipFree(b->errs_d);
     if (b->flags_h) (void)hipHostFree(b->flags_h);
     if (b->errs_h) (void)hipHostFree(b->errs_h);
+    if (b->tms_d) (void)hipFree(b->tms_d);
+    if (b->tms_h) (void)hipHostFree(b->tms_h);
     if (b->s) (void)hipStreamDestroy(b->s);
     delete b;
     return EPIC_OK;
@@ -1287,25 +1425,30 @@ int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const doub
             BHIPOK(b, hipStreamSynchronize(m->s));
         }
     }
-    bool same = b->dts.size() == (size_t)nm && b->gens.size() == (size_t)nm;
-    for (int i = 0; same && i < nm; ++i) same = (b->dts[i] == dt[i]) && (b->gens[i] == b->ms[i]->gen);
+    bool same = b->gens.size() == (size_t)nm;
+    for (int i = 0; same && i < nm; ++i) same = (b->gens[i] == b->ms[i]->gen);
     if (b->plan.empty() || !same) {
         b->plan.release();
         std::vector<StepItem> items;
         std::vector<ErrArgs> errs;
         for (int i = 0; i < nm; ++i) {
-            items.push_back(StepItem{b->ms[i], dt[i], b->flags_d + i});
+            items.push_back(StepItem{b->ms[i], b->tms_d + i, b->flags_d + i, false});
             errs.push_back(ErrArgs{b->ms[i]->err_d, b->errs_d + i});
         }
         build_step(b->plan, items);
         add_gather_errors(b->plan, errs);
-        b->dts.assign(dt, dt + nm);
         b->gens.clear();
         for (int i = 0; i < nm; ++i) b->gens.push_back(b->ms[i]->gen);
     }
     // the first par2grid of a step reuses the cached axes (ls_rk.f90:93)
-    for (int i = 0; i < nm; ++i) b->flags_h[i] = (t[i] > t_initial[i]) ? 1 : 0;
+    for (int i = 0; i < nm; ++i) {
+        b->flags_h[i] = (t[i] > t_initial[i]) ? 1 : 0;
+        b->tms_h[i] = StepTime{};
+        b->tms_h[i].dt = dt[i];
+        b->tms_h[i].t = t[i];
+    }
     BHIPOK(b, hipMemcpyAsync(b->flags_d, b->flags_h, sizeof(int) * nm, hipMemcpyHostToDevice, b->s));
+    BHIPOK(b, hipMemcpyAsync(b->tms_d, b->tms_h, sizeof(StepTime) * nm, hipMemcpyHostToDevice, b->s));
     BHIPOK(b, b->plan.run_graph(b->s));
     BHIPOK(b, hipMemcpyAsync(b->errs_h, b->errs_d, sizeof(int) * nm, hipMemcpyDeviceToHost, b->s));
     BHIPOK(b, hipStreamSynchronize(b->s));
@@ -1313,9 +1456,12 @@ int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const doub
     int first = EPIC_OK;
     for (int i = 0; i < nm; ++i) {
         epic_member *m = b->ms[i];
-        const int rc = error_code(m, b->errs_h[i], "epic_batch_step");
+        int rc = error_code(m, b->errs_h[i], "epic_batch_step");
+        if (rc == EPIC_OK) {
+            t[i] += dt[i];
+            if (m->resort_every > 0 && ++m->steps_since_sort >= m->resort_every) rc = resort(m);
+        }
         if (status) status[i] = rc;
-        if (rc == EPIC_OK) t[i] += dt[i];
         else if (first == EPIC_OK) {
             first = rc;
             b->msg = m->msg;
@@ -1372,6 +1518,7 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
         if (rc) return rc;
     }
     if (n_split) *n_split = total;
+    m->n_splits += total;
     return EPIC_OK;
 }
 
@@ -1406,6 +1553,7 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
         if (rc) return rc;
     }
     if (n_merge) *n_merge = nm;
+    m->n_merges += nm;
     return EPIC_OK;
 }
 
@@ -1487,6 +1635,79 @@ int epic_apply_vortcor(epic_handle m)
     launch_vortcor(m->pd, d, m->s);
     HIPOK(m, hipGetLastError());
     return check(m, "epic_apply_vortcor");
+}
+
+// run a diagnostics reduction and copy its nv totals to the host
+static int diag_run(epic_member *m, bool parcels, double *out, int nv, const char *where)
+{
+    const int64_t tot = diag_scratch_doubles(m->g, m->cap);
+    if (!m->diag_d) {
+        HIPOK(m, dalloc(&m->diag_d, tot));
+        HIPOK(m, hipHostMalloc((void **)&m->diag_h, sizeof(double) * 16, hipHostMallocDefault));
+    }
+    double *res = m->diag_d + tot - 16;
+    if (parcels) launch_parcel_diagnostics(m->g, m->pd, m->diag_d, res, m->err_d, m->s);
+    else launch_field_diagnostics(m->g, m->fld, m->nparg, m->nsparg, m->diag_d, res, m->s);
+    HIPOK(m, hipGetLastError());
+    HIPOK(m, hipMemcpyAsync(m->diag_h, res, sizeof(double) * nv, hipMemcpyDeviceToHost, m->s));
+    const int rc = check(m, where);   // synchronises
+    if (rc) return rc;
+    for (int c = 0; c < nv; ++c) out[c] = m->diag_h[c];
+    return EPIC_OK;
+}
+
+int epic_parcel_diagnostics(epic_handle m, double *stats, int reset_counters)
+{
+    if (!m || !stats) return EPIC_E_INVALID;
+    for (int i = 0; i < 18; ++i) stats[i] = 0.0;
+    const Grid &g = m->g;
+    if (m->n > 0) {
+        double s[12];
+        const int rc = diag_run(m, true, s, 12, "epic_parcel_diagnostics");
+        if (rc) return rc;
+        const double vdomaini = 1.0 / (g.extent[0] * g.extent[1] * g.extent[2]);
+        const double ntoti = 1.0 / (double)m->n;
+        stats[1] = 0.5 * s[0] * vdomaini;
+        stats[2] = s[6];
+        stats[3] = s[2] * ntoti;
+        stats[4] = s[4] * ntoti;
+        stats[5] = std::sqrt(std::fabs(s[3] * ntoti - stats[3] * stats[3]));
+        stats[6] = std::sqrt(std::fabs(s[5] * ntoti - stats[4] * stats[4]));
+        stats[7] = s[4];
+        for (int c = 0; c < 3; ++c) stats[8 + c] = std::sqrt(s[7 + c] / s[4]);
+        stats[12] = 0.5 * s[1] * vdomaini;
+        stats[16] = std::min(0.0, s[11]);   // the extremes start from zero (parcel_diagnostics.f90:60-61)
+        stats[17] = std::max(0.0, s[10]);
+    }
+    stats[11] = (double)m->n;
+    stats[13] = (double)m->n_splits;
+    stats[15] = (double)m->n_merges;
+    if (reset_counters) m->n_splits = m->n_merges = 0;   // as after a write (parcel_diagnostics_netcdf.f90:181-183)
+    return EPIC_OK;
+}
+
+int epic_field_diagnostics(epic_handle m, double *stats)
+{
+    if (!m || !stats) return EPIC_E_INVALID;
+    const Grid &g = m->g;
+    double s[10];
+    const int rc = diag_run(m, false, s, 10, "epic_field_diagnostics");
+    if (rc) return rc;
+    const double vdomaini = 1.0 / (g.extent[0] * g.extent[1] * g.extent[2]);
+    const double ngridi = 1.0 / ((double)g.nx * g.ny * (g.nz + 1)), ncelli = 1.0 / ((double)g.nx * g.ny * g.nz);
+    const double vcelli = 1.0 / g.vcell;
+    for (int i = 0; i < 11; ++i) stats[i] = 0.0;
+    stats[0] = std::sqrt(s[0] * ngridi) * vcelli;
+    stats[1] = s[1] * ncelli;
+    stats[2] = s[2] * ncelli;
+    stats[3] = s[3] * vdomaini;
+    stats[5] = s[4] * vdomaini;
+    stats[6] = s[5] * vcelli;
+    stats[7] = s[6];
+    stats[8] = s[7];
+    stats[9] = s[8];
+    stats[10] = s[9];
+    return EPIC_OK;
 }
 
 int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)

@@ -29,6 +29,18 @@ enum : int {
     ERR_OUT_OF_GRID = 4,    // support point outside the node grid / neighbour cells
     ERR_CELL_MISMATCH = 8,  // a sorted parcel is not in its sorted cell
     ERR_MERGE = 16,         // a small parcel found no neighbour to merge with
+    ERR_DT = 32,            // the adaptive time step is not positive
+};
+
+// the time step of one member's RK step, resident in device memory: the
+// host writes dt (fixed step) or t, alpha, limit, precise_stop (adaptive)
+// before the step; the adaptive time-step kernels fill dt, gmax and bmax
+struct StepTime {
+    double dt;
+    double t, alpha, limit;
+    double gmax, bmax;                          // maxima of the last adaptive step
+    unsigned long long gmax_bits, b2max_bits;   // running maxima (non-negative doubles as bits), kept at 0
+    int precise_stop, adaptive;
 };
 
 // field slots inside Member::fld (FP32 node fields)

@@ -289,12 +289,13 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
 #pragma unroll
             for (int c = 0; c < 5; ++c) db[c] = acc_prev ? a.p.dB[c][pi] + (float)dB[c] : (float)dB[c];
             // push (ls_rk.f90:183-191), periodic wrap (parcel_bc.f90:17-21)
+            const double cbdt = a.cb * (*a.dt);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) pos[c] = pos[c] + a.cbdt * (double)dp[c];
+            for (int c = 0; c < 3; ++c) pos[c] = pos[c] + cbdt * (double)dp[c];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) a.p.vor[c][pi] = (float)((double)a.p.vor[c][pi] + a.cbdt * (double)dv[c]);
+            for (int c = 0; c < 3; ++c) a.p.vor[c][pi] = (float)((double)a.p.vor[c][pi] + cbdt * (double)dv[c]);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) B[c] = B[c] + a.cbdt * (double)db[c];
+            for (int c = 0; c < 5; ++c) B[c] = B[c] + cbdt * (double)db[c];
             pos[0] = pos[0] - g.extent[0] * trunc((pos[0] - g.center[0]) * g.hli[0]);
             pos[1] = pos[1] - g.extent[1] * trunc((pos[1] - g.center[1]) * g.hli[1]);
             if (a.reflect) {   // parcel_bc.f90:26-40

@@ -91,7 +91,8 @@ struct G2PTileArgs {
     const int *cell_start;
     const float *fld;
     int stage, nstages, reflect;
-    double cbdt, ca;
+    double cb, ca;               // the stage pushes by cb * (*dt)
+    const double *dt;            // the step's dt (device)
     int *keys, *count, *scount;
     int *err;
     int tz, ntx, nty, nchz;
@@ -182,6 +183,38 @@ int64_t vw_partials(int64_t n);   // doubles of scratch launch_volume_weighted_v
 // out[0] = sum vol, out[1..3] = sum vol * vorticity (FP64, fixed order)
 void launch_volume_weighted_vorticity(const ParcelsDev &p, double *part, double *out, hipStream_t s);
 void launch_vortcor(const ParcelsDev &p, const double d[3], hipStream_t s);
+
+// ---- adaptive time step (timestep.hip) ----
+struct TsArgs {
+    Grid g;
+    const float *fld;
+    StepTime *tm;
+    int *err;
+};
+// largest strain eigenvalue magnitude and squared buoyancy gradient over the
+// nodes, then dt (two launches; nothing happens for a fixed-step item)
+void add_time_step(Plan &P, int phase, const std::vector<TsArgs> &items);
+
+// ---- diagnostics (diagnostics.hip); direct launches on s ----
+int64_t diag_scratch_doubles(const Grid &g, int64_t cap);
+// out[0..11]: sums of vol |u|^2, vol |w|^2, lam, lam^2, vol, vol^2, small,
+// vol xi^2, vol eta^2, vol zeta^2; max and min buoyancy
+void launch_parcel_diagnostics(const Grid &g, const ParcelsDev &p, double *scratch, double *out, int *err,
+                               hipStream_t s);
+// out[0..9]: sums of (volg - vcell)^2, nparg, nsparg, weighted volg |u|^2,
+// volg |w|^2; max |volg - vcell|, nparg, tbuoyg; min nparg, tbuoyg
+void launch_field_diagnostics(const Grid &g, const float *fld, const int *nparg, const int *nsparg, double *scratch,
+                              double *out, hipStream_t s);
+
+// ---- physical re-sort (resort.hip) ----
+struct PermArgs {
+    const void *src;
+    void *dst;
+    const int *perm;        // dst[s] = src[perm[s]]; null: a plain copy
+    int64_t n;
+    int wide;               // 8-byte elements, else 4-byte
+};
+void add_permute(Plan &P, int phase, const std::vector<PermArgs> &items);
 
 // ---- utilities (abi.hip) ----
 struct FillArgs {

@@ -7,7 +7,9 @@
 !   grid2par(add)           parcel_interpl.f90:405
 !   vor2vel()               inversion.f90:25
 !   vorticity_tendency()    inversion.f90:321
-!   ls_rk_step(t)           ls_rk.f90:88   (fixed time step)
+!   ls_rk_step(t)           ls_rk.f90:88   (fixed, or adaptive after
+!                           epic_gpu_set_adaptive_dt)
+!   get_time_step(t)        rk_utils.f90:73
 !   parcel_split()          parcel_split.f90:37
 !   parcel_merge()          parcel_merge.f90:48
 !   init_parcel_correction  parcel_correction.f90:42
@@ -41,6 +43,13 @@ module epic_gpu_mod
         integer(c_int) :: rk_order
         real(c_double) :: vmin
     end type epic_flags_t
+
+    ! epic_time_t: time options (options.f90:99-107)
+    type, bind(C) :: epic_time_t
+        integer(c_int) :: fixed
+        real(c_double) :: fixed_dt, alpha, limit
+        integer(c_int) :: precise_stop
+    end type epic_time_t
 
     integer(c_int), parameter, public :: EPIC_FIELD_VOLG = 0, EPIC_FIELD_VORTG = 1, EPIC_FIELD_TBUOYG = 2, &
                                          EPIC_FIELD_DBUOYG = 3, EPIC_FIELD_HUMG = 4, EPIC_FIELD_VELOG = 5, &
@@ -105,6 +114,22 @@ module epic_gpu_mod
             real(c_double), intent(inout) :: t
             real(c_double), value       :: dt, t_initial
         end function
+        integer(c_int) function c_epic_ls_rk_step_adaptive(h, t, opts, t_initial, dt) &
+                bind(C, name="epic_ls_rk_step_adaptive")
+            import :: c_int, c_ptr, c_double, epic_time_t
+            type(c_ptr), value            :: h
+            real(c_double), intent(inout) :: t
+            type(epic_time_t), intent(in) :: opts
+            real(c_double), value         :: t_initial
+            real(c_double), intent(out)   :: dt
+        end function
+        integer(c_int) function c_epic_get_time_step(h, t, opts, dt, gmax, bmax) bind(C, name="epic_get_time_step")
+            import :: c_int, c_ptr, c_double, epic_time_t
+            type(c_ptr), value            :: h
+            real(c_double), value         :: t
+            type(epic_time_t), intent(in) :: opts
+            real(c_double), intent(out)   :: dt, gmax, bmax
+        end function
         integer(c_int) function c_epic_parcel_split(h, lambda_max, amax, n_split) bind(C, name="epic_parcel_split")
             import :: c_int, c_ptr, c_double, c_int64_t
             type(c_ptr), value              :: h
@@ -151,10 +176,12 @@ module epic_gpu_mod
     type(c_ptr) :: handle = c_null_ptr
     integer(c_int64_t) :: num_parcels = 0
     double precision :: fixed_dt = 0.0d0, time_initial = 0.0d0
+    logical :: adaptive_dt = .false.
+    type(epic_time_t) :: time_opts
 
     public :: epic_gpu_init, epic_gpu_finalise, epic_gpu_upload_parcels, epic_gpu_download_parcels, &
               epic_gpu_get_field, epic_gpu_num_parcels, par2grid, grid2par, vor2vel, vorticity_tendency, &
-              ls_rk_step, parcel_split, parcel_merge, init_parcel_correction, apply_vortcor, apply_laplace, &
+              ls_rk_step, get_time_step, epic_gpu_set_adaptive_dt, parcel_split, parcel_merge, init_parcel_correction, apply_vortcor, apply_laplace, &
               apply_gradient
 
 contains
@@ -252,13 +279,45 @@ contains
         call check(c_epic_vorticity_tendency(handle), "vorticity_tendency")
     end subroutine vorticity_tendency
 
+    ! time%l_use_fixed_dt = .false.: the adaptive step of get_time_step with
+    ! time%alpha, time%limit and time%precise_stop
+    subroutine epic_gpu_set_adaptive_dt(alpha, limit, precise_stop)
+        double precision, intent(in) :: alpha, limit
+        logical, intent(in)          :: precise_stop
+        adaptive_dt = .true.
+        time_opts%fixed = 0
+        time_opts%fixed_dt = fixed_dt
+        time_opts%alpha = alpha
+        time_opts%limit = limit
+        time_opts%precise_stop = merge(1, 0, precise_stop)
+    end subroutine epic_gpu_set_adaptive_dt
+
     subroutine ls_rk_step(t)
         double precision, intent(inout) :: t
-        real(c_double) :: tt
+        real(c_double) :: tt, dt
         tt = t
-        call check(c_epic_ls_rk_step(handle, tt, fixed_dt, time_initial), "ls_rk_step")
+        if (adaptive_dt) then
+            call check(c_epic_ls_rk_step_adaptive(handle, tt, time_opts, time_initial, dt), "ls_rk_step")
+        else
+            call check(c_epic_ls_rk_step(handle, tt, fixed_dt, time_initial), "ls_rk_step")
+        endif
         t = tt
     end subroutine ls_rk_step
+
+    ! get_time_step from the current fields (after par2grid and vor2vel)
+    function get_time_step(t) result(dt)
+        double precision, intent(in) :: t
+        double precision :: dt
+        type(epic_time_t) :: o
+        real(c_double) :: h, gmax, bmax
+        o = time_opts
+        if (.not. adaptive_dt) then
+            o%fixed = 1
+            o%fixed_dt = fixed_dt
+        endif
+        call check(c_epic_get_time_step(handle, t, o, h, gmax, bmax), "get_time_step")
+        dt = h
+    end function get_time_step
 
     ! parcel_split with the reference's defaults (parcel%lambda_max = 4,
     ! amax from the grid); the parcel count grows by the number of splits

@@ -124,6 +124,30 @@ int epic_grid2par(epic_handle h, int add);
  * t > t_initial. */
 int epic_ls_rk_step(epic_handle h, double *t, double dt, double t_initial);
 
+/* Time options (options.f90:99-107): fixed (time%l_use_fixed_dt) with
+ * fixed_dt, else the adaptive step with alpha (default 0.2); with
+ * precise_stop the step is cut at limit. */
+typedef struct {
+    int fixed;
+    double fixed_dt;
+    double alpha;
+    double limit;
+    int precise_stop;
+} epic_time_t;
+/* get_time_step(t) (rk_utils.f90:73-183) from the current velgradg and
+ * tbuoyg (after epic_vor2vel and epic_par2grid): dt = min(alpha/gmax,
+ * alpha/bmax), gmax the largest |eigenvalue| of the symmetrised velocity
+ * gradient over the nodes, bmax the fourth root of the largest squared
+ * buoyancy gradient (central differences), both at least epsilon; the fixed
+ * branch returns fixed_dt.  dt <= 0 fails with EPIC_E_DT.  gmax and bmax
+ * (may be NULL) receive the maxima (adaptive branch). */
+int epic_get_time_step(epic_handle h, double t, const epic_time_t *opts, double *dt, double *gmax, double *bmax);
+/* ls_rk_step(t) with the time step of get_time_step, taken after the first
+ * vorticity_tendency as in ls_rk.f90:88-136; the whole step, time step
+ * included, is one captured HIP graph.  *dt (may be NULL) receives the step
+ * taken; t is advanced by it. */
+int epic_ls_rk_step_adaptive(epic_handle h, double *t, const epic_time_t *opts, double t_initial, double *dt);
+
 /* parcel_split (parcel_split.f90:37-197): every parcel whose aspect ratio
  * sqrt(D1/D3) >= lambda_max or whose largest eigenvalue D1 >= amax^2 splits
  * along its major axis into two parcels of half the volume, with B reduced by
@@ -168,6 +192,32 @@ int epic_init_parcel_correction(epic_handle h, double *vor_bar);
 /* apply_vortcor (parcel_correction.f90:89-138): shifts every parcel's
  * vorticity so that the volume-weighted mean equals the stored target. */
 int epic_apply_vortcor(epic_handle h);
+
+/* calculate_parcel_diagnostics (parcel_diagnostics.f90:51-159) on one rank
+ * with ape_calculation = 'none': stats[18] in the reference's IDX order
+ * (ape, ke, n_small, avg_lam, avg_vol, std_lam, std_vol, sum_vol, rms_xi,
+ * rms_eta, rms_zeta, ntot_par, enstrophy, nsplits, nbig_iclo, nmerges,
+ * min_buoy, max_buoy).  The velocity is delta_pos, so call it after
+ * epic_grid2par, as ls_rk_step does (ls_rk.f90:108-110).  nsplits and
+ * nmerges count epic_parcel_split / epic_parcel_merge since the last reset;
+ * reset_counters != 0 clears them (parcel_diagnostics_netcdf.f90:181-183);
+ * nbig_iclo is not tracked (0).  Sums are FP64 in a fixed order. */
+int epic_parcel_diagnostics(epic_handle h, double *stats, int reset_counters);
+/* calculate_field_diagnostics (field_diagnostics.f90:40-154) on one rank
+ * from volg, nparg, nsparg, tbuoyg, velog and vortg (after epic_par2grid and
+ * epic_vor2vel): stats[11] = rms_v, avg_npar, avg_nspar, keg, apeg (0),
+ * eng, abserr_v, max_npar, max_buoy, min_npar, min_buoy. */
+int epic_field_diagnostics(epic_handle h, double *stats);
+
+/* Physical re-sort: the parcel arrays are reordered into the cell order of
+ * the current positions, so that the permutation gathers of the step stay
+ * coalesced as parcels drift.  Not observable through the API
+ * (epic_get_parcels returns the caller's order); the stage tendencies are
+ * not kept.  No counterpart in the reference, whose parcels are not
+ * sorted.  epic_ls_rk_step and epic_batch_step re-sort a member every
+ * `steps` steps (10 by default; 0 never). */
+int epic_resort(epic_handle h);
+int epic_set_resort_interval(epic_handle h, int steps);
 
 /* Cell counting sort of the parcels' current positions with the reference's
  * key ijk = 1 + ix + hx iy + hx hy iz on the halo box

@@ -1513,19 +1513,70 @@ static int substep(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, 
     return orc_par2grid(d, p, f, -1);
 }
 
-int orc_ls_rk_step(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, double *t, double dt,
-                   double t_initial, int rk_order)
+/* get_strain (rk_utils.f90:30-66): the symmetrised velocity gradient, dw/dz
+ * from incompressibility */
+static void get_strain(const double g[8], double S[9])
+{
+    S[0] = g[0];
+    S[1] = f12 * (g[1] + g[3]);
+    S[2] = f12 * (g[2] + g[6]);
+    S[3] = S[1];
+    S[4] = g[4];
+    S[5] = f12 * (g[5] + g[7]);
+    S[6] = S[2];
+    S[7] = S[5];
+    S[8] = -(g[0] + g[4]);
+}
+
+int orc_get_time_step(const orc_dom *d, const orc_fields *f, double t, const orc_time *tm, double *dt,
+                      double *gmax_out, double *bmax_out)
+{
+    const double eps = 2.220446049250313e-16;   /* epsilon(1d0) */
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const int64_t H = orc_halo_size(d);
+    double gmax = eps, b2max = zero;
+#pragma omp parallel for schedule(static) num_threads(nthreads_of(d)) reduction(max : gmax, b2max)
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int iz = 0; iz <= nz; ++iz) {
+                double g[8], S[9], D[3];
+                for (int c = 0; c < 8; ++c) g[c] = f->velgradg[c * H + HIDX(d, iz, iy, ix)];
+                get_strain(g, S);
+                orc_scherzinger_eigenvalues(S, D);
+                for (int c = 0; c < 3; ++c) gmax = fmax(gmax, fabs(D[c]));
+                /* central differences of tbuoyg; x and y periodic (field_halo_fill_scalar) */
+                const int xp = (ix + 1) % nx, xm = (ix + nx - 1) % nx;
+                const int yp = (iy + 1) % ny, ym = (iy + ny - 1) % ny;
+                const double gx = f12 * d->dxi[0] * (f->tbuoyg[HIDX(d, iz, iy, xp)] - f->tbuoyg[HIDX(d, iz, iy, xm)]);
+                const double gy = f12 * d->dxi[1] * (f->tbuoyg[HIDX(d, iz, yp, ix)] - f->tbuoyg[HIDX(d, iz, ym, ix)]);
+                const double gz = f12 * d->dxi[2] * (f->tbuoyg[HIDX(d, iz + 1, iy, ix)] - f->tbuoyg[HIDX(d, iz - 1, iy, ix)]);
+                b2max = fmax(b2max, gx * gx + gy * gy + gz * gz);
+            }
+    double bmax = fmax(eps, sqrt(sqrt(b2max)));
+    if (gmax_out) *gmax_out = gmax;
+    if (bmax_out) *bmax_out = bmax;
+    double h = tm->fixed ? tm->fixed_dt : fmin(tm->alpha / gmax, tm->alpha / bmax);
+    if (tm->precise_stop && (t + h > tm->limit)) h = tm->limit - t;
+    *dt = h;
+    return (h <= zero) ? ORC_E_DT : ORC_OK;
+}
+
+static int rk_step(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, double *t, double dt_fixed,
+                   const orc_time *tm, double t_initial, int rk_order, double *dt_out)
 {
     const double *ca, *cb;
     int ns;
     if (rk_order == 4) { ca = cas4; cb = cbs4; ns = 5; }
     else if (rk_order == 3) { ca = cas3; cb = cbs3; ns = 3; }
     else return ORC_E_RK_ORDER;
-    if (!(dt > 0.0)) return ORC_E_DT;
+    if (!tm && !(dt_fixed > 0.0)) return ORC_E_DT;
     int rc;
     if ((rc = orc_par2grid(d, p, f, (*t > t_initial) ? 1 : 0))) return rc;
     if ((rc = orc_vor2vel(d, v, f))) return rc;
     if ((rc = orc_vorticity_tendency(d, v, f))) return rc;
+    double dt = dt_fixed;
+    if (tm && (rc = orc_get_time_step(d, f, *t, tm, &dt, NULL, NULL))) return rc;
+    if (dt_out) *dt_out = dt;
     if ((rc = orc_grid2par(d, p, f, -1))) return rc;
     for (int s = 1; s <= ns; ++s)
         if ((rc = substep(d, v, p, f, dt, s, ns, ca, cb))) return rc;
@@ -1535,6 +1586,119 @@ int orc_ls_rk_step(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, 
     g_timers.push += now_s() - t0;
     *t = *t + dt;
     return ORC_OK;
+}
+
+int orc_ls_rk_step(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, double *t, double dt,
+                   double t_initial, int rk_order)
+{
+    return rk_step(d, v, p, f, t, dt, NULL, t_initial, rk_order, NULL);
+}
+
+int orc_ls_rk_step_adaptive(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, double *t,
+                            const orc_time *tm, double t_initial, int rk_order, double *dt)
+{
+    return rk_step(d, v, p, f, t, 0.0, tm, t_initial, rk_order, dt);
+}
+
+/* ------------------------------------------------------------------------ */
+/* diagnostics: parcel_diagnostics.f90:51-159, field_diagnostics.f90:40-154  */
+/* ------------------------------------------------------------------------ */
+void orc_parcel_diagnostics(const orc_dom *d, const orc_parcels *p, const double counters[3], double stats[18])
+{
+    double ke = zero, ens = zero, lam1 = zero, lam2 = zero, v1 = zero, v2 = zero, rx = zero, ry = zero, rz = zero;
+    double nsmall = zero, bmin = zero, bmax = zero;
+    const double vmin = d->vmin > 0.0 ? d->vmin : d->vcell / 20.0;
+    for (int64_t n = 0; n < p->n; ++n) {
+        const double *vel = p->delta_pos + 3 * n, *vor = p->vorticity + 3 * n;
+        const double vol = p->volume[n], b = p->buoyancy[n];
+        ke += (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]) * vol;
+        ens += (vor[0] * vor[0] + vor[1] * vor[1] + vor[2] * vor[2]) * vol;
+        double A[9], D[3], b33;
+        const double *B = p->B + 5 * n;
+        if (orc_get_B33(B, vol, &b33) != ORC_OK) b33 = zero;
+        A[0] = B[B11]; A[1] = B[B12]; A[2] = B[B13];
+        A[3] = B[B12]; A[4] = B[B22]; A[5] = B[B23];
+        A[6] = B[B13]; A[7] = B[B23]; A[8] = b33;
+        orc_scherzinger_eigenvalues(A, D);
+        const double lam = sqrt(D[0] / D[2]);   /* get_aspect_ratio (parcel_ellipsoid.f90:423-429) */
+        lam1 += lam;
+        lam2 += lam * lam;
+        v1 += vol;
+        v2 += vol * vol;
+        if (vol <= vmin) nsmall += one;
+        bmax = fmax(bmax, b);
+        bmin = fmin(bmin, b);
+        rx += vol * vor[0] * vor[0];
+        ry += vol * vor[1] * vor[1];
+        rz += vol * vor[2] * vor[2];
+    }
+    const double vdomaini = one / (d->extent[0] * d->extent[1] * d->extent[2]);
+    const double ntot = (double)p->n, ntoti = one / ntot;
+    for (int i = 0; i < 18; ++i) stats[i] = zero;
+    stats[1] = f12 * ke * vdomaini;
+    stats[2] = nsmall;
+    stats[3] = lam1 * ntoti;
+    stats[5] = sqrt(fabs(lam2 * ntoti - stats[3] * stats[3]));
+    stats[7] = v1;
+    stats[8] = sqrt(rx / v1);
+    stats[9] = sqrt(ry / v1);
+    stats[10] = sqrt(rz / v1);
+    stats[11] = ntot;
+    stats[12] = f12 * ens * vdomaini;
+    stats[13] = counters[0];
+    stats[14] = counters[1];
+    stats[15] = counters[2];
+    stats[4] = v1 * ntoti;
+    stats[6] = sqrt(fabs(v2 * ntoti - stats[4] * stats[4]));
+    stats[16] = bmin;
+    stats[17] = bmax;
+}
+
+void orc_field_diagnostics(const orc_dom *d, const orc_fields *f, double stats[11])
+{
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const int64_t H = orc_halo_size(d);
+    const double vcell = d->vcell, vcelli = one / vcell;
+    double rms = zero, aerr = zero, np = zero, nsp = zero, ke = zero, en = zero;
+    double npmax = -1e300, npmin = 1e300, bmax = -1e300, bmin = 1e300;
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int iz = 0; iz <= nz; ++iz) {
+                const int64_t o = HIDX(d, iz, iy, ix);
+                const double v = f->volg[o];
+                rms += (v - vcell) * (v - vcell);
+                aerr = fmax(aerr, fabs(v - vcell));
+                bmax = fmax(bmax, f->tbuoyg[o]);
+                bmin = fmin(bmin, f->tbuoyg[o]);
+                if (iz < nz) {
+                    np += f->nparg[o];
+                    nsp += f->nsparg[o];
+                    npmax = fmax(npmax, (double)f->nparg[o]);
+                    npmin = fmin(npmin, (double)f->nparg[o]);
+                }
+                /* half weights for the boundary planes */
+                const double w = (iz == 0 || iz == nz) ? f14 : f12;
+                double u2 = zero, w2 = zero;
+                for (int c = 0; c < 3; ++c) {
+                    u2 += f->velog[c * H + o] * f->velog[c * H + o];
+                    w2 += f->vortg[c * H + o] * f->vortg[c * H + o];
+                }
+                ke += w * v * u2;
+                en += w * v * w2;
+            }
+    const double vdomaini = one / (d->extent[0] * d->extent[1] * d->extent[2]);
+    const double ngridi = one / ((double)nx * ny * (nz + 1)), ncelli = one / ((double)nx * ny * nz);
+    for (int i = 0; i < 11; ++i) stats[i] = zero;
+    stats[0] = sqrt(rms * ngridi) * vcelli;
+    stats[1] = np * ncelli;
+    stats[2] = nsp * ncelli;
+    stats[3] = ke * vdomaini;
+    stats[5] = en * vdomaini;
+    stats[6] = aerr * vcelli;
+    stats[7] = npmax;
+    stats[8] = bmax;
+    stats[9] = npmin;
+    stats[10] = bmin;
 }
 
 /* ------------------------------------------------------------------------ */

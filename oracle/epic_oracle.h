@@ -116,6 +116,23 @@ void orc_apply_reflective_bc(const orc_dom *d, double pos[3], double B[5]);
 int orc_ls_rk_step(const orc_dom *d, orc_inv *inv, orc_parcels *p, orc_fields *f,
                    double *t, double dt, double t_initial, int rk_order);
 
+/* time options (options.f90:99-107): fixed dt or adaptive with alpha */
+typedef struct {
+    int fixed;              /* time%l_use_fixed_dt */
+    double fixed_dt, alpha, limit;
+    int precise_stop;
+} orc_time;
+/* get_time_step (rk_utils.f90:73-183) on one rank: dt = min(alpha/gmax,
+ * alpha/bmax) from the largest strain eigenvalue magnitude and the fourth
+ * root of the largest squared buoyancy gradient on the nodes; gmax and bmax
+ * (may be NULL) receive the two maxima */
+int orc_get_time_step(const orc_dom *d, const orc_fields *f, double t, const orc_time *tm, double *dt,
+                      double *gmax, double *bmax);
+/* ls_rk_step with the time step of get_time_step (ls_rk.f90:88-136); *dt
+ * receives the step taken */
+int orc_ls_rk_step_adaptive(const orc_dom *d, orc_inv *inv, orc_parcels *p, orc_fields *f, double *t,
+                            const orc_time *tm, double t_initial, int rk_order, double *dt);
+
 /* stability polynomial of the low-storage scheme (test of the coefficients) */
 double orc_lsrk_linear(double lam, int rk_order);
 
@@ -142,6 +159,17 @@ int orc_apply_gradient(const orc_dom *d, orc_parcels *p, orc_fields *f, double p
                        int l_reuse);
 void orc_volume_weighted_vorticity(const orc_parcels *p, double vor_bar[3]);
 void orc_apply_vortcor(orc_parcels *p, const double vor_bar[3]);
+
+/* ---- diagnostics (parcel_diagnostics.f90:51-159, field_diagnostics.f90:40-154), one rank,
+ * ape_calculation = 'none' (the APE entries stay 0) ---- */
+/* stats[0..17] in the order of the reference's IDX_* (1-based there):
+ * ape, ke, n_small, avg_lam, avg_vol, std_lam, std_vol, sum_vol, rms_xi,
+ * rms_eta, rms_zeta, ntot_par, enstrophy, nsplits, nbig_iclo, nmerges,
+ * min_buoy, max_buoy; the three counters are taken from counters[3] */
+void orc_parcel_diagnostics(const orc_dom *d, const orc_parcels *p, const double counters[3], double stats[18]);
+/* stats[0..10]: rms_v, avg_npar, avg_nspar, keg, apeg, eng, abserr_v,
+ * max_npar, max_buoy, min_npar, min_buoy */
+void orc_field_diagnostics(const orc_dom *d, const orc_fields *f, double stats[11]);
 
 int64_t orc_nearest_ncell(const orc_dom *d);
 int orc_nearest_cell_sort(const orc_dom *d, int64_t n, const double *position,

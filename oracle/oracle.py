@@ -72,6 +72,11 @@ class Fields(C.Structure):
     ] + [("nparg", C.POINTER(C.c_int)), ("nsparg", C.POINTER(C.c_int))]
 
 
+class Time(C.Structure):
+    _fields_ = [("fixed", C.c_int), ("fixed_dt", C.c_double), ("alpha", C.c_double), ("limit", C.c_double),
+                ("precise_stop", C.c_int)]
+
+
 class Timers(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("par2grid", "grid2par", "vor2vel", "vtend", "push")]
 
@@ -116,6 +121,12 @@ def _declare(L):
     L.orc_apply_reflective_bc.argtypes = [P(Dom), P(d), P(d)]
     L.orc_apply_reflective_bc.restype = None
     L.orc_ls_rk_step.argtypes = [P(Dom), C.c_void_p, P(Parcels), P(Fields), P(d), d, d, C.c_int]
+    L.orc_get_time_step.argtypes = [P(Dom), P(Fields), d, P(Time), P(d), P(d), P(d)]
+    L.orc_ls_rk_step_adaptive.argtypes = [P(Dom), C.c_void_p, P(Parcels), P(Fields), P(d), P(Time), d, C.c_int, P(d)]
+    L.orc_parcel_diagnostics.argtypes = [P(Dom), P(Parcels), P(d), P(d)]
+    L.orc_parcel_diagnostics.restype = None
+    L.orc_field_diagnostics.argtypes = [P(Dom), P(Fields), P(d)]
+    L.orc_field_diagnostics.restype = None
     L.orc_lsrk_linear.argtypes = [d, C.c_int]
     L.orc_lsrk_linear.restype = d
     L.orc_get_timers.argtypes = [P(Timers)]
@@ -385,6 +396,44 @@ class State:
 
     def vorticity_tendency(self):
         _chk(lib().orc_vorticity_tendency(C.byref(self.dom), self.inv(), C.byref(self._fields())))
+
+    def get_time_step(self, t, alpha=0.2, limit=0.0, precise_stop=False, fixed_dt=None):
+        """get_time_step (rk_utils.f90:73-183) from the current velgradg and
+        tbuoyg; returns (dt, gmax, bmax)."""
+        tm = Time(1 if fixed_dt is not None else 0, fixed_dt or 0.0, alpha, limit, int(bool(precise_stop)))
+        dt, g, b = C.c_double(0), C.c_double(0), C.c_double(0)
+        _chk(lib().orc_get_time_step(C.byref(self.dom), C.byref(self._fields()), t, C.byref(tm), C.byref(dt),
+                                     C.byref(g), C.byref(b)))
+        return dt.value, g.value, b.value
+
+    def ls_rk_step_adaptive(self, t, alpha=0.2, limit=0.0, precise_stop=False, t_initial=0.0, rk_order=4):
+        """ls_rk_step with get_time_step (ls_rk.f90:88-136); returns (t, dt)."""
+        tm = Time(0, 0.0, alpha, limit, int(bool(precise_stop)))
+        tt, dt = C.c_double(t), C.c_double(0)
+        _chk(lib().orc_ls_rk_step_adaptive(C.byref(self.dom), self.inv(), C.byref(self._parcels()),
+                                           C.byref(self._fields()), C.byref(tt), C.byref(tm), t_initial, rk_order,
+                                           C.byref(dt)))
+        return tt.value, dt.value
+
+    PARCEL_STATS = ("ape", "ke", "n_small", "avg_lam", "avg_vol", "std_lam", "std_vol", "sum_vol", "rms_xi",
+                    "rms_eta", "rms_zeta", "ntot_par", "enstrophy", "nsplits", "nbig_iclo", "nmerges",
+                    "min_buoy", "max_buoy")
+    FIELD_STATS = ("rms_v", "avg_npar", "avg_nspar", "keg", "apeg", "eng", "abserr_v", "max_npar", "max_buoy",
+                   "min_npar", "min_buoy")
+
+    def parcel_diagnostics(self, counters=(0, 0, 0)):
+        """calculate_parcel_diagnostics (parcel_diagnostics.f90:51-159); the
+        velocity is delta_pos (after grid2par)."""
+        out = np.zeros(18)
+        c = np.asarray(counters, dtype=np.float64)
+        lib().orc_parcel_diagnostics(C.byref(self.dom), C.byref(self._parcels()), ptr(c), ptr(out))
+        return dict(zip(self.PARCEL_STATS, out))
+
+    def field_diagnostics(self):
+        """calculate_field_diagnostics (field_diagnostics.f90:40-154)."""
+        out = np.zeros(11)
+        lib().orc_field_diagnostics(C.byref(self.dom), C.byref(self._fields()), ptr(out))
+        return dict(zip(self.FIELD_STATS, out))
 
     def ls_rk_step(self, t, dt, t_initial=0.0, rk_order=4):
         tt = C.c_double(t)
