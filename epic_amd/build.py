@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["abi.hip", "plan.hip", "p2g.hip", "g2p.hip", "fft.hip", "inversion.hip", "split.hip", "merge.hip",
            "correction.hip", "timestep.hip", "diagnostics.hip",
-           "resort.hip"]
+           "resort.hip", "ncio.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -39,7 +39,7 @@ def build(verbose=False, jobs=5):
     cc = hipcc()
     objs, procs = [], []
     for src in SOURCES:
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         cmd = [cc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

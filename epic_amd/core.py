@@ -90,6 +90,14 @@ def load_library(path=LIB_PATH):
         "epic_ls_rk_step_adaptive": ([h, dp, P(TimeOpts), d, dp], i),
         "epic_parcel_diagnostics": ([h, dp, i], i),
         "epic_resort": ([h], i),
+        "epic_get_config": ([h, P(Grid), P(Phys), P(Flags)], i),
+        "epic_write_parcels_nc": ([h, C.c_char_p, d], i),
+        "epic_read_parcels_nc": ([h, C.c_char_p, dp], i),
+        "epic_write_fields_nc": ([h, C.c_char_p, d], i),
+        "epic_nc_write_parcels": ([C.c_char_p, P(Grid), P(Phys), P(Flags), i64, dp, dp, dp, dp, dp, dp, d], i),
+        "epic_nc_read_parcels": ([C.c_char_p, i64, P(i64), P(Grid), dp, dp, dp, dp, dp, dp, dp], i),
+        "epic_nc_write_fields": ([C.c_char_p, P(Grid), P(Phys), P(Flags), d] + [dp] * 9, i),
+        "epic_nc_last_error": ([], C.c_char_p),
         "epic_set_resort_interval": ([h, i], i),
         "epic_field_diagnostics": ([h, dp], i),
         "epic_cell_sort": ([h, ip, ip, ip, ip], i),
@@ -326,6 +334,22 @@ class Member:
         """apply_vortcor (parcel_correction.f90:89-138)."""
         self._chk(load_library().epic_apply_vortcor(self._h))
 
+    # ---- NetCDF files (NetCDF classic format, the reference's schema) ----
+    def write_parcels_nc(self, path, t):
+        """write_netcdf_parcels (parcel_netcdf.f90:201-309)."""
+        self._chk(load_library().epic_write_parcels_nc(self._h, str(path).encode(), t))
+
+    def read_parcels_nc(self, path):
+        """read_netcdf_parcels (parcel_netcdf.f90:311-724); returns the time."""
+        t = C.c_double(0)
+        self._chk(load_library().epic_read_parcels_nc(self._h, str(path).encode(), C.byref(t)))
+        self.n = int(load_library().epic_num_parcels(self._h))
+        return t.value
+
+    def write_fields_nc(self, path, t):
+        """write_netcdf_fields (field_netcdf.f90:182-266)."""
+        self._chk(load_library().epic_write_fields_nc(self._h, str(path).encode(), t))
+
     # ---- physical re-sort ----
     def resort(self):
         """Reorder the parcel arrays into cell order (not observable)."""
@@ -349,6 +373,58 @@ def member_bytes(nx, ny, nz, max_parcels):
     """Device bytes a member of this grid and capacity holds."""
     g = Grid(nx, ny, nz, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
     return int(load_library().epic_member_bytes(C.byref(g), int(max_parcels)))
+
+
+def _nc_chk(rc):
+    if rc != 0:
+        raise EpicError(rc, load_library().epic_nc_last_error().decode())
+
+
+def _config(nx, ny, nz, lower, extent, glat=0.0, lambda_c=0.0, q0=0.0, f_cor=(0.0, 0.0, 0.0),
+            zeta_zero=(False, False)):
+    g = Grid(nx, ny, nz, (C.c_double * 3)(*lower), (C.c_double * 3)(*extent))
+    p = Phys(glat, lambda_c, q0, (C.c_double * 3)(*f_cor))
+    f = Flags((C.c_int * 2)(*[int(bool(z)) for z in zeta_zero]), 4, 0.0)
+    return g, p, f
+
+
+def nc_write_parcels(path, grid, parcels, t, phys=None):
+    """Parcel file from host arrays (epic_nc_write_parcels); grid = (nx, ny,
+    nz, lower, extent); parcels = dict of position (n,3), vorticity (n,3),
+    B (n,5), volume, buoyancy, humidity (n,)."""
+    g, p, f = _config(*grid, **(phys or {}))
+    arrs = [np.ascontiguousarray(parcels[k], dtype=np.float64)
+            for k in ("position", "vorticity", "B", "volume", "buoyancy", "humidity")]
+    n = arrs[0].shape[0]
+    _nc_chk(load_library().epic_nc_write_parcels(str(path).encode(), C.byref(g), C.byref(p), C.byref(f), n,
+                                                 *[_dp(a) for a in arrs], t))
+
+
+def nc_read_parcels(path):
+    """Parcel file to host arrays (epic_nc_read_parcels); returns (parcels,
+    (nx, ny, nz, lower, extent), t)."""
+    L = load_library()
+    n, t, g = C.c_int64(0), C.c_double(0), Grid()
+    _nc_chk(L.epic_nc_read_parcels(str(path).encode(), 0, C.byref(n), C.byref(g), None, None, None, None, None,
+                                   None, C.byref(t)))
+    m = n.value
+    out = {"position": np.zeros((m, 3)), "vorticity": np.zeros((m, 3)), "B": np.zeros((m, 5)),
+           "volume": np.zeros(m), "buoyancy": np.zeros(m), "humidity": np.zeros(m)}
+    _nc_chk(L.epic_nc_read_parcels(str(path).encode(), m, C.byref(n), None,
+                                   *[_dp(out[k]) for k in ("position", "vorticity", "B", "volume", "buoyancy",
+                                                           "humidity")], C.byref(t)))
+    return out, (g.nx, g.ny, g.nz, tuple(g.lower), tuple(g.extent)), t.value
+
+
+def nc_write_fields(path, grid, fields, t, phys=None):
+    """Field file from halo-layout host arrays (epic_nc_write_fields);
+    fields = dict velog (3,H), vtend (3,H), vortg (3,H), tbuoyg, dbuoyg, humg,
+    volg, nparg, nsparg (H) with H = (nx+3)(ny+3)(nz+3)."""
+    g, p, f = _config(*grid, **(phys or {}))
+    names = ("velog", "vtend", "vortg", "tbuoyg", "dbuoyg", "humg", "volg", "nparg", "nsparg")
+    arrs = [np.ascontiguousarray(fields[k], dtype=np.float64) for k in names]
+    _nc_chk(load_library().epic_nc_write_fields(str(path).encode(), C.byref(g), C.byref(p), C.byref(f), t,
+                                                *[_dp(a) for a in arrs]))
 
 
 class Batch:

@@ -1710,6 +1710,34 @@ int epic_field_diagnostics(epic_handle m, double *stats)
     return EPIC_OK;
 }
 
+int epic_get_config(epic_handle m, epic_grid_t *grid, epic_phys_t *phys, epic_flags_t *flags)
+{
+    if (!m) return EPIC_E_INVALID;
+    const Grid &g = m->g;
+    if (grid) {
+        grid->nx = g.nx;
+        grid->ny = g.ny;
+        grid->nz = g.nz;
+        for (int d = 0; d < 3; ++d) {
+            grid->lower[d] = g.lower[d];
+            grid->extent[d] = g.extent[d];
+        }
+    }
+    if (phys) {
+        phys->glat = g.glat;
+        phys->lambda_c = g.lambda_c;
+        phys->q0 = g.q0;
+        for (int d = 0; d < 3; ++d) phys->f_cor[d] = g.f_cor[d];
+    }
+    if (flags) {
+        flags->zeta_bndry_zero[0] = g.zeta_zero[0];
+        flags->zeta_bndry_zero[1] = g.zeta_zero[1];
+        flags->rk_order = m->rk_order;
+        flags->vmin = g.vmin;
+    }
+    return EPIC_OK;
+}
+
 int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)
 {
     if (!m) return EPIC_E_INVALID;
@@ -1759,3 +1787,6 @@ int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)
 }
 
 }  // extern "C"
+
+// failure message on a handle for the host-side file I/O (ncio.cpp)
+int epic_internal_fail(epic_handle m, int code, const std::string &what) { return fail(m, code, what); }

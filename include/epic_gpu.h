@@ -219,6 +219,45 @@ int epic_field_diagnostics(epic_handle h, double *stats);
 int epic_resort(epic_handle h);
 int epic_set_resort_interval(epic_handle h, int steps);
 
+/* The grid, physics and flags the handle was created with. */
+int epic_get_config(epic_handle h, epic_grid_t *grid, epic_phys_t *phys, epic_flags_t *flags);
+
+/* ---- NetCDF files (NetCDF classic format; parcel_netcdf.f90, field_netcdf.f90) ----
+ * The reference's dimensions, variables, units and long names in the classic
+ * binary format (CDF-2, or CDF-5 when a record exceeds 4 GiB), written and
+ * read without a netCDF library.  The reference's NetCDF-4 groups
+ * ('parameters', 'physical_quantities') become global attributes
+ * "<group>.<name>".  NetCDF-4/HDF5 files are not readable here. */
+/* write_netcdf_parcels(t) (parcel_netcdf.f90:201-309): one file, one record */
+int epic_write_parcels_nc(epic_handle h, const char *path, double t);
+/* the same from host arrays (Fortran layouts), without a handle */
+int epic_nc_write_parcels(const char *path, const epic_grid_t *grid, const epic_phys_t *phys,
+                          const epic_flags_t *flags, int64_t n, const double *position, const double *vorticity,
+                          const double *B, const double *volume, const double *buoyancy, const double *humidity,
+                          double t);
+/* read a parcel file into host arrays of `capacity` parcels; with position
+ * NULL only *n, *grid (may be NULL; nx = ny = nz = 0 when the file has no
+ * box attributes) and *t (may be NULL) are filled */
+int epic_nc_read_parcels(const char *path, int64_t capacity, int64_t *n, epic_grid_t *grid, double *position,
+                         double *vorticity, double *B, double *volume, double *buoyancy, double *humidity,
+                         double *t);
+/* field file from host arrays in the halo layout of epic_get_field
+ * (nparg and nsparg as doubles) */
+int epic_nc_write_fields(const char *path, const epic_grid_t *grid, const epic_phys_t *phys,
+                         const epic_flags_t *flags, double t, const double *velog, const double *vtend,
+                         const double *vortg, const double *tbuoyg, const double *dbuoyg, const double *humg,
+                         const double *volg, const double *nparg, const double *nsparg);
+/* message of the last failure of the epic_nc_* calls (this thread) */
+const char *epic_nc_last_error(void);
+/* read_netcdf_parcels (parcel_netcdf.f90:311-724) on one rank: the last
+ * record; B11..B23, positions and volume required, vorticity, buoyancy and
+ * humidity optional (0); a box in the file must match the grid; *t (may
+ * be NULL) receives the record's time */
+int epic_read_parcels_nc(epic_handle h, const char *path, double *t);
+/* write_netcdf_fields(t) (field_netcdf.f90:182-266): the node fields of
+ * planes 0..nz with the coordinate axes x, y, z and one time record */
+int epic_write_fields_nc(epic_handle h, const char *path, double t);
+
 /* Cell counting sort of the parcels' current positions with the reference's
  * key ijk = 1 + ix + hx iy + hx hy iz on the halo box
  * (parcel_nearest.f90:196-257, 411-424, 531-541): loca(n) 1-based cell of

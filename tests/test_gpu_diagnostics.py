@@ -24,6 +24,14 @@ def _close(got, ref, name):
         g = got[k]
         if k in EXACT:
             assert g == r, (name, k, g, r)
+        elif k in ("min_buoy", "max_buoy"):
+            # extremes of a field: within the field tolerance of its scale (max-rel)
+            scale = max(abs(ref["min_buoy"]), abs(ref["max_buoy"]))
+            assert abs(g - r) <= TOL * scale, (name, k, g, r)
+        elif k.startswith("std_"):
+            # sqrt(|sum x^2 / n - mean^2|) cancels: its resolution is sqrt(eps) * mean
+            mean = ref["avg_" + k[4:]]
+            assert abs(g - r) <= TOL * abs(r) + 1e-6 * abs(mean), (name, k, g, r)
         else:
             assert abs(g - r) <= TOL * max(abs(r), 1e-30) + 1e-12 * (abs(r) == 0), (name, k, g, r)
 
