@@ -134,3 +134,50 @@ def test_field_file_layout(tmp_path):
         lwc = f.variables["liquid_water_content"][0]
         assert np.allclose(lwc, (interior(fields["tbuoyg"]) - interior(fields["dbuoyg"])) / 4.0)
         assert f.variables["t"][:].tolist() == [0.75]
+
+
+def _cdf5_file(path, n, data, t):
+    """A CDF-5 (64-bit data) parcel file built byte by byte from the
+    published format description: 8-byte counts, sizes and offsets."""
+    import struct
+
+    def name(s):
+        b = s.encode()
+        return struct.pack(">Q", len(b)) + b + b"\0" * ((-len(b)) % 4)
+
+    names = list(data)
+    hdr = b"CDF\x05" + struct.pack(">Q", 1)
+    hdr += struct.pack(">IQ", 0x0A, 2) + name("t") + struct.pack(">Q", 0) + name("n_parcels") + struct.pack(">Q", n)
+    hdr += struct.pack(">IQ", 0, 0)   # no global attributes
+    nv = len(names) + 1
+    var_hdr_len = None
+    # two passes: the header length fixes the offsets
+    for _ in range(2):
+        body = struct.pack(">IQ", 0x0B, nv)
+        off = (len(hdr) + (var_hdr_len or 0))
+        body += name("t") + struct.pack(">Q", 1) + struct.pack(">Q", 0) + struct.pack(">IQ", 0, 0)
+        body += struct.pack(">I", 6) + struct.pack(">Q", 8) + struct.pack(">Q", off)
+        off += 8
+        for k in names:
+            body += name(k) + struct.pack(">Q", 2) + struct.pack(">QQ", 0, 1) + struct.pack(">IQ", 0, 0)
+            body += struct.pack(">I", 6) + struct.pack(">Q", 8 * n) + struct.pack(">Q", off)
+            off += 8 * n
+        var_hdr_len = len(body)
+    blob = hdr + body + struct.pack(">d", t)
+    for k in names:
+        blob += np.asarray(data[k], dtype=">f8").tobytes()
+    path.write_bytes(blob)
+
+
+def test_reads_cdf5(tmp_path):
+    n = 5
+    rng = np.random.default_rng(11)
+    data = {k: rng.normal(size=n) for k in ("B11", "B12", "B13", "B22", "B23", "x_position", "y_position",
+                                             "z_position", "volume", "buoyancy")}
+    path = tmp_path / "cdf5.nc"
+    _cdf5_file(path, n, data, 3.25)
+    back, grid, t = core.nc_read_parcels(path)
+    assert t == 3.25 and grid[:3] == (0, 0, 0)   # no box attributes
+    assert np.array_equal(back["position"][:, 2], data["z_position"])
+    assert np.array_equal(back["B"][:, 1], data["B12"])
+    assert np.array_equal(back["buoyancy"], data["buoyancy"])
