@@ -283,7 +283,7 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     const int lplane = P2G_BOX * lp;              // words per LDS node plane
     const int lfield = nbox * lplane;             // words per field
     extern __shared__ unsigned long long acc[];   // [field][plane][y][x]
-    __shared__ float wmax[1024 / 64][P2G_NF];
+    __shared__ float wmax[1024 / 64][P2G_NF + 1];
     __shared__ int s_err;
 
     for (int u = tid; u < P2G_NF * lfield; u += blockDim.x) acc[u] = 0ull;
@@ -313,32 +313,43 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
         v[6] = v0 * a.p.hum[pi];
     };
 
-    // ---- pass 1: the tile's largest contribution of every field ----
-    float mx[P2G_NF];
+    // ---- pass 1: the tile's largest contribution of every field, and
+    // whether the moist fields differ from the dry ones: with no
+    // condensation anywhere in the tile the total buoyancy equals the
+    // buoyancy (field 5 is a copy of field 4) and with no humidity field 6 is
+    // zero; their atomics are then skipped and the results are the same bits
+    float mx[P2G_NF + 1];   // [P2G_NF]: largest |tbuoy - buoy| contribution
 #pragma unroll
-    for (int f = 0; f < P2G_NF; ++f) mx[f] = 0.0f;
+    for (int f = 0; f <= P2G_NF; ++f) mx[f] = 0.0f;
     for (int s = s0 + sub; s < s1; s += P2G_TPC) {
         const int pi = a.perm[s];
         float v[P2G_NF];
         values(pi, a.p.z[pi], v);   // the same values pass 2 scales
 #pragma unroll
         for (int f = 0; f < P2G_NF; ++f) mx[f] = fmaxf(mx[f], fabsf(v[f]));
+        mx[P2G_NF] = fmaxf(mx[P2G_NF], fabsf(v[4] - v[5]));
     }
 #pragma unroll
-    for (int f = 0; f < P2G_NF; ++f)
+    for (int f = 0; f <= P2G_NF; ++f)
         for (int o = 32; o > 0; o >>= 1) mx[f] = fmaxf(mx[f], __shfl_xor(mx[f], o, 64));
     if ((tid & 63) == 0)
 #pragma unroll
-        for (int f = 0; f < P2G_NF; ++f) wmax[tid >> 6][f] = mx[f];
+        for (int f = 0; f <= P2G_NF; ++f) wmax[tid >> 6][f] = mx[f];
     __syncthreads();
     float sc[P2G_NF];
 #pragma unroll
-    for (int f = 0; f < P2G_NF; ++f) {
+    for (int f = 0; f <= P2G_NF; ++f) {
         float m = 0.0f;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, wmax[w][f]);
-        int e;
-        sc[f] = fixed_scale(m, e);
+        mx[f] = m;
     }
+#pragma unroll
+    for (int f = 0; f < P2G_NF; ++f) {
+        int e;
+        sc[f] = fixed_scale(mx[f], e);
+    }
+    const bool own5 = mx[P2G_NF] > 0.0f;   // tbuoy differs from buoy somewhere
+    const bool own6 = mx[6] > 0.0f;        // humidity somewhere
 
     // ---- pass 2: support points, trilinear weights, integer accumulation ----
     int err = 0;
@@ -404,6 +415,7 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
                 const int node = tgt[jp] + (c >> 2) * lplane + ((c >> 1) & 1) * lp + (c & 1);
 #pragma unroll
                 for (int f = 0; f < P2G_NF; ++f) {
+                    if ((f == 5 && !own5) || (f == 6 && !own6)) continue;   // uniform over the work-group
                     const long long q = (long long)(int)rintf(w[c] * cs[f]);
                     atomicAdd(&acc[f * lfield + node], (unsigned long long)q);
                 }
@@ -423,8 +435,9 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
         const int f = u % P2G_NF, nd = u / P2G_NF;
         const int pz = nd / (P2G_BOX * P2G_BOX), yx = nd - pz * (P2G_BOX * P2G_BOX);
         const int y = yx / P2G_BOX, x = yx - y * P2G_BOX;
-        const long long q = (long long)acc[f * lfield + pz * lplane + y * lp + x];
-        stage[u] = (float)((double)q * (double)inv[f]);
+        const int fs = (f == 5 && !own5) ? 4 : f;   // a skipped field 5 is field 4
+        const long long q = (f == 6 && !own6) ? 0ll : (long long)acc[fs * lfield + pz * lplane + y * lp + x];
+        stage[u] = (float)((double)q * (double)inv[fs]);
     }
     if (tid == 0 && s_err) atomicOr(a.err, s_err);
 }
