@@ -67,9 +67,18 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         const double q3 = 3.0 / j2;
         double smp = 0.5 * j3 * (q3 * sqrt(q3));   // (3/j2)^(3/2)
         smp = fmin(fmax(-1.0, smp), 1.0);
-        double alpha = (1.0 / 3.0) * acos(smp);
-        if (alpha > 3.14159265358979323846 / 6.0) alpha = alpha + (2.0 / 3.0) * 3.14159265358979323846;
-        const double eta1 = 2.0 * sqrt((1.0 / 3.0) * j2) * cos(alpha);
+        // cos(alpha), alpha = acos(smp)/3, shifted by 2 pi/3 when alpha >
+        // pi/6 (scherzinger.f90:175-181): by cos(3a) = 4 cos^3 a - 3 cos a it
+        // is the root of 4c^3 - 3c = smp of largest magnitude, with the sign
+        // of smp (+sqrt(3)/2 at 0).  Newton from 1 on 4c^3 - 3c = |smp|,
+        // convex on [1/2, 1] where that root lies, converges monotonically
+        // and quadratically: six steps reach it in FP64 for every |smp| <= 1.
+        const double sabs = fabs(smp);
+        double cr = 1.0;
+#pragma unroll
+        for (int it = 0; it < 6; ++it) cr = cr - ((4.0 * cr * cr - 3.0) * cr - sabs) / (12.0 * cr * cr - 3.0);
+        const double cos_alpha = (smp < 0.0) ? -cr : cr;
+        const double eta1 = 2.0 * sqrt((1.0 / 3.0) * j2) * cos_alpha;
 
         double r[3][3];   // r[c] = column c of A - eta1 I
 #pragma unroll
