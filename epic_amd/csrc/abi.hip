@@ -519,6 +519,8 @@ void build_vtend(Plan &P, const Members &ms)
 // (never the one the stage read, so par2grid can clear it in parallel).
 // Half axes: stages 1..ns-1 use ax_cur, the last stage ax_stale, which is
 // then the cache the next step's first par2grid takes when *reuse is set.
+// Stage 1 computes ax_cur from B; the push of stage s < ns stores the axes of
+// the B it writes, which are stage s+1's (ax_cur, or ax_stale for the last).
 struct StepItem {
     epic_member *m;
     StepTime *tm;        // device: dt (written by the host, or by the adaptive time step)
@@ -549,7 +551,7 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
             const int rd = (s - 1) % 3, wr = last ? 0 : (s % 3);
             sel.push_back(P2GSel{a, m->ax_stale, s == 1 ? it->reuse : nullptr, rd, wr});
         }
-        build_axes(P, ms, ax);
+        if (s == 1) build_axes(P, ms, ax);   // later stages: stored by the previous push
         build_p2g(P, ms, sel);
         build_vor2vel(P, ms);
         build_vtend(P, ms);
@@ -572,6 +574,8 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
             a.g = m->g;
             a.p = m->pd;
             for (int c = 0; c < 6; ++c) a.axes[c] = ax[i][c];
+            if (!last)
+                for (int c = 0; c < 6; ++c) a.axes_next[c] = (s + 1 == ns) ? m->ax_stale[c] : m->ax_cur[c];
             a.perm = m->perm;
             a.cell_start = m->cell_start;
             a.fld = m->fld;
@@ -1311,7 +1315,7 @@ int epic_autotune(epic_handle m, int *tz_p2g, int *tz_g2p)
         G2PTileArgs a{};
         a.g = m->g;
         a.p = m->pd;
-        for (int c = 0; c < 6; ++c) a.axes[c] = m->ax_cur[c];
+        for (int c = 0; c < 6; ++c) a.axes[c] = a.axes_next[c] = m->ax_cur[c];   // rewritten unchanged, as a step's push does
         a.perm = m->perm;
         a.cell_start = m->cell_start;
         a.fld = m->fld;

@@ -88,6 +88,21 @@ struct FftPlan {
     int radix[16];
 };
 
+// Global-address-space views of pointers kept in argument tables.  Loads
+// through a generic pointer are flat loads, which count in the LDS/scalar
+// counter too: every wait for a scalar load of a further table entry (the
+// next attribute's pointer) would also drain the loads in flight.  Global
+// loads count only in the vector-memory counter, so one parcel's loads all
+// stay in flight together.  The host pass only parses device code; there the
+// qualifier is dropped so that the runtime's generic-pointer overloads apply.
+#ifdef __HIP_DEVICE_COMPILE__
+#define EPIC_GLOBAL __attribute__((address_space(1)))
+#else
+#define EPIC_GLOBAL
+#endif
+template <class T> __device__ inline EPIC_GLOBAL T *gl(T *p) { return (EPIC_GLOBAL T *)p; }
+template <class T> __device__ inline const EPIC_GLOBAL T *gl(const T *p) { return (const EPIC_GLOBAL T *)p; }
+
 __host__ __device__ inline int64_t field_stride(const Grid &g) { return (int64_t)(g.nz + 3) * g.ny * g.nx; }
 __host__ __device__ inline int64_t plane_stride(const Grid &g) { return (int64_t)g.ny * g.nx; }
 __host__ __device__ inline int64_t spec_size(const Grid &g) { return (int64_t)g.ny * g.nxh * g.nzp; }
@@ -108,7 +123,7 @@ __device__ inline double ftrunc(double v) { return trunc(v); }
 // ticket (old counter value + its rank within the run).  key < 0: no ticket.
 // Must be called by all lanes of the wavefront.
 // ---------------------------------------------------------------------------
-__device__ inline int wave_run_ticket(int key, int *counter)
+template <class P> __device__ inline int wave_run_ticket(int key, P counter)
 {
     const int lane = threadIdx.x & 63;
     const int prev = __shfl_up(key, 1, 64);
@@ -128,7 +143,7 @@ __device__ inline int wave_run_ticket(int key, int *counter)
 // wave-aggregated integer increments: counter[key] += 1 for every lane, one
 // atomic per run of equal keys in consecutive lanes, no return value waited
 // for.  key < 0: nothing.  Must be called by all lanes of the wavefront.
-__device__ inline void wave_run_add(int key, int *counter)
+template <class P> __device__ inline void wave_run_add(int key, P counter)
 {
     const int lane = threadIdx.x & 63;
     const int prev = __shfl_up(key, 1, 64);

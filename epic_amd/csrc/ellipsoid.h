@@ -6,6 +6,9 @@
 //    algorithm with the reference's branch structure (scherzinger.f90:31-243):
 //    diagonal shortcut, maxloc tie-breaking (first maximum) and the
 //    three-swap descending sort, so near-degenerate shapes pick the same basis
+//    (column lengths are compared squared: two whose lengths round to one
+//    FP64 value but whose squares differ take the longer, an equally valid
+//    pivot, where the first would be taken)
 //  * half axes Veta = rho sqrt(|a2 - c2|) v_a, Vtau = rho sqrt(|b2 - c2|) v_b,
 //    rho = sqrt(2/5), and the four support points x + Veta cos(th_j) +
 //    Vtau sin(th_j), th_j = (2j+1) pi/4 (parcel_ellipsoid.f90:29-34, 216-260)
@@ -29,7 +32,8 @@ __device__ inline bool b33_of(const double B[5], double vol, double &b33)
     return true;
 }
 
-__device__ inline double dnorm3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
+__device__ inline double dsq3(double a, double b, double c) { return a * a + b * b + c * c; }
+__device__ inline double dnorm3(double a, double b, double c) { return sqrt(dsq3(a, b, c)); }
 
 // single-precision literal 1.0e-12 widened, as declared by the reference
 #define EPIC_SCH_ATOL ((double)1.0e-12f)
@@ -70,13 +74,14 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         // cos(alpha), alpha = acos(smp)/3, shifted by 2 pi/3 when alpha >
         // pi/6 (scherzinger.f90:175-181): by cos(3a) = 4 cos^3 a - 3 cos a it
         // is the root of 4c^3 - 3c = smp of largest magnitude, with the sign
-        // of smp (+sqrt(3)/2 at 0).  Newton from 1 on 4c^3 - 3c = |smp|,
-        // convex on [1/2, 1] where that root lies, converges monotonically
-        // and quadratically: six steps reach it in FP64 for every |smp| <= 1.
+        // of smp (+sqrt(3)/2 at 0).  That root of 4c^3 - 3c = |smp| lies in
+        // [sqrt(3)/2, 1]; a degree-4 least-squares fit of cos(acos(s)/3) on
+        // [0, 1] starts within 1e-5 of it, and two Newton steps (error
+        // ratio below 1.7 e^2 there) reach it to FP64 rounding.
         const double sabs = fabs(smp);
-        double cr = 1.0;
+        double cr = 0.8660344784 + sabs * (0.1663758323 + sabs * (-0.04585590254 + sabs * (0.01750308532 + sabs * -0.004064325447)));
 #pragma unroll
-        for (int it = 0; it < 6; ++it) cr = cr - ((4.0 * cr * cr - 3.0) * cr - sabs) / (12.0 * cr * cr - 3.0);
+        for (int it = 0; it < 2; ++it) cr = cr - ((4.0 * cr * cr - 3.0) * cr - sabs) / (12.0 * cr * cr - 3.0);
         const double cos_alpha = (smp < 0.0) ? -cr : cr;
         const double eta1 = 2.0 * sqrt((1.0 / 3.0) * j2) * cos_alpha;
 
@@ -85,15 +90,16 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int i = 0; i < 3; ++i) r[c][i] = A[3 * i + c] - (i == c ? eta1 : 0.0);
-        double l0 = dnorm3(r[0][0], r[0][1], r[0][2]);
-        double l1 = dnorm3(r[1][0], r[1][1], r[1][2]);
-        double l2 = dnorm3(r[2][0], r[2][1], r[2][2]);
+        // the longest column (first of equals), compared by squared length
+        double l0 = dsq3(r[0][0], r[0][1], r[0][2]);
+        double l1 = dsq3(r[1][0], r[1][1], r[1][2]);
+        double l2 = dsq3(r[2][0], r[2][1], r[2][2]);
         int im = 0;
         double lm = l0;
         if (l1 > lm) { im = 1; lm = l1; }
         if (l2 > lm) { im = 2; lm = l2; }
         double s1[3];
-        const double ilm = 1.0 / lm;
+        const double ilm = 1.0 / sqrt(lm);
 #pragma unroll
         for (int i = 0; i < 3; ++i) s1[i] = r[im][i] * ilm;
         if (im == 1) {
@@ -112,11 +118,11 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
             r[1][i] = r[1][i] - t * r[0][i];
             r[2][i] = r[2][i] - s * r[0][i];
         }
-        l1 = dnorm3(r[1][0], r[1][1], r[1][2]);
-        l2 = dnorm3(r[2][0], r[2][1], r[2][2]);
+        l1 = dsq3(r[1][0], r[1][1], r[1][2]);
+        l2 = dsq3(r[2][0], r[2][1], r[2][2]);
         {
             const bool use2 = l2 > l1;
-            const double il = 1.0 / (use2 ? l2 : l1);
+            const double il = 1.0 / sqrt(use2 ? l2 : l1);
 #pragma unroll
             for (int i = 0; i < 3; ++i) r[1][i] = (use2 ? r[2][i] : r[1][i]) * il;
         }
@@ -151,11 +157,11 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
             u0[i] = A[3 * i + 0] * r[0][0] + A[3 * i + 1] * r[0][1] + A[3 * i + 2] * r[0][2];
             u1[i] = A[3 * i + 0] * r[1][0] + A[3 * i + 1] * r[1][1] + A[3 * i + 2] * r[1][2];
         }
-        const double m0 = dnorm3(u0[0], u0[1], u0[2]);
-        const double m1 = dnorm3(u1[0], u1[1], u1[2]);
+        const double m0 = dsq3(u0[0], u0[1], u0[2]);
+        const double m1 = dsq3(u1[0], u1[1], u1[2]);
         {
             const bool use1 = m1 > m0;
-            const double im_ = 1.0 / (use1 ? m1 : m0);
+            const double im_ = 1.0 / sqrt(use1 ? m1 : m0);
 #pragma unroll
             for (int i = 0; i < 3; ++i) V[2][i] = (use1 ? u1[i] : u0[i]) * im_;
         }

@@ -26,12 +26,12 @@ __device__ inline float load_phys(const FftXArgs &a, int k, int j, int i)
     const Grid &g = a.g;
     const int64_t FS = field_stride(g), PS = plane_stride(g);
     const int64_t o = (int64_t)(k + 1) * PS + (int64_t)j * g.nx + i;
-    if (a.mode == 0) return a.fld[a.slot * FS + o];
+    if (a.mode == 0) return gl(a.fld)[a.slot * FS + o];
     // parcel-correction source: volg / vcell - 1
-    if (a.mode == 3) return a.fld[F_VOLG * FS + o] * (float)(1.0 / g.vcell) - 1.0f;
+    if (a.mode == 3) return gl(a.fld)[F_VOLG * FS + o] * (float)(1.0 / g.vcell) - 1.0f;
     // vorticity-tendency flux: (omega_a + f_a) * vel_b + sgn * tbuoy
-    float v = (a.fld[(F_VORTG + a.comp_a) * FS + o] + (float)g.f_cor[a.comp_a]) * a.fld[(F_VELOG + a.comp_b) * FS + o];
-    if (a.sgn != 0) v += (float)a.sgn * a.fld[F_TBUOYG * FS + o];
+    float v = (gl(a.fld)[(F_VORTG + a.comp_a) * FS + o] + (float)g.f_cor[a.comp_a]) * gl(a.fld)[(F_VELOG + a.comp_b) * FS + o];
+    if (a.sgn != 0) v += (float)a.sgn * gl(a.fld)[F_TBUOYG * FS + o];
     return v;
 }
 
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_fft_xf(const FftXArgs *__restrict__ tab
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
     float2 *tw = bufb + ZB * (m + 1);   // twiddles staged in LDS
-    for (int t = threadIdx.x; t < g.nx; t += blockDim.x) tw[t] = a.tw[t];
+    for (int t = threadIdx.x; t < g.nx; t += blockDim.x) tw[t] = gl(a.tw)[t];
     for (int t = threadIdx.x; t < ZB * m; t += blockDim.x) {
         const int r = t / m, q = t - r * m;
         const int k = zp0 + r;
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_fft_xf(const FftXArgs *__restrict__ tab
     const int nxh = m + 1;
     for (int t = threadIdx.x; t < ZB * nxh; t += blockDim.x) {
         const int kx = t / ZB, r = t - kx * ZB;
-        a.out[((int64_t)j * nxh + kx) * g.nzp + zp0 + r] = other[r * (m + 1) + kx];
+        gl(a.out)[((int64_t)j * nxh + kx) * g.nzp + zp0 + r] = other[r * (m + 1) + kx];
     }
 }
 
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_fft_y(const FftYArgs *__restrict__ tab,
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * ny;
     float2 *tw = bufb + ZB * ny;   // twiddles staged in LDS
-    for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = a.tw[t];
+    for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = gl(a.tw)[t];
     // inverse combine factors: i kx (0 at the x Nyquist) and i ky (0 at the y Nyquist)
     const float kxw = (2 * kx == g.nx) ? 0.0f : (float)(2.0 * 3.14159265358979323846 * kx / g.extent[0]);
     for (int t = threadIdx.x; t < ZB * ny; t += blockDim.x) {
@@ -102,11 +102,11 @@ __global__ __launch_bounds__(256) void k_fft_y(const FftYArgs *__restrict__ tab,
         if (a.mode == 2) {
             const int kys = (jy <= ny / 2) ? jy : jy - ny;
             const float kyw = (2 * jy == ny) ? 0.0f : (float)(2.0 * 3.14159265358979323846 * kys / g.extent[1]);
-            const float2 f1 = a.in[idx], f2 = a.in2[idx];
+            const float2 f1 = gl(a.in)[idx], f2 = gl(a.in2)[idx];
             // i kx f1 + i ky f2
             v = float2{-(kxw * f1.y + kyw * f2.y), kxw * f1.x + kyw * f2.x};
         } else {
-            v = a.in[idx];
+            v = gl(a.in)[idx];
         }
         bufa[r * ny + jy] = v;
     }
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_fft_y(const FftYArgs *__restrict__ tab,
     float2 *R = lds_fft(bufa, bufb, ZB, a.plan, tw, 1, a.mode != 0);
     for (int t = threadIdx.x; t < ZB * ny; t += blockDim.x) {
         const int jy = t / ZB, r = t - jy * ZB;
-        a.out[((int64_t)jy * nxh + kx) * g.nzp + zp0 + r] = R[r * ny + jy];
+        gl(a.out)[((int64_t)jy * nxh + kx) * g.nzp + zp0 + r] = R[r * ny + jy];
     }
 }
 
@@ -132,10 +132,10 @@ __global__ __launch_bounds__(256) void k_fft_xi(const FftXArgs *__restrict__ tab
     extern __shared__ float2 lds[];
     float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
     float2 *tw = bufb + ZB * (m + 1);   // twiddles staged in LDS
-    for (int t = threadIdx.x; t < g.nx; t += blockDim.x) tw[t] = a.tw[t];
+    for (int t = threadIdx.x; t < g.nx; t += blockDim.x) tw[t] = gl(a.tw)[t];
     for (int t = threadIdx.x; t < ZB * nxh; t += blockDim.x) {
         const int kx = t / ZB, r = t - kx * ZB;
-        bufb[r * nxh + kx] = a.in[((int64_t)j * nxh + kx) * g.nzp + zp0 + r];
+        bufb[r * nxh + kx] = gl(a.in)[((int64_t)j * nxh + kx) * g.nzp + zp0 + r];
     }
     __syncthreads();
     for (int t = threadIdx.x; t < ZB * m; t += blockDim.x) {
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_fft_xi(const FftXArgs *__restrict__ tab
             const float fz = (float)g.f_cor[2];
             auto f3 = [&](int kk) {
                 const int64_t oo = (int64_t)(kk + 1) * PS + (int64_t)j * g.nx + i;
-                return (a.fld[(F_VORTG + 2) * FS + oo] + fz) * a.fld[(F_VELOG + a.comp_b) * FS + oo];
+                return (gl(a.fld)[(F_VORTG + 2) * FS + oo] + fz) * gl(a.fld)[(F_VELOG + a.comp_b) * FS + oo];
             };
             const float dzi = (float)g.dxi[2];
             float dd;
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_fft_xi(const FftXArgs *__restrict__ tab
             else dd = (f3(k + 1) - f3(k - 1)) * (0.5f * dzi);
             v = v + dd;
         }
-        a.fld_out[a.slot * FS + o] = v;
+        gl(a.fld_out)[a.slot * FS + o] = v;
     }
 }
 

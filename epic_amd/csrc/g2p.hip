@@ -32,11 +32,11 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
     int e = 0;
     const int64_t q = ib.blk * (int64_t)blockDim.x + threadIdx.x;
     if (q < a.p.n) {
-        const double pos[3] = {a.p.x[q], a.p.y[q], a.p.z[q]};
+        const double pos[3] = {gl(a.p.x)[q], gl(a.p.y)[q], gl(a.p.z)[q]};
         double B[5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) B[c] = (double)a.p.B[c][q];
-        const double vol = (double)a.p.vol[q];
+        for (int c = 0; c < 5; ++c) B[c] = (double)gl(a.p.B[c])[q];
+        const double vol = (double)gl(a.p.vol)[q];
         double ve[3], vt[3];
         if (!support_axes(B, vol, ve, vt)) {
             e |= ERR_B33;
@@ -44,14 +44,14 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            a.p.veta[d][q] = (float)ve[d];
-            a.p.vtau[d][q] = (float)vt[d];
+            gl(a.p.veta[d])[q] = (float)ve[d];
+            gl(a.p.vtau[d])[q] = (float)vt[d];
         }
         float dp[3], dv[3], st[8];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            dp[c] = a.add ? a.p.dpos[c][q] : 0.0f;
-            dv[c] = a.add ? a.p.dvor[c][q] : 0.0f;
+            dp[c] = a.add ? gl(a.p.dpos[c])[q] : 0.0f;
+            dv[c] = a.add ? gl(a.p.dvor[c])[q] : 0.0f;
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c) st[c] = 0.0f;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
                                  pzc * w10, w[2] * w10, pzc * w11, w[2] * w11};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float *f = a.fld + (F_VELOG + c) * FS;
+                const EPIC_GLOBAL float *f = gl(a.fld) + (F_VELOG + c) * FS;
                 float s = 0.0f;
 #pragma unroll
                 for (int m = 0; m < 8; ++m) s += wt[m] * f[off[m]];
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
             }
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const float *f = a.fld + (F_VELGRADG + c) * FS;
+                const EPIC_GLOBAL float *f = gl(a.fld) + (F_VELGRADG + c) * FS;
                 float s = 0.0f;
 #pragma unroll
                 for (int m = 0; m < 8; ++m) s += wt[m] * f[off[m]];
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float *f = a.fld + (F_VTEND + c) * FS;
+                const EPIC_GLOBAL float *f = gl(a.fld) + (F_VTEND + c) * FS;
                 float s = 0.0f;
 #pragma unroll
                 for (int m = 0; m < 8; ++m) s += wt[m] * f[off[m]];
@@ -111,11 +111,11 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
 
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            a.p.dpos[c][q] = dp[c];
-            a.p.dvor[c][q] = dv[c];
+            gl(a.p.dpos[c])[q] = dp[c];
+            gl(a.p.dvor[c])[q] = dv[c];
         }
 #pragma unroll
-        for (int c = 0; c < 8; ++c) a.p.strain[c][q] = st[c];
+        for (int c = 0; c < 8; ++c) gl(a.p.strain[c])[q] = st[c];
     }
     if (e) atomicOr(a.err, e);
 }
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             float v = 0.0f;
             if (P >= -1 && P <= g.nz + 1) {
                 const int gi = wrap(i0 - 1 + lx, g.nx), gj = wrap(j0 - 1 + ly, g.ny);
-                v = a.fld[(F_VELOG + f) * FS + (int64_t)(P + 1) * PS + (int64_t)gj * g.nx + gi];
+                v = gl(a.fld)[(F_VELOG + f) * FS + (int64_t)(P + 1) * PS + (int64_t)gj * g.nx + gi];
             }
             const int nd = lp * NN + node;
             if (f < 12) ((float *)box)[((f >> 2) * NB + nd) * 4 + (f & 3)] = v;
@@ -178,8 +178,8 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
     if (tid < nrow) {
         const int k = k0 + tid / P2G_TY, r = tid % P2G_TY;
         const int c0 = ((k + 1) * g.ny + (j0 + r)) * g.nx + i0;
-        row_slot[tid] = a.cell_start[c0];
-        row_off[tid + 1] = a.cell_start[c0 + P2G_TX] - a.cell_start[c0];
+        row_slot[tid] = gl(a.cell_start)[c0];
+        row_off[tid + 1] = gl(a.cell_start)[c0 + P2G_TX] - gl(a.cell_start)[c0];
     }
     __syncthreads();
     if (tid < 64) {   // inclusive scan of the row sizes in one wave
@@ -206,14 +206,14 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             for (int step = 32; step > 0; step >>= 1)
                 if (r + step < nrow && row_off[r + step] <= q) r += step;
             const int k = k0 + r / P2G_TY;
-            const int pi = a.perm[row_slot[r] + (q - row_off[r])];
-            double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
+            const int pi = gl(a.perm)[row_slot[r] + (q - row_off[r])];
+            double pos[3] = {gl(a.p.x)[pi], gl(a.p.y)[pi], gl(a.p.z)[pi]};
             double B[5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) B[c] = (double)a.p.B[c][pi];
-            const double vol = (double)a.p.vol[pi];
-            const double ve[3] = {a.axes[0][pi], a.axes[1][pi], a.axes[2][pi]};
-            const double vt[3] = {a.axes[3][pi], a.axes[4][pi], a.axes[5][pi]};
+            for (int c = 0; c < 5; ++c) B[c] = (double)gl(a.p.B[c])[pi];
+            const double vol = (double)gl(a.p.vol)[pi];
+            const double ve[3] = {gl(a.axes[0])[pi], gl(a.axes[1])[pi], gl(a.axes[2])[pi]};
+            const double vt[3] = {gl(a.axes[3])[pi], gl(a.axes[4])[pi], gl(a.axes[5])[pi]};
             int cc[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
@@ -226,8 +226,8 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             float dp[3], dv[3], st[8];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                dp[c] = acc_prev ? a.p.dpos[c][pi] : 0.0f;
-                dv[c] = acc_prev ? a.p.dvor[c][pi] : 0.0f;
+                dp[c] = acc_prev ? gl(a.p.dpos[c])[pi] : 0.0f;
+                dv[c] = acc_prev ? gl(a.p.dvor[c])[pi] : 0.0f;
             }
 #pragma unroll
             for (int c = 0; c < 8; ++c) st[c] = 0.0f;
@@ -287,13 +287,13 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             }
             float db[5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) db[c] = acc_prev ? a.p.dB[c][pi] + (float)dB[c] : (float)dB[c];
+            for (int c = 0; c < 5; ++c) db[c] = acc_prev ? gl(a.p.dB[c])[pi] + (float)dB[c] : (float)dB[c];
             // push (ls_rk.f90:183-191), periodic wrap (parcel_bc.f90:17-21)
             const double cbdt = a.cb * (*a.dt);
 #pragma unroll
             for (int c = 0; c < 3; ++c) pos[c] = pos[c] + cbdt * (double)dp[c];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) a.p.vor[c][pi] = (float)((double)a.p.vor[c][pi] + cbdt * (double)dv[c]);
+            for (int c = 0; c < 3; ++c) gl(a.p.vor[c])[pi] = (float)((double)gl(a.p.vor[c])[pi] + cbdt * (double)dv[c]);
 #pragma unroll
             for (int c = 0; c < 5; ++c) B[c] = B[c] + cbdt * (double)db[c];
             pos[0] = pos[0] - g.extent[0] * trunc((pos[0] - g.center[0]) * g.hli[0]);
@@ -309,20 +309,20 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
                     B[4] = -B[4];
                 }
             }
-            a.p.x[pi] = pos[0];
-            a.p.y[pi] = pos[1];
-            a.p.z[pi] = pos[2];
+            gl(a.p.x)[pi] = pos[0];
+            gl(a.p.y)[pi] = pos[1];
+            gl(a.p.z)[pi] = pos[2];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) a.p.B[c][pi] = (float)B[c];
+            for (int c = 0; c < 5; ++c) gl(a.p.B[c])[pi] = (float)B[c];
             if (a.stage < a.nstages) {
                 const float ca = (float)a.ca;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    a.p.dpos[c][pi] = ca * dp[c];
-                    a.p.dvor[c][pi] = ca * dv[c];
+                    gl(a.p.dpos[c])[pi] = ca * dp[c];
+                    gl(a.p.dvor[c])[pi] = ca * dv[c];
                 }
 #pragma unroll
-                for (int c = 0; c < 5; ++c) a.p.dB[c][pi] = ca * db[c];
+                for (int c = 0; c < 5; ++c) gl(a.p.dB[c])[pi] = ca * db[c];
             }
             // cell of the new position for the next stage's sort
             const int ni = (int)floor((pos[0] - g.lower[0]) * g.dxi[0]);
@@ -330,13 +330,27 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             const int nk = (int)floor((pos[2] - g.lower[2]) * g.dxi[2]);
             if (nk < -1 || nk > g.nz) e |= ERR_PARCEL_COUNT;
             else key = ((nk + 1) * g.ny + wrap(nj, g.ny)) * g.nx + wrap(ni, g.nx);
-            a.keys[pi] = key;
+            gl(a.keys)[pi] = key;
             small = (vol <= g.vmin);
+            if (a.axes_next[0]) {   // the next stage's support axes, from B as stored
+                double Bs[5], ve2[3], vt2[3];
+#pragma unroll
+                for (int c = 0; c < 5; ++c) Bs[c] = (double)(float)B[c];
+                if (!support_axes(Bs, vol, ve2, vt2)) {
+                    e |= ERR_B33;
+                    ve2[0] = ve2[1] = ve2[2] = vt2[0] = vt2[1] = vt2[2] = 0.0;
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    gl(a.axes_next[d])[pi] = (float)ve2[d];
+                    gl(a.axes_next[3 + d])[pi] = (float)vt2[d];
+                }
+            }
         };
         process();
         // next stage's cell histogram, one atomic per run of equal keys
-        wave_run_add(key, a.count);
-        wave_run_add(small ? key : -1, a.scount);
+        wave_run_add(key, gl(a.count));
+        wave_run_add(small ? key : -1, gl(a.scount));
     }
     if (e) atomicOr(a.err, e);
 }

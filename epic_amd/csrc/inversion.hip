@@ -131,7 +131,7 @@ __global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restric
     // ---- load the three vorticity spectra and the phi tables ----
     const int64_t base = ((int64_t)ky * g.nxh + kx) * g.nzp;
     for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        const float2 x = a.in[0][base + z], y = a.in[1][base + z], w = a.in[2][base + z];
+        const float2 x = gl(a.in[0])[base + z], y = gl(a.in[1])[base + z], w = gl(a.in[2])[base + z];
         SX[z] = double2{x.x, x.y};
         SY[z] = double2{y.x, y.y};
         C[z] = double2{w.x, w.y};
@@ -196,9 +196,9 @@ __global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restric
     combine(cc, DS, phim, phip);
     combine(cc, AS, phim, phip);
     for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        a.out[0][base + z] = float2{(float)DS[z].x, (float)DS[z].y};
-        a.out[1][base + z] = float2{(float)AS[z].x, (float)AS[z].y};
-        a.out[2][base + z] = float2{(float)C[z].x, (float)C[z].y};
+        gl(a.out[0])[base + z] = float2{(float)DS[z].x, (float)DS[z].y};
+        gl(a.out[1])[base + z] = float2{(float)AS[z].x, (float)AS[z].y};
+        gl(a.out[2])[base + z] = float2{(float)C[z].x, (float)C[z].y};
     }
     __syncthreads();
 
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restric
             v = double2{k2l2i * (ey.x - cx.x), k2l2i * (ey.y - cx.y)};
         }
         const double2 w = DS[z];
-        auto st = [&](int o, double2 val) { a.out[o][base + z] = float2{(float)val.x, (float)val.y}; };
+        auto st = [&](int o, double2 val) { gl(a.out[o])[base + z] = float2{(float)val.x, (float)val.y}; };
         st(3, u);
         st(4, v);
         st(5, w);
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void k_vor2vel_fixup(const FixArgs *__restrict
     const int k = (int)(tq / ncol) - 1;
     const int64_t col = tq - (int64_t)(k + 1) * ncol;
     const int64_t FS = field_stride(g), PS = plane_stride(g);
-    auto F = [&](int slot, int kk) -> float & { return fld[slot * FS + (int64_t)(kk + 1) * PS + col]; };
+    auto F = [&](int slot, int kk) -> EPIC_GLOBAL float & { return gl(fld)[slot * FS + (int64_t)(kk + 1) * PS + col]; };
     float dwdx, dudy, dwdy;
     if (k == -1 || k == nz + 1) {
         const int kb = (k < 0) ? 0 : nz, kn = (k < 0) ? 1 : nz - 1;
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void k_vtend_fixup(const FixArgs *__restrict__
     if (col >= (int64_t)g.nx * g.ny) return;
     const int64_t FS = field_stride(g), PS = plane_stride(g);
     const int nz = g.nz;
-    auto F = [&](int slot, int k) -> float & { return fld[slot * FS + (int64_t)(k + 1) * PS + col]; };
+    auto F = [&](int slot, int k) -> EPIC_GLOBAL float & { return gl(fld)[slot * FS + (int64_t)(k + 1) * PS + col]; };
     if (g.zeta_zero[0]) F(F_VTEND + 2, 0) = 0.0f;
     if (g.zeta_zero[1]) F(F_VTEND + 2, nz) = 0.0f;
     for (int c = 0; c < 3; ++c) {

@@ -87,6 +87,7 @@ struct G2PTileArgs {
     Grid g;
     ParcelsDev p;
     const float *axes[6];
+    float *axes_next[6];         // null, or where the half axes of the pushed B go (the next stage's)
     const int *perm;
     const int *cell_start;
     const float *fld;

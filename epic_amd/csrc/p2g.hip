@@ -55,11 +55,11 @@ __global__ __launch_bounds__(256) void k_cell_keys(const KeysArgs *__restrict__ 
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
         if (q >= a.p.n) break;
-        const int key = cell_key(g, a.p.x[q], a.p.y[q], a.p.z[q], e);
-        a.keys[q] = key;
+        const int key = cell_key(g, gl(a.p.x)[q], gl(a.p.y)[q], gl(a.p.z)[q], e);
+        gl(a.keys)[q] = key;
         if (key >= 0) {
-            atomicAdd(&a.count[key], 1);
-            if (a.p.vol[q] <= g.vmin) atomicAdd(&a.scount[key], 1);
+            atomicAdd(&gl(a.count)[key], 1);
+            if (gl(a.p.vol)[q] <= g.vmin) atomicAdd(&gl(a.scount)[key], 1);
         }
     }
     if (e) atomicOr(a.err, e);
@@ -98,10 +98,10 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const ScanArgs *__restri
     int s = 0;
 #pragma unroll
     for (int u = 0; u < SCAN_PER; ++u)
-        if (base + u < a.n) s += a.in[base + u];
+        if (base + u < a.n) s += gl(a.in)[base + u];
     int total;
     block_excl_scan(s, lds, total);
-    if (threadIdx.x == 0) a.bsum[ib.blk] = total;
+    if (threadIdx.x == 0) gl(a.bsum)[ib.blk] = total;
 }
 
 // one block per item: exclusive scan of the item's block sums in place
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_blocks(const ScanArgs *__restri
         int v[SCAN_PER], s = 0;
 #pragma unroll
         for (int u = 0; u < SCAN_PER; ++u) {
-            v[u] = (i0 + u < nb) ? a.bsum[i0 + u] : 0;
+            v[u] = (i0 + u < nb) ? gl(a.bsum)[i0 + u] : 0;
             s += v[u];
         }
         int total;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_blocks(const ScanArgs *__restri
         int run = carry + ex;
 #pragma unroll
         for (int u = 0; u < SCAN_PER; ++u) {
-            if (i0 + u < nb) a.bsum[i0 + u] = run;
+            if (i0 + u < nb) gl(a.bsum)[i0 + u] = run;
             run += v[u];
         }
         carry += total;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_blocks(const ScanArgs *__restri
     }
 }
 
-// out[c] = exclusive prefix, out[n] = total; clears a.clear[c] (the
+// out[c] = exclusive prefix, out[n] = total; clears gl(a.clear)[c] (the
 // placement cursors) on the way
 __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const ScanArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
@@ -145,21 +145,21 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const ScanArgs *__restric
     int v[SCAN_PER], s = 0;
 #pragma unroll
     for (int u = 0; u < SCAN_PER; ++u) {
-        v[u] = (base + u < n) ? a.in[base + u] : 0;
+        v[u] = (base + u < n) ? gl(a.in)[base + u] : 0;
         s += v[u];
     }
     int total;
-    int run = block_excl_scan(s, lds, total) + a.bsum[ib.blk];
+    int run = block_excl_scan(s, lds, total) + gl(a.bsum)[ib.blk];
 #pragma unroll
     for (int u = 0; u < SCAN_PER; ++u) {
         if (base + u < n) {
-            a.out[base + u] = run;
-            if (a.clear) a.clear[base + u] = 0;
+            gl(a.out)[base + u] = run;
+            if (a.clear) gl(a.clear)[base + u] = 0;
         }
         run += v[u];
     }
     // the thread owning element n-1 also writes the grand total out[n]
-    if (base <= n - 1 && n - 1 < base + SCAN_PER) a.out[n] = run;
+    if (base <= n - 1 && n - 1 < base + SCAN_PER) gl(a.out)[n] = run;
 }
 
 void add_exclusive_scan(Plan &P, int phase, const std::vector<ScanArgs> &items)
@@ -188,9 +188,9 @@ __global__ __launch_bounds__(256) void k_place(const PlaceArgs *__restrict__ tab
 #pragma unroll
     for (int r = 0; r < PER_BLOCK / 256; ++r) {   // every lane of a wave runs the same rounds
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
-        const int key = (q < a.n) ? a.keys[q] : -1;
-        const int base = wave_run_ticket(key, a.cursor);
-        if (key >= 0) a.perm[a.cell_start[key] + base] = (int)q;
+        const int key = (q < a.n) ? gl(a.keys)[q] : -1;
+        const int base = wave_run_ticket(key, gl(a.cursor));
+        if (key >= 0) gl(a.perm)[gl(a.cell_start)[key] + base] = (int)q;
     }
 }
 
@@ -296,21 +296,24 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     int s0 = 0, s1 = 0;
     if (k <= g.nz) {
         const int c = ((k + 1) * g.ny + j0 + cy) * g.nx + i0 + cx;
-        s0 = a.cell_start[c];
-        s1 = a.cell_start[c + 1];
+        s0 = gl(a.cell_start)[c];
+        s1 = gl(a.cell_start)[c + 1];
     }
     // the gridded quantities of a parcel: vol/4 and vol/4 times vorticity,
     // total buoyancy (with condensation), buoyancy and humidity
+    // (a parcel without humidity has no condensate: the saturation humidity
+    // is positive, so its exp is skipped)
     auto values = [&](int pi, double z, float v[P2G_NF]) {
-        const double qc = fmax(0.0, (double)a.p.hum[pi] - g.q0 * exp(g.lambda_c * (g.lower[2] - z)));
-        const float v0 = (float)(0.25 * (double)a.p.vol[pi]);
+        const float h = gl(a.p.hum)[pi];
+        const double qc = (h > 0.0f) ? fmax(0.0, (double)h - g.q0 * exp(g.lambda_c * (g.lower[2] - z))) : 0.0;
+        const float v0 = (float)(0.25 * (double)gl(a.p.vol)[pi]);
         v[0] = v0;
-        v[1] = v0 * a.p.vor[0][pi];
-        v[2] = v0 * a.p.vor[1][pi];
-        v[3] = v0 * a.p.vor[2][pi];
-        v[4] = v0 * (float)((double)a.p.buoy[pi] + g.glat * qc);
-        v[5] = v0 * a.p.buoy[pi];
-        v[6] = v0 * a.p.hum[pi];
+        v[1] = v0 * gl(a.p.vor[0])[pi];
+        v[2] = v0 * gl(a.p.vor[1])[pi];
+        v[3] = v0 * gl(a.p.vor[2])[pi];
+        v[4] = v0 * (float)((double)gl(a.p.buoy)[pi] + g.glat * qc);
+        v[5] = v0 * gl(a.p.buoy)[pi];
+        v[6] = v0 * h;
     };
 
     // ---- pass 1: the tile's largest contribution of every field, and
@@ -322,9 +325,9 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
 #pragma unroll
     for (int f = 0; f <= P2G_NF; ++f) mx[f] = 0.0f;
     for (int s = s0 + sub; s < s1; s += P2G_TPC) {
-        const int pi = a.perm[s];
+        const int pi = gl(a.perm)[s];
         float v[P2G_NF];
-        values(pi, a.p.z[pi], v);   // the same values pass 2 scales
+        values(pi, gl(a.p.z)[pi], v);   // the same values pass 2 scales
 #pragma unroll
         for (int f = 0; f < P2G_NF; ++f) mx[f] = fmaxf(mx[f], fabsf(v[f]));
         mx[P2G_NF] = fmaxf(mx[P2G_NF], fabsf(v[4] - v[5]));
@@ -354,10 +357,10 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     // ---- pass 2: support points, trilinear weights, integer accumulation ----
     int err = 0;
     for (int s = s0 + sub; s < s1; s += P2G_TPC) {
-        const int pi = a.perm[s];
-        const double pos[3] = {a.p.x[pi], a.p.y[pi], a.p.z[pi]};
-        const double ve[3] = {axes[0][pi], axes[1][pi], axes[2][pi]};
-        const double vt[3] = {axes[3][pi], axes[4][pi], axes[5][pi]};
+        const int pi = gl(a.perm)[s];
+        const double pos[3] = {gl(a.p.x)[pi], gl(a.p.y)[pi], gl(a.p.z)[pi]};
+        const double ve[3] = {gl(axes[0])[pi], gl(axes[1])[pi], gl(axes[2])[pi]};
+        const double vt[3] = {gl(axes[3])[pi], gl(axes[4])[pi], gl(axes[5])[pi]};
         float v[P2G_NF];
         values(pi, pos[2], v);
         float cs[P2G_NF];
@@ -430,7 +433,7 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
 #pragma unroll
     for (int f = 0; f < P2G_NF; ++f) inv[f] = (sc[f] > 0.0f) ? 1.0f / sc[f] : 0.0f;   // a power of two: exact
     constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
-    float *stage = a.staging + (int64_t)ib.blk * nbox * PLANE;
+    EPIC_GLOBAL float *stage = gl(a.staging) + (int64_t)ib.blk * nbox * PLANE;
     for (int u = tid; u < nbox * PLANE; u += blockDim.x) {
         const int f = u % P2G_NF, nd = u / P2G_NF;
         const int pz = nd / (P2G_BOX * P2G_BOX), yx = nd - pz * (P2G_BOX * P2G_BOX);
@@ -469,7 +472,7 @@ __device__ inline void stage_sum(const FinalizeArgs &a, int i, int j, int k, flo
                     const int tx = wrap(txa + dtx, a.ntx);
                     for (int lx = wrap(i - (tx * P2G_TX - 1), g.nx); lx < P2G_BOX; lx += g.nx) {
                         const int wg = (cz * a.nty + ty) * a.ntx + tx;
-                        const float *s = a.staging + ((int64_t)wg * nbox + lp) * PLANE + (ly * P2G_BOX + lx) * P2G_NF;
+                        const EPIC_GLOBAL float *s = gl(a.staging) + ((int64_t)wg * nbox + lp) * PLANE + (ly * P2G_BOX + lx) * P2G_NF;
 #pragma unroll
                         for (int f = 0; f < P2G_NF; ++f) out[f] += s[f];
                     }
@@ -497,7 +500,7 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     const int i = (int)(col % g.nx), j = (int)(col / g.nx);
     const int64_t PS = plane_stride(g), FS = field_stride(g);
     const int64_t base = (int64_t)j * g.nx + i;
-    auto F = [&](int slot, int kk) -> float & { return a.fld[slot * FS + (int64_t)(kk + 1) * PS + base]; };
+    auto F = [&](int slot, int kk) -> EPIC_GLOBAL float & { return gl(a.fld)[slot * FS + (int64_t)(kk + 1) * PS + base]; };
 
     // the node value of plane kk (folds of parcel_interpl.f90:258-287)
     auto folded = [&](int kk, float r[P2G_NF]) {
@@ -549,7 +552,7 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     }
     // parcel counts (:201-205, folds :239-243)
     auto cnt = [&](const int *c, int kk) -> int {
-        return (kk >= -1 && kk <= nz) ? c[((int64_t)(kk + 1) * g.ny + j) * g.nx + i] : 0;
+        return (kk >= -1 && kk <= nz) ? gl(c)[((int64_t)(kk + 1) * g.ny + j) * g.nx + i] : 0;
     };
     int np = cnt(a.count, k), ns = cnt(a.scount, k);
     if (k == 0) {
@@ -560,13 +563,13 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
         np += cnt(a.count, nz);
         ns += cnt(a.scount, nz);
     }
-    a.nparg[(int64_t)(k + 1) * PS + base] = np;
-    a.nsparg[(int64_t)(k + 1) * PS + base] = ns;
+    gl(a.nparg)[(int64_t)(k + 1) * PS + base] = np;
+    gl(a.nsparg)[(int64_t)(k + 1) * PS + base] = ns;
     // clear the histogram the next push fills (never the one read here)
     if (a.clear_count && k <= nz) {
         const int64_t c = ((int64_t)(k + 1) * g.ny + j) * g.nx + i;
-        a.clear_count[c] = 0;
-        a.clear_scount[c] = 0;
+        gl(a.clear_count)[c] = 0;
+        gl(a.clear_scount)[c] = 0;
     }
 }
 
@@ -583,16 +586,16 @@ __global__ __launch_bounds__(256) void k_axes(const AxesArgs *__restrict__ tab, 
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
         if (q >= p.n) break;
-        const double B[5] = {p.B[0][q], p.B[1][q], p.B[2][q], p.B[3][q], p.B[4][q]};
+        const double B[5] = {gl(p.B[0])[q], gl(p.B[1])[q], gl(p.B[2])[q], gl(p.B[3])[q], gl(p.B[4])[q]};
         double ve[3], vt[3];
-        if (!support_axes(B, (double)p.vol[q], ve, vt)) {
+        if (!support_axes(B, (double)gl(p.vol)[q], ve, vt)) {
             e |= ERR_B33;
             ve[0] = ve[1] = ve[2] = vt[0] = vt[1] = vt[2] = 0.0;
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            a.ax[d][q] = (float)ve[d];
-            a.ax[3 + d][q] = (float)vt[d];
+            gl(a.ax[d])[q] = (float)ve[d];
+            gl(a.ax[3 + d])[q] = (float)vt[d];
         }
     }
     if (e) atomicOr(a.err, e);

@@ -20,9 +20,9 @@ __global__ __launch_bounds__(256) void k_permute(const PermArgs *__restrict__ ta
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t s = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
         if (s >= a.n) break;
-        const int64_t q = a.perm ? a.perm[s] : s;   // no permutation: a copy
-        if (a.wide) ((double *)a.dst)[s] = ((const double *)a.src)[q];
-        else ((float *)a.dst)[s] = ((const float *)a.src)[q];   // 4-byte words (float or int)
+        const int64_t q = a.perm ? gl(a.perm)[s] : s;   // no permutation: a copy
+        if (a.wide) gl((double *)a.dst)[s] = gl((const double *)a.src)[q];
+        else gl((float *)a.dst)[s] = gl((const float *)a.src)[q];   // 4-byte words (float or int)
     }
 }
 

@@ -31,14 +31,14 @@ __global__ __launch_bounds__(256) void k_ts_max(const TsArgs *__restrict__ tab, 
         const int64_t o = (int64_t)(k + 1) * PS + (int64_t)j * g.nx + i;
         double v[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) v[c] = (double)a.fld[(F_VELGRADG + c) * FS + o];
+        for (int c = 0; c < 8; ++c) v[c] = (double)gl(a.fld)[(F_VELGRADG + c) * FS + o];
         const double A[9] = {v[DUDX], 0.5 * (v[DUDY] + v[DVDX]), 0.5 * (v[DUDZ] + v[DWDX]),
                              0.5 * (v[DUDY] + v[DVDX]), v[DVDY], 0.5 * (v[DVDZ] + v[DWDY]),
                              0.5 * (v[DUDZ] + v[DWDX]), 0.5 * (v[DVDZ] + v[DWDY]), -(v[DUDX] + v[DVDY])};
         double D[3], V[3][3];
         scherzinger(A, D, V);
         gm = fmax(fabs(D[0]), fmax(fabs(D[1]), fabs(D[2])));
-        const float *b = a.fld + F_TBUOYG * FS;
+        const EPIC_GLOBAL float *b = gl(a.fld) + F_TBUOYG * FS;
         const int ip = (i + 1 == g.nx) ? 0 : i + 1, im = (i == 0) ? g.nx - 1 : i - 1;
         const int jp = (j + 1 == g.ny) ? 0 : j + 1, jm = (j == 0) ? g.ny - 1 : j - 1;
         const int64_t ro = (int64_t)(k + 1) * PS;
