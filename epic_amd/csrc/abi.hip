@@ -85,7 +85,7 @@ struct epic_member {
     int64_t n_splits = 0, n_merges = 0;
     double *diag_d = nullptr, *diag_h = nullptr;
     // physical re-sort into cell order every resort_every steps (0: never)
-    int resort_every = 10, steps_since_sort = 0;
+    int resort_every = 3, steps_since_sort = 0;
     double *sort_tmp = nullptr;
     Plan pl_resort;
     uint64_t gen = 0;   // bumped whenever plans built over this member go stale

@@ -195,18 +195,30 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
     __syncthreads();
     const int total = row_off[nrow];
 
+    // the row holding entry q: largest r with row_off[r] <= q
+    auto row_of = [&](int q) {
+        int r = 0;
+        for (int step = 32; step > 0; step >>= 1)
+            if (r + step < nrow && row_off[r + step] <= q) r += step;
+        return r;
+    };
+    int r_nx = 0, pi_nx = 0;   // the next round's entry, loaded one round ahead
+    if (tid < total) {
+        r_nx = row_of(tid);
+        pi_nx = gl(a.perm)[row_slot[r_nx] + (tid - row_off[r_nx])];
+    }
     for (int base = 0; base < total; base += G2P_THREADS) {   // uniform rounds: the ballot needs every lane
         const int q = base + tid;
+        const int r = r_nx, pi = pi_nx;
+        if (q + G2P_THREADS < total) {
+            r_nx = row_of(q + G2P_THREADS);
+            pi_nx = gl(a.perm)[row_slot[r_nx] + (q + G2P_THREADS - row_off[r_nx])];
+        }
         int key = -1;
         bool small = false;
         auto process = [&]() {
             if (q >= total) return;
-            // the row holding entry q: largest r with row_off[r] <= q
-            int r = 0;
-            for (int step = 32; step > 0; step >>= 1)
-                if (r + step < nrow && row_off[r + step] <= q) r += step;
             const int k = k0 + r / P2G_TY;
-            const int pi = gl(a.perm)[row_slot[r] + (q - row_off[r])];
             double pos[3] = {gl(a.p.x)[pi], gl(a.p.y)[pi], gl(a.p.z)[pi]};
             double B[5];
 #pragma unroll

@@ -215,7 +215,7 @@ int epic_field_diagnostics(epic_handle h, double *stats);
  * (epic_get_parcels returns the caller's order); the stage tendencies are
  * not kept.  No counterpart in the reference, whose parcels are not
  * sorted.  epic_ls_rk_step and epic_batch_step re-sort a member every
- * `steps` steps (10 by default; 0 never). */
+ * `steps` steps (3 by default; 0 never). */
 int epic_resort(epic_handle h);
 int epic_set_resort_interval(epic_handle h, int steps);
 
