@@ -87,6 +87,8 @@ struct epic_member {
     // physical re-sort into cell order every resort_every steps (0: never)
     int resort_every = 3, steps_since_sort = 0;
     double *sort_tmp = nullptr;
+    // p2g's fixed-point maxima (P2G_NF + 1)
+    float *pmax_out = nullptr;
     Plan pl_resort;
     uint64_t gen = 0;   // bumped whenever plans built over this member go stale
     void invalidate_plans()
@@ -368,6 +370,7 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
         a.cell_start = m->cell_start;
         a.staging = m->staging;
         a.err = m->err_d;
+        a.tmax = m->pmax_out;
         a.tz = L.tz;
         a.ntx = L.ntx;
         a.nty = L.nty;
@@ -383,12 +386,16 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
         f.nsparg = m->nsparg;
         f.clear_count = q.clr >= 0 ? m->cnt[q.clr] : nullptr;
         f.clear_scount = q.clr >= 0 ? m->scnt[q.clr] : nullptr;
+        f.clear_tmax = m->pmax_out;
         f.tz = L.tz;
         f.ntx = L.ntx;
         f.nty = L.nty;
         f.nchz = L.nchz;
         fi.push_back(f);
     }
+    std::vector<P2GMaxArgs> mx;
+    for (epic_member *m : ms) mx.push_back(P2GMaxArgs{m->g, m->pd, p2g_max_blocks(m->n), m->pmax_out});
+    add_p2g_max(P, PH_P2G, mx);
     add_exclusive_scan(P, PH_SORT, sc);
     add_place(P, PH_SORT, pl);
     add_p2g(P, PH_P2G, ti, fi);
@@ -785,6 +792,7 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     for (int i = 0; i < NSPEC; ++i)
         if (bad(dalloc(&m->spec[i], spec_size(g)))) return EPIC_E_NOMEM;
     if (bad(dalloc(&m->err_d, 4)) || bad(dalloc(&m->flags_d, 4)) || bad(dalloc(&m->tm_d, 1))) return EPIC_E_NOMEM;
+    if (bad(dalloc(&m->pmax_out, P2G_NF + 1))) return EPIC_E_NOMEM;
     if (bad(hipHostMalloc((void **)&m->err_h, sizeof(int) * 4, hipHostMallocDefault)) ||
         bad(hipHostMalloc((void **)&m->tm_h, sizeof(StepTime), hipHostMallocDefault)))
         return EPIC_E_NOMEM;
@@ -821,7 +829,7 @@ int epic_destroy(epic_handle m)
     void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
                     m->cnt[2], m->scnt[2], m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
                     m->twx, m->twy, m->twz, m->err_d, m->flags_d, m->vw_d, m->tm_d, m->diag_d,
-                    m->sort_tmp};
+                    m->sort_tmp, m->pmax_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < NSPEC; ++i)
@@ -1047,25 +1055,37 @@ int epic_grid2par(epic_handle m, int add)
 }
 
 // the arrays that outlive a step: positions, vorticity, B, volume,
-// buoyancy, humidity, caller index, next-stage keys, cached half axes; each
-// is gathered into the cell order through sort_tmp and copied back (stage
-// deltas and strain are rewritten before they are read again)
+// buoyancy, humidity, caller index, next-stage keys, cached half axes; all
+// are gathered into the cell order in sort_tmp by one launch and copied back
+// by another (stage deltas and strain are rewritten before they are read
+// again)
+static_assert(RESORT_W == 3 && RESORT_F == 11 + 2 + 6, "positions; attributes, orig, keys, half axes");
+static int64_t resort_tmp_doubles(int64_t cap) { return (RESORT_W + (RESORT_F + 1) / 2) * cap; }
+
 static void build_resort(epic_member *m)
 {
     Plan &P = m->pl_resort;
     const int64_t n = m->n, c = m->cap, nc = ncells(m->g);
     add_exclusive_scan(P, PH_SORT, {ScanArgs{m->cnt[0], nc, m->cell_start, m->bsum, m->cursor}});
     add_place(P, PH_SORT, {PlaceArgs{n, m->keys, m->cell_start, m->cursor, m->perm}});
-    std::vector<std::pair<void *, int>> arrays;
-    for (int d = 0; d < 3; ++d) arrays.push_back({m->pos + d * c, 1});
-    for (int a = 0; a < 11; ++a) arrays.push_back({m->attr + a * c, 0});
-    arrays.push_back({m->orig, 0});
-    arrays.push_back({m->keys, 0});
-    for (int a = 0; a < 6; ++a) arrays.push_back({m->ax_stale[a], 0});
-    for (const auto &ar : arrays) {
-        add_permute(P, PH_SORT, {PermArgs{ar.first, m->sort_tmp, m->perm, n, ar.second}});
-        add_permute(P, PH_SORT, {PermArgs{m->sort_tmp, ar.first, nullptr, n, ar.second}});
+    ResortArgs a{};
+    a.perm = m->perm;
+    a.n = n;
+    for (int d = 0; d < RESORT_W; ++d) {
+        a.w[d] = m->pos + d * c;
+        a.tw[d] = m->sort_tmp + d * c;
     }
+    int k = 0;
+    for (int i = 0; i < 11; ++i) a.f[k++] = m->attr + i * c;   // vorticity, B, volume, buoyancy, humidity
+    a.f[k++] = (float *)m->orig;
+    a.f[k++] = (float *)m->keys;
+    for (int i = 0; i < 6; ++i) a.f[k++] = m->ax_stale[i];
+    float *tf = (float *)(m->sort_tmp + RESORT_W * c);
+    for (int i = 0; i < RESORT_F; ++i) a.tf[i] = tf + i * c;
+    a.gather = 1;
+    add_resort(P, PH_SORT, {a});
+    a.gather = 0;
+    add_resort(P, PH_SORT, {a});
 }
 
 static int resort(epic_member *m)
@@ -1074,7 +1094,7 @@ static int resort(epic_member *m)
     if (m->n == 0) return EPIC_OK;
     int rc = ensure_keys(m);   // keys and the histogram cnt[0] of the current positions
     if (rc) return rc;
-    if (!m->sort_tmp) HIPOK(m, dalloc(&m->sort_tmp, m->cap));
+    if (!m->sort_tmp) HIPOK(m, dalloc(&m->sort_tmp, resort_tmp_doubles(m->cap)));
     if (m->pl_resort.empty()) build_resort(m);
     rc = run_plan(m, m->pl_resort);
     if (rc) return rc;

@@ -60,9 +60,20 @@ struct P2GArgs {
     const int *cell_start;
     float *staging;
     int *err;
+    const float *tmax;          // P2G_NF + 1 maxima from k_p2g_max (fixed-point scales)
     int tz, ntx, nty, nchz;
     int lpitch;                 // LDS row pitch of the node box (set by add_p2g)
 };
+// largest |contribution| per field (and |tbuoy - buoy|) over a member's parcels
+constexpr int P2G_MAX_BLOCKS = 4096;
+struct P2GMaxArgs {
+    Grid g;
+    ParcelsDev p;
+    int nblk;                   // p2g_max_blocks(p.n)
+    float *out;                 // P2G_NF + 1, zero on entry; cleared again by the finalize kernel
+};
+int p2g_max_blocks(int64_t n);
+void add_p2g_max(Plan &P, int phase, const std::vector<P2GMaxArgs> &items);
 struct FinalizeArgs {
     Grid g;
     const float *staging;
@@ -70,6 +81,7 @@ struct FinalizeArgs {
     float *fld;     // F_VOLG .. F_HUMG
     int *nparg, *nsparg;
     int *clear_count, *clear_scount;   // histograms cleared for the next push (may be null)
+    float *clear_tmax;                 // the tile kernel's maxima, zeroed for the next k_p2g_max (may be null)
     int tz, ntx, nty, nchz;
 };
 void add_axes(Plan &P, int phase, const std::vector<AxesArgs> &items);
@@ -208,14 +220,19 @@ void launch_field_diagnostics(const Grid &g, const float *fld, const int *nparg,
                               double *out, hipStream_t s);
 
 // ---- physical re-sort (resort.hip) ----
-struct PermArgs {
-    const void *src;
-    void *dst;
-    const int *perm;        // dst[s] = src[perm[s]]; null: a plain copy
+// the arrays of a member that outlive a step: positions (8-byte), then the
+// 11 attributes, the caller index, the keys and the 6 cached half axes
+constexpr int RESORT_W = 3, RESORT_F = 19;
+struct ResortArgs {
+    const int *perm;
     int64_t n;
-    int wide;               // 8-byte elements, else 4-byte
+    int gather;             // 1: tw/tf[s] = w/f[perm[s]]; 0: w/f[s] = tw/tf[s]
+    double *w[RESORT_W];
+    float *f[RESORT_F];     // float or int bits
+    double *tw[RESORT_W];
+    float *tf[RESORT_F];
 };
-void add_permute(Plan &P, int phase, const std::vector<PermArgs> &items);
+void add_resort(Plan &P, int phase, const std::vector<ResortArgs> &items);
 
 // ---- utilities (abi.hip) ----
 struct FillArgs {

@@ -11,7 +11,7 @@
 //   5. k_p2g_tiles     one work-group per (8x8 column tile, z chunk of tz
 //                      layers), one thread per cell: support points in FP64,
 //                      each of the 4 x 8 x 7 contributions of a parcel scaled
-//                      by a per-tile power of two, rounded and added to a
+//                      by a per-member power of two, rounded and added to a
 //                      64-bit integer LDS accumulator of its node; the box is
 //                      converted back and written to a per-tile staging box
 //   6. k_p2g_finalize  per node, sums the staging boxes that cover it in a
@@ -239,13 +239,14 @@ __global__ void k_cell_order(int64_t nc, const int *__restrict__ cell_start, con
 // cell taking every P2G_TPC-th parcel of it (the lanes of a wave are 64
 // distinct cells, so their atomics rarely meet on one node).
 // Every contribution w * (vol/4) * value is scaled by a power of two chosen
-// per tile and field (so that no single point's contribution exceeds 2^28),
+// per member and field from k_p2g_max's maxima (so that no single point's
+// contribution exceeds 2^28),
 // rounded to an integer and added to a 64-bit LDS accumulator of its node;
 // the points of one parcel that fall in the same cell are combined first.  Integer
 // sums do not depend on the order of the additions, so the node sums are
 // bitwise reproducible however the waves are scheduled, with no ordering
 // pass and no barriers between parcels; the only rounding is that of each
-// contribution (2^-29 of the tile's largest) and of the final conversion.
+// contribution (2^-29 of the member's largest) and of the final conversion.
 // Node box of a tile: x, y in [-1, 9] around the tile, node planes
 // [k0 - 1, k0 + tz + 1]; rows padded to lpitch 64-bit words (12 spreads the
 // 8 x 8 lanes of a wave evenly over the 32 LDS banks).
@@ -263,6 +264,60 @@ __device__ inline float fixed_scale(float max_abs, int &e)
     }
     frexpf(max_abs, &e);   // max_abs < 2^e
     return ldexpf(1.0f, 28 - e);
+}
+
+// the gridded quantities of a parcel: vol/4 and vol/4 times vorticity,
+// total buoyancy (with condensation), buoyancy and humidity (a parcel
+// without humidity has no condensate: the saturation humidity is positive,
+// so its exp is skipped)
+__device__ inline void p2g_values(const Grid &g, const ParcelsDev &p, int pi, double z, float v[P2G_NF])
+{
+    const float h = gl(p.hum)[pi];
+    const double qc = (h > 0.0f) ? fmax(0.0, (double)h - g.q0 * exp(g.lambda_c * (g.lower[2] - z))) : 0.0;
+    const float v0 = (float)(0.25 * (double)gl(p.vol)[pi]);
+    v[0] = v0;
+    v[1] = v0 * gl(p.vor[0])[pi];
+    v[2] = v0 * gl(p.vor[1])[pi];
+    v[3] = v0 * gl(p.vor[2])[pi];
+    v[4] = v0 * (float)((double)gl(p.buoy)[pi] + g.glat * qc);
+    v[5] = v0 * gl(p.buoy)[pi];
+    v[6] = v0 * h;
+}
+
+// The largest |contribution| of every field over a member's parcels, and
+// the largest |total buoyancy - buoyancy| one: one coalesced pass in
+// storage order by at most P2G_MAX_BLOCKS work-groups, whose block maxima
+// go to out[] by integer atomicMax on the bits of the non-negative floats.
+// out[] is zero when this runs: it starts zeroed, and the finalize kernel
+// of the same par2grid clears it once the tile kernel has taken its scales
+// from it (a value left by an interrupted sequence would only be larger, a
+// coarser but still valid scale).
+__global__ __launch_bounds__(256) void k_p2g_max(const P2GMaxArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+{
+    const ItemBlock ib = item_block(bmap);
+    const P2GMaxArgs &a = tab[ib.item];
+    float mx[P2G_NF + 1];
+#pragma unroll
+    for (int f = 0; f <= P2G_NF; ++f) mx[f] = 0.0f;
+    for (int64_t q = (int64_t)ib.blk * 256 + threadIdx.x; q < a.p.n; q += (int64_t)a.nblk * 256) {
+        float v[P2G_NF];
+        p2g_values(a.g, a.p, (int)q, gl(a.p.z)[q], v);
+#pragma unroll
+        for (int f = 0; f < P2G_NF; ++f) mx[f] = fmaxf(mx[f], fabsf(v[f]));
+        mx[P2G_NF] = fmaxf(mx[P2G_NF], fabsf(v[4] - v[5]));
+    }
+    __shared__ float wm[4][P2G_NF + 1];
+#pragma unroll
+    for (int f = 0; f <= P2G_NF; ++f) {
+        for (int o = 32; o > 0; o >>= 1) mx[f] = fmaxf(mx[f], __shfl_xor(mx[f], o, 64));
+        if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6][f] = mx[f];
+    }
+    __syncthreads();
+    if (threadIdx.x <= P2G_NF) {
+        const int f = threadIdx.x;
+        const float m = fmaxf(fmaxf(wm[0][f], wm[1][f]), fmaxf(wm[2][f], wm[3][f]));
+        if (m > 0.0f) atomicMax((unsigned int *)a.out + f, __float_as_uint(m));
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
@@ -283,7 +338,6 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     const int lplane = P2G_BOX * lp;              // words per LDS node plane
     const int lfield = nbox * lplane;             // words per field
     extern __shared__ unsigned long long acc[];   // [field][plane][y][x]
-    __shared__ float wmax[1024 / 64][P2G_NF + 1];
     __shared__ int s_err;
 
     for (int u = tid; u < P2G_NF * lfield; u += blockDim.x) acc[u] = 0ull;
@@ -299,53 +353,14 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
         s0 = gl(a.cell_start)[c];
         s1 = gl(a.cell_start)[c + 1];
     }
-    // the gridded quantities of a parcel: vol/4 and vol/4 times vorticity,
-    // total buoyancy (with condensation), buoyancy and humidity
-    // (a parcel without humidity has no condensate: the saturation humidity
-    // is positive, so its exp is skipped)
-    auto values = [&](int pi, double z, float v[P2G_NF]) {
-        const float h = gl(a.p.hum)[pi];
-        const double qc = (h > 0.0f) ? fmax(0.0, (double)h - g.q0 * exp(g.lambda_c * (g.lower[2] - z))) : 0.0;
-        const float v0 = (float)(0.25 * (double)gl(a.p.vol)[pi]);
-        v[0] = v0;
-        v[1] = v0 * gl(a.p.vor[0])[pi];
-        v[2] = v0 * gl(a.p.vor[1])[pi];
-        v[3] = v0 * gl(a.p.vor[2])[pi];
-        v[4] = v0 * (float)((double)gl(a.p.buoy)[pi] + g.glat * qc);
-        v[5] = v0 * gl(a.p.buoy)[pi];
-        v[6] = v0 * h;
-    };
-
-    // ---- pass 1: the tile's largest contribution of every field, and
-    // whether the moist fields differ from the dry ones: with no
-    // condensation anywhere in the tile the total buoyancy equals the
-    // buoyancy (field 5 is a copy of field 4) and with no humidity field 6 is
-    // zero; their atomics are then skipped and the results are the same bits
-    float mx[P2G_NF + 1];   // [P2G_NF]: largest |tbuoy - buoy| contribution
+    // fixed-point scales from the member's largest contributions; with no
+    // condensation anywhere the total buoyancy equals the buoyancy (field 5
+    // is a copy of field 4) and with no humidity field 6 is zero: their
+    // atomics are then skipped and the results are the same bits
+    float mx[P2G_NF + 1];
 #pragma unroll
-    for (int f = 0; f <= P2G_NF; ++f) mx[f] = 0.0f;
-    for (int s = s0 + sub; s < s1; s += P2G_TPC) {
-        const int pi = gl(a.perm)[s];
-        float v[P2G_NF];
-        values(pi, gl(a.p.z)[pi], v);   // the same values pass 2 scales
-#pragma unroll
-        for (int f = 0; f < P2G_NF; ++f) mx[f] = fmaxf(mx[f], fabsf(v[f]));
-        mx[P2G_NF] = fmaxf(mx[P2G_NF], fabsf(v[4] - v[5]));
-    }
-#pragma unroll
-    for (int f = 0; f <= P2G_NF; ++f)
-        for (int o = 32; o > 0; o >>= 1) mx[f] = fmaxf(mx[f], __shfl_xor(mx[f], o, 64));
-    if ((tid & 63) == 0)
-#pragma unroll
-        for (int f = 0; f <= P2G_NF; ++f) wmax[tid >> 6][f] = mx[f];
-    __syncthreads();
+    for (int f = 0; f <= P2G_NF; ++f) mx[f] = gl(a.tmax)[f];
     float sc[P2G_NF];
-#pragma unroll
-    for (int f = 0; f <= P2G_NF; ++f) {
-        float m = 0.0f;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, wmax[w][f]);
-        mx[f] = m;
-    }
 #pragma unroll
     for (int f = 0; f < P2G_NF; ++f) {
         int e;
@@ -353,6 +368,7 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     }
     const bool own5 = mx[P2G_NF] > 0.0f;   // tbuoy differs from buoy somewhere
     const bool own6 = mx[6] > 0.0f;        // humidity somewhere
+    __syncthreads();   // the accumulators are cleared
 
     // ---- pass 2: support points, trilinear weights, integer accumulation ----
     int err = 0;
@@ -362,7 +378,7 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
         const double ve[3] = {gl(axes[0])[pi], gl(axes[1])[pi], gl(axes[2])[pi]};
         const double vt[3] = {gl(axes[3])[pi], gl(axes[4])[pi], gl(axes[5])[pi]};
         float v[P2G_NF];
-        values(pi, pos[2], v);
+        p2g_values(g, a.p, pi, pos[2], v);
         float cs[P2G_NF];
 #pragma unroll
         for (int f = 0; f < P2G_NF; ++f) cs[f] = v[f] * sc[f];   // exact: sc is a power of two
@@ -494,6 +510,7 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     const int nz = g.nz;
     const int64_t ncol = (int64_t)g.nx * g.ny;
     const int64_t tq = ib.blk * (int64_t)blockDim.x + threadIdx.x;
+    if (a.clear_tmax && tq <= P2G_NF) gl(a.clear_tmax)[tq] = 0.0f;   // the tile kernel has read them
     if (tq >= ncol * (nz + 3)) return;
     const int k = (int)(tq / ncol) - 1;
     const int64_t col = tq - (int64_t)(k + 1) * ncol;
@@ -624,6 +641,15 @@ void add_place(Plan &P, int phase, const std::vector<PlaceArgs> &items)
     for (const PlaceArgs &a : items) n.push_back(a.n);
     P.add((const void *)k_place, 256, 0, phase, items, parcel_blocks_of(n));
 }
+
+void add_p2g_max(Plan &P, int phase, const std::vector<P2GMaxArgs> &items)
+{
+    std::vector<int> nb;
+    for (const P2GMaxArgs &a : items) nb.push_back(a.nblk);
+    P.add((const void *)k_p2g_max, 256, 0, phase, items, nb);
+}
+
+int p2g_max_blocks(int64_t n) { return (int)std::min<int64_t>((n + 1023) / 1024, P2G_MAX_BLOCKS); }
 
 void add_axes(Plan &P, int phase, const std::vector<AxesArgs> &items)
 {

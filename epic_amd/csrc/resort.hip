@@ -2,8 +2,8 @@
 // parcels through the cell-sort permutation, so its gathers coalesce only as
 // long as the physical order follows the cells; as parcels drift, every few
 // steps the attribute arrays that outlive a step are gathered into the
-// sorted order (dst[s] = src[perm[s]], through one scratch array) and
-// copied back.  Parcel order is not observable: epic_get_parcels restores
+// sorted order (dst[s] = src[perm[s]], into scratch arrays) and copied
+// back.  Parcel order is not observable: epic_get_parcels restores
 // the caller's order through orig, which is permuted with the rest.
 #include "common.h"
 #include "launch.h"
@@ -12,25 +12,48 @@
 
 namespace epic {
 
-__global__ __launch_bounds__(256) void k_permute(const PermArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+// Every array of a member in one pass: a thread gathers its slot's 22
+// values (3 of 8 bytes, 19 of 4) into the scratch arrays, reading the
+// permutation once and keeping all 22 loads in flight; a second launch
+// copies them back.
+__global__ __launch_bounds__(256) void k_resort(const ResortArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
-    const PermArgs &a = tab[ib.item];
+    const ResortArgs &a = tab[ib.item];
 #pragma unroll
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t s = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
         if (s >= a.n) break;
-        const int64_t q = a.perm ? gl(a.perm)[s] : s;   // no permutation: a copy
-        if (a.wide) gl((double *)a.dst)[s] = gl((const double *)a.src)[q];
-        else gl((float *)a.dst)[s] = gl((const float *)a.src)[q];   // 4-byte words (float or int)
+        double w[RESORT_W];
+        float f[RESORT_F];
+        if (a.gather) {
+            const int64_t q = gl(a.perm)[s];
+#pragma unroll
+            for (int c = 0; c < RESORT_W; ++c) w[c] = gl(a.w[c])[q];
+#pragma unroll
+            for (int c = 0; c < RESORT_F; ++c) f[c] = gl(a.f[c])[q];
+#pragma unroll
+            for (int c = 0; c < RESORT_W; ++c) gl(a.tw[c])[s] = w[c];
+#pragma unroll
+            for (int c = 0; c < RESORT_F; ++c) gl(a.tf[c])[s] = f[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < RESORT_W; ++c) w[c] = gl(a.tw[c])[s];
+#pragma unroll
+            for (int c = 0; c < RESORT_F; ++c) f[c] = gl(a.tf[c])[s];
+#pragma unroll
+            for (int c = 0; c < RESORT_W; ++c) gl(a.w[c])[s] = w[c];
+#pragma unroll
+            for (int c = 0; c < RESORT_F; ++c) gl(a.f[c])[s] = f[c];
+        }
     }
 }
 
-void add_permute(Plan &P, int phase, const std::vector<PermArgs> &items)
+void add_resort(Plan &P, int phase, const std::vector<ResortArgs> &items)
 {
     std::vector<int> nb;
-    for (const PermArgs &a : items) nb.push_back((int)std::max<int64_t>(1, (a.n + PER_BLOCK - 1) / PER_BLOCK));
-    P.add((const void *)k_permute, 256, 0, phase, items, nb);
+    for (const ResortArgs &a : items) nb.push_back((int)std::max<int64_t>(1, (a.n + PER_BLOCK - 1) / PER_BLOCK));
+    P.add((const void *)k_resort, 256, 0, phase, items, nb);
 }
 
 }  // namespace epic
