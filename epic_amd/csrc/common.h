@@ -20,7 +20,7 @@ constexpr int P2G_TX = 8;        // p2g tile cells in x
 constexpr int P2G_TY = 8;        // p2g tile cells in y
 constexpr int P2G_BOX = 11;      // node box per tile side: 1 lower + 8 + 2 upper
 constexpr int P2G_NF = 7;        // vol, xi, eta, zeta, tbuoy, dbuoy, hum
-constexpr int P2G_TPC = 2;       // par2grid threads per cell
+constexpr int P2G_TPC = 2;       // par2grid threads per cell (neighbouring lanes)
 
 // error bits reported through the member's device error word
 enum : int {

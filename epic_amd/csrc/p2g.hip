@@ -236,8 +236,9 @@ __global__ void k_cell_order(int64_t nc, const int *__restrict__ cell_start, con
 // 5. tile scatter into fixed-point LDS accumulators
 // ---------------------------------------------------------------------------
 // One work-group per (8 x 8 column tile, tz cell layers), P2G_TPC threads per
-// cell taking every P2G_TPC-th parcel of it (the lanes of a wave are 64
-// distinct cells, so their atomics rarely meet on one node).
+// cell taking every P2G_TPC-th parcel of it, in neighbouring lanes: a wave
+// covers 64 / P2G_TPC cells, so its loads share lines, and the points of
+// different cells' parcels rarely add to one node in the same instruction.
 // Every contribution w * (vol/4) * value is scaled by a power of two chosen
 // per member and field from k_p2g_max's maxima (so that no single point's
 // contribution exceeds 2^28),
@@ -343,8 +344,10 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     for (int u = tid; u < P2G_NF * lfield; u += blockDim.x) acc[u] = 0ull;
     if (tid == 0) s_err = 0;
 
-    // this thread's cell and its share of the cell's parcels
-    const int ct = tid % (64 * tz), sub = tid / (64 * tz);
+    // this thread's cell and its share of the cell's parcels: the P2G_TPC
+    // threads of a cell are neighbouring lanes, so one load instruction
+    // covers 64 / P2G_TPC cells, whose parcels share fewer lines
+    const int ct = tid / P2G_TPC, sub = tid % P2G_TPC;
     const int cx = ct & 7, cy = (ct >> 3) & 7, lz = ct >> 6;
     const int k = k0 + lz;
     int s0 = 0, s1 = 0;

@@ -70,6 +70,23 @@ def test_par2grid_deterministic():
         assert np.array_equal(a.get_field(f), c.get_field(f)), f
 
 
+def test_par2grid_scales_follow_the_current_parcels():
+    """The fixed-point scales come from the parcels of this par2grid: after
+    one with vorticity a million times larger, the same grid is within the
+    tolerance (left-over maxima would coarsen it by that factor)."""
+    mem = MEMBERS["thermal16"]
+    gm = Member.from_workload(mem)
+    gm.set_parcels(mem.position, mem.vorticity * 1e6, mem.B, mem.volume, mem.buoyancy, mem.humidity)
+    gm.par2grid()
+    gm.set_parcels(mem.position, mem.vorticity, mem.B, mem.volume, mem.buoyancy, mem.humidity)
+    gm.par2grid()
+    st = oracle_state(mem)
+    st.par2grid()
+    nx, ny = mem.nx, mem.ny
+    e = maxrel(interior(gm.get_field("vortg"), nx, ny, 0, None), interior(st.vortg, nx, ny, 0, None))
+    assert e <= TOL, e
+
+
 def _taylor_green(nx, ny, nz):
     lower = (-np.pi, -np.pi, -0.5 * np.pi)
     extent = (2 * np.pi,) * 3
