@@ -342,7 +342,8 @@ struct P2GSel {
     float *const *ax_alt;
     const int *use_alt;
     int rd;
-    int clr;   // -1: none
+    int clr;                  // -1: none
+    bool max_from_push = false;   // the previous push stored the fixed-point maxima: no k_p2g_max
 };
 
 void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
@@ -394,8 +395,9 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
         fi.push_back(f);
     }
     std::vector<P2GMaxArgs> mx;
-    for (epic_member *m : ms) mx.push_back(P2GMaxArgs{m->g, m->pd, p2g_max_blocks(m->n), m->pmax_out});
-    add_p2g_max(P, PH_P2G, mx);
+    for (size_t i = 0; i < ms.size(); ++i)
+        if (!sel[i].max_from_push) mx.push_back(P2GMaxArgs{ms[i]->g, ms[i]->pd, p2g_max_blocks(ms[i]->n), ms[i]->pmax_out});
+    if (!mx.empty()) add_p2g_max(P, PH_P2G, mx);
     add_exclusive_scan(P, PH_SORT, sc);
     add_place(P, PH_SORT, pl);
     add_p2g(P, PH_P2G, ti, fi);
@@ -556,7 +558,7 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
             float *const *a = last ? m->ax_stale : m->ax_cur;
             ax.push_back(a);
             const int rd = (s - 1) % 3, wr = last ? 0 : (s % 3);
-            sel.push_back(P2GSel{a, m->ax_stale, s == 1 ? it->reuse : nullptr, rd, wr});
+            sel.push_back(P2GSel{a, m->ax_stale, s == 1 ? it->reuse : nullptr, rd, wr, s > 1});
         }
         if (s == 1) build_axes(P, ms, ax);   // later stages: stored by the previous push
         build_p2g(P, ms, sel);
@@ -581,8 +583,10 @@ void build_step(Plan &P, const std::vector<StepItem> &items)
             a.g = m->g;
             a.p = m->pd;
             for (int c = 0; c < 6; ++c) a.axes[c] = ax[i][c];
-            if (!last)
+            if (!last) {
                 for (int c = 0; c < 6; ++c) a.axes_next[c] = (s + 1 == ns) ? m->ax_stale[c] : m->ax_cur[c];
+                a.tmax_next = m->pmax_out;   // cleared by this stage's finalize
+            }
             a.perm = m->perm;
             a.cell_start = m->cell_start;
             a.fld = m->fld;

@@ -103,6 +103,24 @@ struct FftPlan {
 template <class T> __device__ inline EPIC_GLOBAL T *gl(T *p) { return (EPIC_GLOBAL T *)p; }
 template <class T> __device__ inline const EPIC_GLOBAL T *gl(const T *p) { return (const EPIC_GLOBAL T *)p; }
 
+// The quantities par2grid accumulates for one parcel, in P2G_NF slots:
+// vol/4 and vol/4 times vorticity, total buoyancy (with condensation),
+// buoyancy and humidity.  A parcel without humidity has no condensate (the
+// saturation humidity is positive), so its exp is skipped.
+__device__ inline void p2g_values_of(const Grid &g, float vol, const float vor[3], float buoy, float hum, double z,
+                                     float v[P2G_NF])
+{
+    const double qc = (hum > 0.0f) ? fmax(0.0, (double)hum - g.q0 * exp(g.lambda_c * (g.lower[2] - z))) : 0.0;
+    const float v0 = (float)(0.25 * (double)vol);
+    v[0] = v0;
+    v[1] = v0 * vor[0];
+    v[2] = v0 * vor[1];
+    v[3] = v0 * vor[2];
+    v[4] = v0 * (float)((double)buoy + g.glat * qc);
+    v[5] = v0 * buoy;
+    v[6] = v0 * hum;
+}
+
 __host__ __device__ inline int64_t field_stride(const Grid &g) { return (int64_t)(g.nz + 3) * g.ny * g.nx; }
 __host__ __device__ inline int64_t plane_stride(const Grid &g) { return (int64_t)g.ny * g.nx; }
 __host__ __device__ inline int64_t spec_size(const Grid &g) { return (int64_t)g.ny * g.nxh * g.nzp; }
@@ -110,6 +128,7 @@ __host__ __device__ inline int64_t ncells(const Grid &g) { return (int64_t)g.nx 
 
 __device__ inline int wrap(int i, int n)
 {
+    if ((unsigned)i < (unsigned)n) return i;   // nearly always: skips the integer division
     int r = i % n;
     return r < 0 ? r + n : r;
 }

@@ -155,23 +155,38 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
     const int NB = nbox * NN;                       // nodes in the box
     float2 *box2 = (float2 *)(box + 3 * NB);
     __shared__ int row_slot[64], row_off[65];       // the chunk's rows of 8 cells
+    __shared__ unsigned int s_pmax[P2G_NF + 1];     // the next par2grid's maxima (float bits)
     int e = 0;
+    if (tid <= P2G_NF) s_pmax[tid] = 0u;   // ordered by the barrier after the box is staged
 
     // ---- stage the node box (plane P = k0 - 1 + lp) ----
+    // thread = (node of the 11 x 11 box column, half of the 14 fields): the
+    // wrapped column offset is found once, then the thread walks the planes
+    // with its 7 loads in flight at a time
+    static_assert(NN <= 128 && G2P_THREADS == 256, "two threads per box node column");
     {
-        for (int u = tid; u < 14 * NN * nbox; u += G2P_THREADS) {
-            const int lp = u / (14 * NN), rem = u - lp * (14 * NN);
-            const int f = rem / NN, node = rem - f * NN;
+        const int node = tid & 127, half = tid >> 7;
+        if (node < NN) {
             const int ly = node / P2G_BOX, lx = node - ly * P2G_BOX;
-            const int P = k0 - 1 + lp;
-            float v = 0.0f;
-            if (P >= -1 && P <= g.nz + 1) {
-                const int gi = wrap(i0 - 1 + lx, g.nx), gj = wrap(j0 - 1 + ly, g.ny);
-                v = gl(a.fld)[(F_VELOG + f) * FS + (int64_t)(P + 1) * PS + (int64_t)gj * g.nx + gi];
+            int gi = i0 - 1 + lx, gj = j0 - 1 + ly;   // in [-1, n + 9]
+            gi = (gi < 0) ? gi + g.nx : (gi >= g.nx ? gi - g.nx : gi);
+            gj = (gj < 0) ? gj + g.ny : (gj >= g.ny ? gj - g.ny : gj);
+            const EPIC_GLOBAL float *col = gl(a.fld) + (int64_t)(F_VELOG + half) * FS + (int64_t)gj * g.nx + gi;
+            for (int lp = 0; lp < nbox; ++lp) {
+                const int P = k0 - 1 + lp;
+                const bool in = (P >= -1 && P <= g.nz + 1);
+                const int64_t po = (int64_t)(P + 1) * PS;
+                float v[7];
+#pragma unroll
+                for (int t = 0; t < 7; ++t) v[t] = in ? col[(int64_t)(2 * t) * FS + po] : 0.0f;
+                const int nd = lp * NN + node;
+#pragma unroll
+                for (int t = 0; t < 7; ++t) {
+                    const int f = 2 * t + half;
+                    if (f < 12) ((float *)box)[((f >> 2) * NB + nd) * 4 + (f & 3)] = v[t];
+                    else ((float *)box2)[nd * 2 + (f - 12)] = v[t];
+                }
             }
-            const int nd = lp * NN + node;
-            if (f < 12) ((float *)box)[((f >> 2) * NB + nd) * 4 + (f & 3)] = v;
-            else ((float *)box2)[nd * 2 + (f - 12)] = v;
         }
     }
     const int nrow = nl * P2G_TY;
@@ -358,11 +373,33 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
                     gl(a.axes_next[3 + d])[pi] = (float)vt2[d];
                 }
             }
+            if (a.tmax_next) {
+                // bounds of the next par2grid's |contributions| of this
+                // parcel, from what it stored: the condensate is at most the
+                // humidity, so |tbuoy| <= |buoy| + glat hum and |tbuoy - buoy|
+                // <= glat hum (exact when there is no humidity); a scale from
+                // a bound is at most one power of two coarser, and the
+                // fixed-point sums keep a factor 2 of headroom for rounding
+                const float v0 = (float)(0.25 * vol), h = gl(a.p.hum)[pi];
+                const float hb = fabsf(v0 * (float)g.glat * h);
+                const float b5 = fabsf(v0 * gl(a.p.buoy)[pi]);
+                const float m[P2G_NF + 1] = {fabsf(v0), fabsf(v0 * gl(a.p.vor[0])[pi]), fabsf(v0 * gl(a.p.vor[1])[pi]),
+                                             fabsf(v0 * gl(a.p.vor[2])[pi]), b5 + hb, b5, fabsf(v0 * h), hb};
+                // into the work-group's maxima (bits of non-negative floats);
+                // a lane below the running maximum skips the atomic
+#pragma unroll
+                for (int f = 0; f <= P2G_NF; ++f)
+                    if (m[f] > __uint_as_float(s_pmax[f])) atomicMax(&s_pmax[f], __float_as_uint(m[f]));
+            }
         };
         process();
         // next stage's cell histogram, one atomic per run of equal keys
         wave_run_add(key, gl(a.count));
         wave_run_add(small ? key : -1, gl(a.scount));
+    }
+    if (a.tmax_next) {
+        __syncthreads();
+        if (tid <= P2G_NF && s_pmax[tid]) atomicMax((unsigned int *)a.tmax_next + tid, s_pmax[tid]);
     }
     if (e) atomicOr(a.err, e);
 }

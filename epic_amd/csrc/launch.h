@@ -100,6 +100,7 @@ struct G2PTileArgs {
     ParcelsDev p;
     const float *axes[6];
     float *axes_next[6];         // null, or where the half axes of the pushed B go (the next stage's)
+    float *tmax_next;            // null, or the next par2grid's maxima (atomicMax; zero on entry)
     const int *perm;
     const int *cell_start;
     const float *fld;

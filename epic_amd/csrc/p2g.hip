@@ -267,22 +267,11 @@ __device__ inline float fixed_scale(float max_abs, int &e)
     return ldexpf(1.0f, 28 - e);
 }
 
-// the gridded quantities of a parcel: vol/4 and vol/4 times vorticity,
-// total buoyancy (with condensation), buoyancy and humidity (a parcel
-// without humidity has no condensate: the saturation humidity is positive,
-// so its exp is skipped)
+// the gridded quantities of parcel pi, read from its arrays
 __device__ inline void p2g_values(const Grid &g, const ParcelsDev &p, int pi, double z, float v[P2G_NF])
 {
-    const float h = gl(p.hum)[pi];
-    const double qc = (h > 0.0f) ? fmax(0.0, (double)h - g.q0 * exp(g.lambda_c * (g.lower[2] - z))) : 0.0;
-    const float v0 = (float)(0.25 * (double)gl(p.vol)[pi]);
-    v[0] = v0;
-    v[1] = v0 * gl(p.vor[0])[pi];
-    v[2] = v0 * gl(p.vor[1])[pi];
-    v[3] = v0 * gl(p.vor[2])[pi];
-    v[4] = v0 * (float)((double)gl(p.buoy)[pi] + g.glat * qc);
-    v[5] = v0 * gl(p.buoy)[pi];
-    v[6] = v0 * h;
+    const float vor[3] = {gl(p.vor[0])[pi], gl(p.vor[1])[pi], gl(p.vor[2])[pi]};
+    p2g_values_of(g, gl(p.vol)[pi], vor, gl(p.buoy)[pi], gl(p.hum)[pi], z, v);
 }
 
 // The largest |contribution| of every field over a member's parcels, and
@@ -292,7 +281,10 @@ __device__ inline void p2g_values(const Grid &g, const ParcelsDev &p, int pi, do
 // out[] is zero when this runs: it starts zeroed, and the finalize kernel
 // of the same par2grid clears it once the tile kernel has taken its scales
 // from it (a value left by an interrupted sequence would only be larger, a
-// coarser but still valid scale).
+// coarser but still valid scale).  Inside an RK step only the first stage
+// runs this pass: each later stage's maxima come from the previous push,
+// which has the parcels' new values at hand and bounds the moist
+// contributions by the humidity (exact without humidity).
 __global__ __launch_bounds__(256) void k_p2g_max(const P2GMaxArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
