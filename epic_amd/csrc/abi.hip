@@ -740,7 +740,7 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     if (flags->rk_order != 3 && flags->rk_order != 4) return EPIC_E_RK_ORDER;
     for (int i = 0; i < 3; ++i)
         if (!(grid->extent[i] > 0.0)) return EPIC_E_INVALID;
-    if (max_parcels > (int64_t)0x7fffffff) return EPIC_E_INVALID;
+    if (max_parcels > EPIC_MAX_PARCELS) return EPIC_E_INVALID;   // 32-bit byte offsets of the 4-byte arrays
 
     epic_member *m = new epic_member();
     Grid &g = m->g;

@@ -121,6 +121,18 @@ __device__ inline void p2g_values_of(const Grid &g, float vol, const float vor[3
     v[6] = v0 * hum;
 }
 
+// element at a byte offset below 4 GiB from a table pointer: the load or
+// store takes the pointer as its scalar base and the offset as a 32-bit
+// vector operand, with no 64-bit address arithmetic per access
+template <class T> __device__ inline EPIC_GLOBAL T &at(T *p, unsigned off)
+{
+    return *(EPIC_GLOBAL T *)((EPIC_GLOBAL char *)p + off);
+}
+template <class T> __device__ inline const EPIC_GLOBAL T &at(const T *p, unsigned off)
+{
+    return *(const EPIC_GLOBAL T *)((const EPIC_GLOBAL char *)p + off);
+}
+
 __host__ __device__ inline int64_t field_stride(const Grid &g) { return (int64_t)(g.nz + 3) * g.ny * g.nx; }
 __host__ __device__ inline int64_t plane_stride(const Grid &g) { return (int64_t)g.ny * g.nx; }
 __host__ __device__ inline int64_t spec_size(const Grid &g) { return (int64_t)g.ny * g.nxh * g.nzp; }

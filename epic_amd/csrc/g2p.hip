@@ -225,6 +225,7 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
     for (int base = 0; base < total; base += G2P_THREADS) {   // uniform rounds: the ballot needs every lane
         const int q = base + tid;
         const int r = r_nx, pi = pi_nx;
+        const unsigned o4 = (unsigned)pi * 4u;   // byte offset (pi < 2^30): scalar base + 32-bit offset addressing
         if (q + G2P_THREADS < total) {
             r_nx = row_of(q + G2P_THREADS);
             pi_nx = gl(a.perm)[row_slot[r_nx] + (q + G2P_THREADS - row_off[r_nx])];
@@ -237,10 +238,10 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             double pos[3] = {gl(a.p.x)[pi], gl(a.p.y)[pi], gl(a.p.z)[pi]};
             double B[5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) B[c] = (double)gl(a.p.B[c])[pi];
-            const double vol = (double)gl(a.p.vol)[pi];
-            const double ve[3] = {gl(a.axes[0])[pi], gl(a.axes[1])[pi], gl(a.axes[2])[pi]};
-            const double vt[3] = {gl(a.axes[3])[pi], gl(a.axes[4])[pi], gl(a.axes[5])[pi]};
+            for (int c = 0; c < 5; ++c) B[c] = (double)at(a.p.B[c], o4);
+            const double vol = (double)at(a.p.vol, o4);
+            const double ve[3] = {at(a.axes[0], o4), at(a.axes[1], o4), at(a.axes[2], o4)};
+            const double vt[3] = {at(a.axes[3], o4), at(a.axes[4], o4), at(a.axes[5], o4)};
             int cc[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
@@ -253,8 +254,8 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             float dp[3], dv[3], st[8];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                dp[c] = acc_prev ? gl(a.p.dpos[c])[pi] : 0.0f;
-                dv[c] = acc_prev ? gl(a.p.dvor[c])[pi] : 0.0f;
+                dp[c] = acc_prev ? at(a.p.dpos[c], o4) : 0.0f;
+                dv[c] = acc_prev ? at(a.p.dvor[c], o4) : 0.0f;
             }
 #pragma unroll
             for (int c = 0; c < 8; ++c) st[c] = 0.0f;
@@ -314,13 +315,13 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             }
             float db[5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) db[c] = acc_prev ? gl(a.p.dB[c])[pi] + (float)dB[c] : (float)dB[c];
+            for (int c = 0; c < 5; ++c) db[c] = acc_prev ? at(a.p.dB[c], o4) + (float)dB[c] : (float)dB[c];
             // push (ls_rk.f90:183-191), periodic wrap (parcel_bc.f90:17-21)
             const double cbdt = a.cb * (*a.dt);
 #pragma unroll
             for (int c = 0; c < 3; ++c) pos[c] = pos[c] + cbdt * (double)dp[c];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) gl(a.p.vor[c])[pi] = (float)((double)gl(a.p.vor[c])[pi] + cbdt * (double)dv[c]);
+            for (int c = 0; c < 3; ++c) at(a.p.vor[c], o4) = (float)((double)at(a.p.vor[c], o4) + cbdt * (double)dv[c]);
 #pragma unroll
             for (int c = 0; c < 5; ++c) B[c] = B[c] + cbdt * (double)db[c];
             pos[0] = pos[0] - g.extent[0] * trunc((pos[0] - g.center[0]) * g.hli[0]);
@@ -340,16 +341,16 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             gl(a.p.y)[pi] = pos[1];
             gl(a.p.z)[pi] = pos[2];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) gl(a.p.B[c])[pi] = (float)B[c];
+            for (int c = 0; c < 5; ++c) at(a.p.B[c], o4) = (float)B[c];
             if (a.stage < a.nstages) {
                 const float ca = (float)a.ca;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    gl(a.p.dpos[c])[pi] = ca * dp[c];
-                    gl(a.p.dvor[c])[pi] = ca * dv[c];
+                    at(a.p.dpos[c], o4) = ca * dp[c];
+                    at(a.p.dvor[c], o4) = ca * dv[c];
                 }
 #pragma unroll
-                for (int c = 0; c < 5; ++c) gl(a.p.dB[c])[pi] = ca * db[c];
+                for (int c = 0; c < 5; ++c) at(a.p.dB[c], o4) = ca * db[c];
             }
             // cell of the new position for the next stage's sort
             const int ni = (int)floor((pos[0] - g.lower[0]) * g.dxi[0]);
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             const int nk = (int)floor((pos[2] - g.lower[2]) * g.dxi[2]);
             if (nk < -1 || nk > g.nz) e |= ERR_PARCEL_COUNT;
             else key = ((nk + 1) * g.ny + wrap(nj, g.ny)) * g.nx + wrap(ni, g.nx);
-            gl(a.keys)[pi] = key;
+            at(a.keys, o4) = key;
             small = (vol <= g.vmin);
             if (a.axes_next[0]) {   // the next stage's support axes, from B as stored
                 double Bs[5], ve2[3], vt2[3];
@@ -369,8 +370,8 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
                 }
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    gl(a.axes_next[d])[pi] = (float)ve2[d];
-                    gl(a.axes_next[3 + d])[pi] = (float)vt2[d];
+                    at(a.axes_next[d], o4) = (float)ve2[d];
+                    at(a.axes_next[3 + d], o4) = (float)vt2[d];
                 }
             }
             if (a.tmax_next) {
@@ -380,11 +381,11 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
                 // <= glat hum (exact when there is no humidity); a scale from
                 // a bound is at most one power of two coarser, and the
                 // fixed-point sums keep a factor 2 of headroom for rounding
-                const float v0 = (float)(0.25 * vol), h = gl(a.p.hum)[pi];
+                const float v0 = (float)(0.25 * vol), h = at(a.p.hum, o4);
                 const float hb = fabsf(v0 * (float)g.glat * h);
-                const float b5 = fabsf(v0 * gl(a.p.buoy)[pi]);
-                const float m[P2G_NF + 1] = {fabsf(v0), fabsf(v0 * gl(a.p.vor[0])[pi]), fabsf(v0 * gl(a.p.vor[1])[pi]),
-                                             fabsf(v0 * gl(a.p.vor[2])[pi]), b5 + hb, b5, fabsf(v0 * h), hb};
+                const float b5 = fabsf(v0 * at(a.p.buoy, o4));
+                const float m[P2G_NF + 1] = {fabsf(v0), fabsf(v0 * at(a.p.vor[0], o4)), fabsf(v0 * at(a.p.vor[1], o4)),
+                                             fabsf(v0 * at(a.p.vor[2], o4)), b5 + hb, b5, fabsf(v0 * h), hb};
                 // into the work-group's maxima (bits of non-negative floats);
                 // a lane below the running maximum skips the atomic
 #pragma unroll

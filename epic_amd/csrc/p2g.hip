@@ -267,11 +267,11 @@ __device__ inline float fixed_scale(float max_abs, int &e)
     return ldexpf(1.0f, 28 - e);
 }
 
-// the gridded quantities of parcel pi, read from its arrays
-__device__ inline void p2g_values(const Grid &g, const ParcelsDev &p, int pi, double z, float v[P2G_NF])
+// the gridded quantities of parcel pi, read from its arrays (o4 = 4 pi)
+__device__ inline void p2g_values(const Grid &g, const ParcelsDev &p, unsigned o4, double z, float v[P2G_NF])
 {
-    const float vor[3] = {gl(p.vor[0])[pi], gl(p.vor[1])[pi], gl(p.vor[2])[pi]};
-    p2g_values_of(g, gl(p.vol)[pi], vor, gl(p.buoy)[pi], gl(p.hum)[pi], z, v);
+    const float vor[3] = {at(p.vor[0], o4), at(p.vor[1], o4), at(p.vor[2], o4)};
+    p2g_values_of(g, at(p.vol, o4), vor, at(p.buoy, o4), at(p.hum, o4), z, v);
 }
 
 // The largest |contribution| of every field over a member's parcels, and
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_p2g_max(const P2GMaxArgs *__restrict__ 
     for (int f = 0; f <= P2G_NF; ++f) mx[f] = 0.0f;
     for (int64_t q = (int64_t)ib.blk * 256 + threadIdx.x; q < a.p.n; q += (int64_t)a.nblk * 256) {
         float v[P2G_NF];
-        p2g_values(a.g, a.p, (int)q, gl(a.p.z)[q], v);
+        p2g_values(a.g, a.p, (unsigned)q * 4u, gl(a.p.z)[q], v);
 #pragma unroll
         for (int f = 0; f < P2G_NF; ++f) mx[f] = fmaxf(mx[f], fabsf(v[f]));
         mx[P2G_NF] = fmaxf(mx[P2G_NF], fabsf(v[4] - v[5]));
@@ -369,11 +369,12 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     int err = 0;
     for (int s = s0 + sub; s < s1; s += P2G_TPC) {
         const int pi = gl(a.perm)[s];
+        const unsigned o4 = (unsigned)pi * 4u;   // byte offset (pi < 2^30): scalar base + 32-bit offset addressing
         const double pos[3] = {gl(a.p.x)[pi], gl(a.p.y)[pi], gl(a.p.z)[pi]};
-        const double ve[3] = {gl(axes[0])[pi], gl(axes[1])[pi], gl(axes[2])[pi]};
-        const double vt[3] = {gl(axes[3])[pi], gl(axes[4])[pi], gl(axes[5])[pi]};
+        const double ve[3] = {at(axes[0], o4), at(axes[1], o4), at(axes[2], o4)};
+        const double vt[3] = {at(axes[3], o4), at(axes[4], o4), at(axes[5], o4)};
         float v[P2G_NF];
-        p2g_values(g, a.p, pi, pos[2], v);
+        p2g_values(g, a.p, o4, pos[2], v);
         float cs[P2G_NF];
 #pragma unroll
         for (int f = 0; f < P2G_NF; ++f) cs[f] = v[f] * sc[f];   // exact: sc is a power of two

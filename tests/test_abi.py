@@ -36,6 +36,8 @@ def test_create_rejects_bad_grid_without_touching_the_gpu():
     g = core.Grid(16, 16, 16, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
     f = core.Flags((C.c_int * 2)(0, 0), 2, 0.0)   # unsupported RK order
     assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 10, C.byref(h)) == -6
+    f = core.Flags((C.c_int * 2)(0, 0), 4, 0.0)   # beyond EPIC_MAX_PARCELS = 2^30 - 1
+    assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 1 << 30, C.byref(h)) == -1
 
 
 def test_fortran_binding_names_declared():
