@@ -62,7 +62,6 @@ struct P2GArgs {
     int *err;
     const float *tmax;          // P2G_NF + 1 maxima from k_p2g_max (fixed-point scales)
     int tz, ntx, nty, nchz;
-    int lpitch;                 // LDS row pitch of the node box (set by add_p2g)
 };
 // largest |contribution| per field (and |tbuoy - buoy|) over a member's parcels
 constexpr int P2G_MAX_BLOCKS = 4096;

@@ -252,7 +252,10 @@ __global__ void k_cell_order(int64_t nc, const int *__restrict__ cell_start, con
 // [k0 - 1, k0 + tz + 1]; rows padded to lpitch 64-bit words (12 spreads the
 // 8 x 8 lanes of a wave evenly over the 32 LDS banks).
 
-__host__ __device__ inline int box_planes(int tz) { return tz + 3; }
+__host__ __device__ constexpr int box_planes(int tz) { return tz + 3; }
+// LDS row pitch in 64-bit words; at tz = 8 the unpadded 11 keeps two
+// work-groups per CU within the LDS
+__host__ __device__ constexpr int p2g_lpitch(int tz) { return tz >= 8 ? P2G_BOX : 12; }
 
 // power-of-two scale s with max_abs * s < 2^28 (0 when max_abs is 0): the
 // summed weights of up to four points of one parcel stay below 4, so every
@@ -313,6 +316,9 @@ __global__ __launch_bounds__(256) void k_p2g_max(const P2GMaxArgs *__restrict__ 
     }
 }
 
+// TZ (= a.tz) is a template parameter, so the LDS strides are constants and
+// the 8 x 7 atomics of a cell are its base word plus immediate offsets
+template <int TZ>
 __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
@@ -320,16 +326,16 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     const Grid &g = a.g;
     // stage 1 of a step after the first takes the cached axes (l_reuse)
     const float *const *axes = (a.use_alt && *a.use_alt) ? a.axes_alt : a.axes;
-    const int tz = a.tz, lp = a.lpitch;
+    constexpr int tz = TZ, lp = p2g_lpitch(TZ);
     const int tx = ib.blk % a.ntx;
     const int ty = (ib.blk / a.ntx) % a.nty;
     const int cz = ib.blk / (a.ntx * a.nty);
     const int i0 = tx * P2G_TX, j0 = ty * P2G_TY;
     const int k0 = -1 + cz * tz;
     const int tid = threadIdx.x;
-    const int nbox = box_planes(tz);
-    const int lplane = P2G_BOX * lp;              // words per LDS node plane
-    const int lfield = nbox * lplane;             // words per field
+    constexpr int nbox = box_planes(tz);
+    constexpr int lplane = P2G_BOX * lp;          // words per LDS node plane
+    constexpr int lfield = nbox * lplane;         // words per field
     extern __shared__ unsigned long long acc[];   // [field][plane][y][x]
     __shared__ int s_err;
 
@@ -671,13 +677,14 @@ P2GLayout p2g_layout(const Grid &g)
     return L;
 }
 
-int p2g_lpitch(int tz) { return tz >= 8 ? P2G_BOX : 12; }   // 8: two work-groups per CU still fit
 
 size_t p2g_lds_bytes(int tz) { return sizeof(unsigned long long) * P2G_NF * box_planes(tz) * P2G_BOX * p2g_lpitch(tz); }
 
 void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin)
 {
-    // one grouped launch per z chunk (the work-group size is 64 tz)
+    // one grouped launch per z chunk (the work-group size is 64 tz P2G_TPC),
+    // the kernel specialised on it (epic_set_tiling and the autotuner admit
+    // 2, 4 and 8 only)
     std::vector<int> tzs;
     for (const P2GArgs &a : tiles)
         if (std::find(tzs.begin(), tzs.end(), a.tz) == tzs.end()) tzs.push_back(a.tz);
@@ -686,11 +693,11 @@ void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::v
         std::vector<int> nb;
         for (P2GArgs a : tiles) {
             if (a.tz != tz) continue;
-            a.lpitch = p2g_lpitch(tz);
             items.push_back(a);
             nb.push_back(a.ntx * a.nty * a.nchz);
         }
-        P.add((const void *)k_p2g_tiles, 64 * tz * P2G_TPC, p2g_lds_bytes(tz), phase, items, nb);
+        const void *k = tz == 2 ? (const void *)k_p2g_tiles<2> : tz == 4 ? (const void *)k_p2g_tiles<4> : (const void *)k_p2g_tiles<8>;
+        P.add(k, 64 * tz * P2G_TPC, p2g_lds_bytes(tz), phase, items, nb);
     }
     std::vector<int> nf;
     for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny * (f.g.nz + 3) + 255) / 256));
