@@ -48,10 +48,10 @@ BYTES_PUSH = 204
 # HBM traffic of k_g2p_tiles per launch on the 128^3 rising thermal, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by what the same
 # counter reads for a 1 GiB device-to-device copy (FETCH x 2.0, WRITE x 1.0):
-# 6.335e9 B per launch over 16,777,216 parcels, averaged over the five stage
+# 6.336e9 B per launch over 16,777,216 parcels, averaged over the five stage
 # launches (stages 1-4 also store the next stage's half axes and read
 # buoyancy and humidity for the next par2grid's fixed-point maxima).
-TRAFFIC_BYTES_PER_PARCEL = 377.6
+TRAFFIC_BYTES_PER_PARCEL = 377.7
 
 
 def log(*a):

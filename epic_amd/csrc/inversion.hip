@@ -26,7 +26,7 @@ struct ColCtx {
     int nz;
     double2 *fa, *fb;
     const FftPlan *plan;
-    const double2 *tw;   // LDS: e^{-2 pi i t / (2 nz)}, t < nz
+    const double2 *tw;   // e^{-2 pi i t / (2 nz)}, t < nz (the member's table; cached reads)
 };
 
 // DST-I of col[1..nz-1] in place (col[0], col[nz] untouched)
@@ -96,12 +96,9 @@ __global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restric
     extern __shared__ double2 sm[];
     double2 *SX = sm, *SY = SX + nz1, *C = SY + nz1, *E = C + nz1, *DS = E + nz1, *AS = DS + nz1;
     double2 *fa = AS + nz1, *fb = fa + 2 * nz;
-    double2 *twh = fb + 2 * nz;
-    double *phim = (double *)(twh + nz), *phip = phim + nz1;
+    double *phim = (double *)(fb + 2 * nz), *phip = phim + nz1;
     __shared__ double2 s_ds0, s_dsn, s_y0, s_yn, s_x0, s_xn;
-    for (int t = threadIdx.x; t < nz; t += blockDim.x) twh[t] = a.ztw[t];
-
-    ColCtx cc{nz, fa, fb, &a.zplan, twh};
+    ColCtx cc{nz, fa, fb, &a.zplan, a.ztw};   // twiddles from global memory (cached)
 
     // wavenumbers (sta3dfft.f90:88-107, sta2dfft.f90:59-64): r = (pi/L) 2k
     const int kys = (2 * ky <= g.ny) ? ky : ky - g.ny;
@@ -389,7 +386,7 @@ __global__ __launch_bounds__(256) void k_vtend_fixup(const FixArgs *__restrict__
 size_t col_lds_bytes(const Grid &g)
 {
     const int nz1 = g.nz + 1;
-    return (size_t)(6 * nz1 + 5 * g.nz) * sizeof(double2) + 2 * (size_t)nz1 * sizeof(double);
+    return (size_t)(6 * nz1 + 4 * g.nz) * sizeof(double2) + 2 * (size_t)nz1 * sizeof(double);
 }
 
 void add_vor2vel_columns(Plan &P, int phase, const std::vector<ColArgs> &items)
