@@ -169,7 +169,10 @@ def test_grid2par_constant_fields():
     assert np.abs(dv - [10.0, 20.0, 30.0]).max() <= 4e-7 * 30
 
 
-@pytest.mark.parametrize("name,nsteps", [("beltrami16", 2), ("thermal16", 2), ("moist16", 1), ("thermal32x16x24", 1)])
+# thermal16 over 4 steps includes a physical re-sort (every 3 steps) and the
+# fixed-point maxima stored by the pushes; moist16 over 3 steps, their
+# humidity bounds
+@pytest.mark.parametrize("name,nsteps", [("beltrami16", 2), ("thermal16", 4), ("moist16", 3), ("thermal32x16x24", 1)])
 def test_ls_rk_step(name, nsteps):
     mem = MEMBERS[name]
     st = oracle_state(mem)
@@ -186,6 +189,7 @@ def test_ls_rk_step(name, nsteps):
             e = np.abs(got[f] - ref).max() / max(mem.dx)   # positions: relative to the cell size
         else:
             e = maxrel(got[f], ref)
+        print(f"  {name} x{nsteps} {f}: {e:.2e}", flush=True)
         assert e <= TOL, (name, f, e)
     nx, ny = mem.nx, mem.ny
     for f in ("velog", "vtend", "volg", "vortg"):
