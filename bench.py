@@ -181,6 +181,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--moist", action="store_true",
+                    help="the large moist member's statistics (humidity plume, 32 buoyancy blobs) instead of the "
+                         "rising thermal; with --size 256: 134M parcels")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--forms", default="plain,batched,served")
@@ -199,7 +202,7 @@ def main():
     from epic_amd.core import Member
 
     t0 = time.perf_counter()
-    mem = W.rising_thermal(args.size)
+    mem = W.moist_member(args.size) if args.moist else W.rising_thermal(args.size)
     log(f"bench: generated {mem.name} with {mem.n} parcels in {time.perf_counter() - t0:.1f} s")
     gm = Member.from_workload(mem)
     dt = 0.01
@@ -227,7 +230,9 @@ def main():
     phases = {k: round(v[0] / max(1, args.profile_steps), 4) for k, v in tm.items() if v[1]}
     push_avg = push_ms / max(1, push_calls)
     achieved = BYTES_PUSH * mem.n / (push_avg * 1e-3) / 1e9 if push_avg > 0 else 0.0
-    traffic = None if TRAFFIC_BYTES_PER_PARCEL is None else TRAFFIC_BYTES_PER_PARCEL * mem.n
+    # the counter figure belongs to the 128^3 rising thermal it was measured on
+    measured = args.size == 128 and not args.moist
+    traffic = TRAFFIC_BYTES_PER_PARCEL * mem.n if measured and TRAFFIC_BYTES_PER_PARCEL is not None else None
     roof = {"kernel": "k_g2p_tiles (fused grid2par + dB/dt + RK push)", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "avg_launch_ms": round(push_avg, 4), "bytes_per_parcel": BYTES_PUSH,
@@ -250,7 +255,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"rising thermal {mem.nx}x{mem.ny}x{mem.nz}, 8 parcels/cell ({mem.n} parcels), "
+        "config": {"workload": f"{'moist member' if args.moist else 'rising thermal'} {mem.nx}x{mem.ny}x{mem.nz}, "
+                               f"8 parcels/cell ({mem.n} parcels), "
                                f"plain LS-RK4 step, dt=0.01; FP32 storage, FP64 positions/eigen/z-transforms",
                    "grid": [mem.nx, mem.ny, mem.nz], "parcels": mem.n, "form": "plain"},
         "phases_ms_per_step": phases,
