@@ -202,7 +202,8 @@ def main():
     from epic_amd.core import Member
 
     t0 = time.perf_counter()
-    mem = W.moist_member(args.size) if args.moist else W.rising_thermal(args.size)
+    gen_log = (lambda m: log(f"bench: generating: {m}")) if args.size > 128 else None
+    mem = W.moist_member(args.size, log=gen_log) if args.moist else W.rising_thermal(args.size, log=gen_log)
     log(f"bench: generated {mem.name} with {mem.n} parcels in {time.perf_counter() - t0:.1f} s")
     gm = Member.from_workload(mem)
     dt = 0.01

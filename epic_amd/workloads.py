@@ -204,9 +204,11 @@ def beltrami(n=32, k=1, l=1, m=1, dtype_round=True):
 
 
 def rising_thermal(n=128, seed=DEFAULT_SEED, ny=None, nz=None, humidity_plume=False, n_blobs=0,
-                   dtype_round=True):
+                   dtype_round=True, log=None):
     """Rising-thermal member on an n x ny x nz grid over [-pi/2, pi/2]^3-like
-    box (cell-isotropic extent pi * (n, ny, nz)/n), 8 parcels per cell."""
+    box (cell-isotropic extent pi * (n, ny, nz)/n), 8 parcels per cell.
+    log(str), if given, is called after each stage of the generation."""
+    say = log or (lambda s: None)
     ny = n if ny is None else ny
     nz = n if nz is None else nz
     rng = np.random.default_rng(seed)
@@ -215,20 +217,27 @@ def rising_thermal(n=128, seed=DEFAULT_SEED, ny=None, nz=None, humidity_plume=Fa
     lower = tuple(-0.5 * e for e in extent)
     pos, dx = _lattice_member(n, ny, nz, lower, extent, 2, rng, 0.25)
     N = pos.shape[0]
+    say(f"lattice: {N} parcels")
     vcell = dx[0] * dx[1] * dx[2]
     vol = np.full(N, vcell / 8.0)
     B = triaxial_B(rng, vol)
+    say("shape matrices")
     grf = [gaussian_random_field(rng, n, ny, nz) for _ in range(3)]
-    vor = np.stack([trilinear_at(f, pos, lower, dx, n, ny) for f in grf], axis=1)
+    vor = np.empty((N, 3))
+    for c, f in enumerate(grf):
+        vor[:, c] = trilinear_at(f, pos, lower, dx, n, ny)
+        say(f"vorticity component {c}")
     L = np.asarray(extent)
     rel = (pos - np.asarray(lower)) / L
     if n_blobs:
         buoy = np.zeros(N)
         amps = rng.normal(0.0, 0.5, n_blobs)
         centres = rng.uniform(0.0, 1.0, (n_blobs, 3))
-        for a, c0 in zip(amps, centres):
+        for b, (a, c0) in enumerate(zip(amps, centres)):
             r2 = np.sum(((rel - c0) * L) ** 2, axis=1)
             buoy += a * np.exp(-r2 / (0.1 * L[0]) ** 2)
+            if b % 8 == 7:
+                say(f"buoyancy blob {b + 1} of {n_blobs}")
     else:
         r2 = np.sum(((rel - np.array([0.5, 0.5, 0.25])) * L) ** 2, axis=1)
         buoy = np.exp(-r2 / (0.2 * L[0]) ** 2)
@@ -243,8 +252,8 @@ def rising_thermal(n=128, seed=DEFAULT_SEED, ny=None, nz=None, humidity_plume=Fa
                   name=f"rising_thermal{n}x{ny}x{nz}")
 
 
-def moist_member(n=256, seed=DEFAULT_SEED):
-    return rising_thermal(n, seed=seed, humidity_plume=True, n_blobs=32)
+def moist_member(n=256, seed=DEFAULT_SEED, log=None):
+    return rising_thermal(n, seed=seed, humidity_plume=True, n_blobs=32, log=log)
 
 
 def ensemble_member(m, seed0=DEFAULT_SEED, sizes_xy=(32, 48, 64, 96), sizes_z=(32, 64)):
