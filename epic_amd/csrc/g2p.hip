@@ -9,7 +9,9 @@
 // gradient gives dB/dt (B33 from det B in FP64), the stage pushes position
 // (FP64), vorticity and B by cb dt delta, applies the periodic wrap (int()
 // truncation), scales the deltas by ca for the next stage and computes the
-// next stage's cell key and cell histogram.
+// next stage's cell key and cell histogram; on every stage but the last it
+// also stores the half axes of the B it wrote (FP64 eigen-solve) and the
+// next par2grid's fixed-point maxima.
 #include "common.h"
 #include "ellipsoid.h"
 #include "launch.h"
@@ -129,7 +131,9 @@ __global__ __launch_bounds__(256) void k_grid2par(const G2PArgs *__restrict__ ta
 // and neighbouring nodes sit in neighbouring banks; then every thread takes parcels of the
 // chunk through the cell sort (the rows of 8 cells are contiguous slot
 // ranges) with no barrier between parcels.  Support points come from the
-// half axes cached for this stage.
+// half axes cached for this stage.  The permutation entry of a thread's next
+// parcel is loaded one round ahead, and parcel arrays are addressed as a
+// table pointer plus a 32-bit byte offset.
 // ---------------------------------------------------------------------------
 constexpr int G2P_THREADS = 256;
 size_t g2p_lds_bytes(int tz) { return (3 * sizeof(float4) + sizeof(float2)) * (tz + 3) * P2G_BOX * P2G_BOX; }
