@@ -185,12 +185,33 @@ __global__ __launch_bounds__(256) void k_place(const PlaceArgs *__restrict__ tab
 {
     const ItemBlock ib = item_block(bmap);
     const PlaceArgs &a = tab[ib.item];
+    // the rounds' keys, cell starts and tickets are each requested together,
+    // so a wave waits three memory round trips, not three per round
+    constexpr int NR = PER_BLOCK / 256;   // every lane of a wave runs the same rounds
+    const int lane = threadIdx.x & 63;
+    int key[NR], start[NR], head_lane[NR], got[NR];
 #pragma unroll
-    for (int r = 0; r < PER_BLOCK / 256; ++r) {   // every lane of a wave runs the same rounds
+    for (int r = 0; r < NR; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
-        const int key = (q < a.n) ? gl(a.keys)[q] : -1;
-        const int base = wave_run_ticket(key, gl(a.cursor));
-        if (key >= 0) gl(a.perm)[gl(a.cell_start)[key] + base] = (int)q;
+        key[r] = (q < a.n) ? gl(a.keys)[q] : -1;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) start[r] = (key[r] >= 0) ? gl(a.cell_start)[key[r]] : 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {   // wave_run_ticket, its atomic left in flight
+        const int prev = __shfl_up(key[r], 1, 64);
+        const bool head = (lane == 0) || (prev != key[r]);
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+        head_lane[r] = 63 - __clzll(below);
+        const unsigned long long above = heads & ~((2ull << lane) - 1 | (lane == 63 ? ~0ull : 0ull));
+        const int next = above ? __ffsll((long long)above) - 1 : 64;
+        got[r] = (head && key[r] >= 0) ? atomicAdd(&gl(a.cursor)[key[r]], next - lane) : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int base = __shfl(got[r], head_lane[r], 64) + (lane - head_lane[r]);
+        if (key[r] >= 0) gl(a.perm)[start[r] + base] = (int)((int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x);
     }
 }
 
