@@ -21,18 +21,78 @@ namespace epic {
 // ---------------------------------------------------------------------------
 // forward x pass: real rows -> half spectra
 // ---------------------------------------------------------------------------
-__device__ inline float load_phys(const FftXArgs &a, int k, int j, int i)
+// The global -> LDS staging loops below request LOAD_BATCH elements per
+// thread before the first LDS store, with clamped (always valid) addresses
+// and no branches between the loads, so a thread waits one memory round
+// trip per batch rather than one per element.
+constexpr int LOAD_BATCH = 8;
+
+// forward x input value at node offset o (mode fixed per launch item)
+enum XfKind { XF_FIELD, XF_FLUX, XF_FLUX_B, XF_PHI };
+template <int KIND> __device__ inline float phys_value(const FftXArgs &a, int64_t o, int64_t FS)
+{
+    if constexpr (KIND == XF_FIELD) return gl(a.fld)[a.slot * FS + o];
+    // parcel-correction source: volg / vcell - 1
+    if constexpr (KIND == XF_PHI) return gl(a.fld)[F_VOLG * FS + o] * (float)(1.0 / a.g.vcell) - 1.0f;
+    // vorticity-tendency flux: (omega_a + f_a) * vel_b (+ sgn * tbuoy)
+    float v = (gl(a.fld)[(F_VORTG + a.comp_a) * FS + o] + (float)a.g.f_cor[a.comp_a]) * gl(a.fld)[(F_VELOG + a.comp_b) * FS + o];
+    if constexpr (KIND == XF_FLUX_B) v += (float)a.sgn * gl(a.fld)[F_TBUOYG * FS + o];
+    return v;
+}
+
+// ZB rows of nx reals (planes zp0.., row j) packed as m = nx/2 complex
+// values per row into buf[r m + q]; planes above nz are zero
+template <int KIND> __device__ inline void stage_rows(const FftXArgs &a, float2 *buf, int j, int zp0, int m)
 {
     const Grid &g = a.g;
     const int64_t FS = field_stride(g), PS = plane_stride(g);
-    const int64_t o = (int64_t)(k + 1) * PS + (int64_t)j * g.nx + i;
-    if (a.mode == 0) return gl(a.fld)[a.slot * FS + o];
-    // parcel-correction source: volg / vcell - 1
-    if (a.mode == 3) return gl(a.fld)[F_VOLG * FS + o] * (float)(1.0 / g.vcell) - 1.0f;
-    // vorticity-tendency flux: (omega_a + f_a) * vel_b + sgn * tbuoy
-    float v = (gl(a.fld)[(F_VORTG + a.comp_a) * FS + o] + (float)g.f_cor[a.comp_a]) * gl(a.fld)[(F_VELOG + a.comp_b) * FS + o];
-    if (a.sgn != 0) v += (float)a.sgn * gl(a.fld)[F_TBUOYG * FS + o];
-    return v;
+    const int total = ZB * m;
+    for (int t0 = threadIdx.x; t0 < total; t0 += LOAD_BATCH * blockDim.x) {
+        float2 z[LOAD_BATCH];
+#pragma unroll
+        for (int u = 0; u < LOAD_BATCH; ++u) {
+            const int t = min(t0 + u * (int)blockDim.x, total - 1);
+            const int r = t / m, q = t - r * m;
+            const int64_t o = (int64_t)(min(zp0 + r, g.nz) + 1) * PS + (int64_t)j * g.nx + 2 * q;
+            z[u] = float2{phys_value<KIND>(a, o, FS), phys_value<KIND>(a, o + 1, FS)};
+        }
+#pragma unroll
+        for (int u = 0; u < LOAD_BATCH; ++u) {
+            const int t = t0 + u * blockDim.x;
+            if (t < total) buf[t] = (zp0 + t / m <= g.nz) ? z[u] : float2{0.0f, 0.0f};
+        }
+    }
+}
+
+// ZB z-contiguous spectra columns: buf[r * len + c] = in[(c * stride + col) * nzp + zp0 + r]
+// for c < len (MODE2: i kx in + i ky in2 of the y pass, kx factor kxw)
+template <bool MODE2>
+__device__ inline void stage_columns(const float2 *in, const float2 *in2, float2 *buf, int len, int64_t stride, int col,
+                                     const Grid &g, int zp0, float kxw)
+{
+    const int total = ZB * len;
+    for (int t0 = threadIdx.x; t0 < total; t0 += LOAD_BATCH * blockDim.x) {
+        float2 v[LOAD_BATCH];
+#pragma unroll
+        for (int u = 0; u < LOAD_BATCH; ++u) {
+            const int t = min(t0 + u * (int)blockDim.x, total - 1);
+            const int c = t / ZB, r = t - c * ZB;
+            const int64_t idx = ((int64_t)c * stride + col) * g.nzp + zp0 + r;
+            if constexpr (MODE2) {
+                const int kys = (c <= len / 2) ? c : c - len;
+                const float kyw = (2 * c == len) ? 0.0f : (float)(2.0 * 3.14159265358979323846 * kys / g.extent[1]);
+                const float2 f1 = gl(in)[idx], f2 = gl(in2)[idx];
+                v[u] = float2{-(kxw * f1.y + kyw * f2.y), kxw * f1.x + kyw * f2.x};   // i kx f1 + i ky f2
+            } else {
+                v[u] = gl(in)[idx];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LOAD_BATCH; ++u) {
+            const int t = t0 + u * blockDim.x, c = t / ZB, r = t - c * ZB;
+            if (t < total) buf[r * len + c] = v[u];
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fft_xf(const FftXArgs *__restrict__ tab, const int2 *__restrict__ bmap)
@@ -47,13 +107,10 @@ __global__ __launch_bounds__(256) void k_fft_xf(const FftXArgs *__restrict__ tab
     float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
     float2 *tw = bufb + ZB * (m + 1);   // twiddles staged in LDS
     for (int t = threadIdx.x; t < g.nx; t += blockDim.x) tw[t] = gl(a.tw)[t];
-    for (int t = threadIdx.x; t < ZB * m; t += blockDim.x) {
-        const int r = t / m, q = t - r * m;
-        const int k = zp0 + r;
-        float2 z = {0.0f, 0.0f};
-        if (k <= g.nz) z = float2{load_phys(a, k, j, 2 * q), load_phys(a, k, j, 2 * q + 1)};
-        bufa[r * m + q] = z;
-    }
+    if (a.mode == 0) stage_rows<XF_FIELD>(a, bufa, j, zp0, m);
+    else if (a.mode == 3) stage_rows<XF_PHI>(a, bufa, j, zp0, m);
+    else if (a.sgn != 0) stage_rows<XF_FLUX_B>(a, bufa, j, zp0, m);
+    else stage_rows<XF_FLUX>(a, bufa, j, zp0, m);
     __syncthreads();
     float2 *Z = lds_fft(bufa, bufb, ZB, a.plan, tw, 2, false);
     float2 *other = (Z == bufa) ? bufb : bufa;
@@ -95,21 +152,8 @@ __global__ __launch_bounds__(256) void k_fft_y(const FftYArgs *__restrict__ tab,
     for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = gl(a.tw)[t];
     // inverse combine factors: i kx (0 at the x Nyquist) and i ky (0 at the y Nyquist)
     const float kxw = (2 * kx == g.nx) ? 0.0f : (float)(2.0 * 3.14159265358979323846 * kx / g.extent[0]);
-    for (int t = threadIdx.x; t < ZB * ny; t += blockDim.x) {
-        const int jy = t / ZB, r = t - jy * ZB;
-        const int64_t idx = ((int64_t)jy * nxh + kx) * g.nzp + zp0 + r;
-        float2 v;
-        if (a.mode == 2) {
-            const int kys = (jy <= ny / 2) ? jy : jy - ny;
-            const float kyw = (2 * jy == ny) ? 0.0f : (float)(2.0 * 3.14159265358979323846 * kys / g.extent[1]);
-            const float2 f1 = gl(a.in)[idx], f2 = gl(a.in2)[idx];
-            // i kx f1 + i ky f2
-            v = float2{-(kxw * f1.y + kyw * f2.y), kxw * f1.x + kyw * f2.x};
-        } else {
-            v = gl(a.in)[idx];
-        }
-        bufa[r * ny + jy] = v;
-    }
+    if (a.mode == 2) stage_columns<true>(a.in, a.in2, bufa, ny, nxh, kx, g, zp0, kxw);
+    else stage_columns<false>(a.in, nullptr, bufa, ny, nxh, kx, g, zp0, kxw);
     __syncthreads();
     float2 *R = lds_fft(bufa, bufb, ZB, a.plan, tw, 1, a.mode != 0);
     for (int t = threadIdx.x; t < ZB * ny; t += blockDim.x) {
@@ -133,10 +177,7 @@ __global__ __launch_bounds__(256) void k_fft_xi(const FftXArgs *__restrict__ tab
     float2 *bufa = lds, *bufb = lds + ZB * (m + 1);
     float2 *tw = bufb + ZB * (m + 1);   // twiddles staged in LDS
     for (int t = threadIdx.x; t < g.nx; t += blockDim.x) tw[t] = gl(a.tw)[t];
-    for (int t = threadIdx.x; t < ZB * nxh; t += blockDim.x) {
-        const int kx = t / ZB, r = t - kx * ZB;
-        bufb[r * nxh + kx] = gl(a.in)[((int64_t)j * nxh + kx) * g.nzp + zp0 + r];
-    }
+    stage_columns<false>(a.in, nullptr, bufb, nxh, 1, j * nxh, g, zp0, 0.0f);
     __syncthreads();
     for (int t = threadIdx.x; t < ZB * m; t += blockDim.x) {
         const int r = t / m, k = t - r * m;
