@@ -26,6 +26,8 @@ def _members():
         "thermal16": W.rising_thermal(16, seed=3),
         "moist16": W.rising_thermal(16, seed=4, humidity_plume=True, n_blobs=4),
         "thermal32x16x24": W.rising_thermal(32, ny=16, nz=24, seed=5),
+        # radix-3 lengths in y and in the z transforms (ny = 24, 2 nz = 24)
+        "thermal16x24x12": W.rising_thermal(16, ny=24, nz=12, seed=6),
     }
 
 
