@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["abi.hip", "plan.hip", "p2g.hip", "g2p.hip", "fft.hip", "inversion.hip", "split.hip", "merge.hip",
            "correction.hip", "timestep.hip", "diagnostics.hip",
-           "resort.hip", "ncio.cpp"]
+           "resort.hip", "eigen.hip", "ncio.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -118,6 +118,8 @@ def load_library(path=LIB_PATH):
         "epic_batch_destroy": ([h], i),
         "epic_batch_last_error": ([h], C.c_char_p),
         "epic_batch_step": ([h, dp, dp, dp, ip], i),
+        "epic_diagonalise": ([i64, dp, dp, dp], i),
+        "epic_diagonalise_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -373,6 +375,23 @@ def member_bytes(nx, ny, nz, max_parcels):
     """Device bytes a member of this grid and capacity holds."""
     g = Grid(nx, ny, nz, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
     return int(load_library().epic_member_bytes(C.byref(g), int(max_parcels)))
+
+
+def diagonalise(A):
+    """scherzinger_diagonalise on the device for symmetric matrices A (n, 3,
+    3); returns D (n, 3) descending and V (n, 3, 3) with V[i, :, k] the
+    eigenvector of D[i, k]."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    # Fortran A(3,3,n): element (r, c, i) at 9 i + r + 3 c
+    a = np.ascontiguousarray(np.transpose(A, (0, 2, 1))).reshape(-1)
+    D = np.zeros((n, 3))
+    V = np.zeros((n, 3, 3))   # V[i, c, r] in memory = Fortran V(r, c, i)
+    L = load_library()
+    rc = L.epic_diagonalise(n, _dp(a), _dp(D), _dp(V))
+    if rc != 0:
+        raise EpicError(rc, L.epic_diagonalise_last_error().decode())
+    return D, np.ascontiguousarray(np.transpose(V, (0, 2, 1)))
 
 
 def _nc_chk(rc):

@@ -22,6 +22,8 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -89,30 +91,63 @@ def lattice_positions(nx, ny, nz, lower, dx, n_per_dim):
     return pos.reshape(-1, 3)
 
 
+def _threads():
+    return max(1, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1))
+
+
+def _chunked(fn, n, min_chunk=1 << 18):
+    """Runs fn(lo, hi) over [0, n) in chunks on a thread pool (numpy releases
+    the GIL inside its loops); the result does not depend on the chunking."""
+    nt = min(_threads(), max(1, n // min_chunk))
+    if nt <= 1:
+        fn(0, n)
+        return
+    bounds = np.linspace(0, n, 4 * nt + 1).astype(np.int64)
+    with ThreadPoolExecutor(nt) as ex:
+        list(ex.map(lambda i: fn(int(bounds[i]), int(bounds[i + 1])), range(len(bounds) - 1)))
+
+
 def trilinear_at(field, pos, lower, dx, nx, ny):
     """Trilinear interpolation (interpl.f90:17-56) of a node field given on
     nodes (nx, ny, nz+1) [x-periodic, y-periodic] at positions pos (n,3);
-    parcels must lie inside [lower_z, upper_z)."""
-    xyz = (pos - np.asarray(lower)) / np.asarray(dx)
-    ijk = np.floor(xyz).astype(np.int64)
-    p = xyz - ijk
-    out = np.zeros(pos.shape[0])
-    for kk in (0, 1):
-        wz = p[:, 2] if kk else 1 - p[:, 2]
-        for jj in (0, 1):
-            wy = p[:, 1] if jj else 1 - p[:, 1]
-            for ii in (0, 1):
-                wx = p[:, 0] if ii else 1 - p[:, 0]
-                out += wz * wy * wx * field[(ijk[:, 0] + ii) % nx, (ijk[:, 1] + jj) % ny, ijk[:, 2] + kk]
-    return out
+    parcels must lie inside [lower_z, upper_z).  field may also be a list of
+    such fields; then a list of results is returned."""
+    fields = field if isinstance(field, (list, tuple)) else [field]
+    # one periodic column and row appended: corner indices need no modulo
+    padded = [np.pad(f, ((0, 1), (0, 1), (0, 0)), mode="wrap").reshape(-1) for f in fields]
+    sy, sx = fields[0].shape[2], (ny + 1) * fields[0].shape[2]
+    outs = [np.empty(pos.shape[0]) for _ in fields]
+    low, dxa = np.asarray(lower), np.asarray(dx)
+
+    def run(lo, hi):
+        xyz = (pos[lo:hi] - low) / dxa
+        ijk = np.floor(xyz).astype(np.int64)
+        p = xyz - ijk
+        ijk[:, 0] %= nx
+        ijk[:, 1] %= ny
+        base = ijk[:, 0] * sx + ijk[:, 1] * sy + ijk[:, 2]
+        acc = [np.zeros(hi - lo) for _ in fields]
+        for kk in (0, 1):
+            wz = p[:, 2] if kk else 1 - p[:, 2]
+            for jj in (0, 1):
+                wy = p[:, 1] if jj else 1 - p[:, 1]
+                for ii in (0, 1):
+                    wx = p[:, 0] if ii else 1 - p[:, 0]
+                    w = wz * wy * wx
+                    idx = base + (ii * sx + jj * sy + kk)
+                    for a, f in zip(acc, padded):
+                        a += w * f[idx]
+        for o, a in zip(outs, acc):
+            o[lo:hi] = a
+
+    _chunked(run, pos.shape[0])
+    return outs if isinstance(field, (list, tuple)) else outs[0]
 
 
-def haar_rotations(rng, n):
-    """n Haar-random rotation matrices from normalised Gaussian quaternions."""
-    q = rng.standard_normal((n, 4))
-    q /= np.linalg.norm(q, axis=1, keepdims=True)
+def _rotation(q):
+    """Rotation matrices (m, 3, 3) of unit quaternions q (m, 4)."""
     w, x, y, z = q.T
-    R = np.empty((n, 3, 3))
+    R = np.empty((q.shape[0], 3, 3))
     R[:, 0, 0] = 1 - 2 * (y * y + z * z)
     R[:, 0, 1] = 2 * (x * y - z * w)
     R[:, 0, 2] = 2 * (x * z + y * w)
@@ -125,19 +160,34 @@ def haar_rotations(rng, n):
     return R
 
 
+def haar_rotations(rng, n):
+    """n Haar-random rotation matrices from normalised Gaussian quaternions."""
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return _rotation(q)
+
+
 def triaxial_B(rng, volume, lam_lo=1.0, lam_hi=4.0):
     """B = R diag(a^2, b^2, c^2) R^T with a/c log-uniform in [lam_lo, lam_hi],
     b = sqrt(ac) and abc = 3V/(4 pi); returns (n,5) = B11 B12 B13 B22 B23."""
     n = volume.shape[0]
     lam = np.exp(rng.uniform(math.log(lam_lo), math.log(lam_hi), n))
-    r = (3.0 * volume / (4.0 * math.pi)) ** (1.0 / 3.0)
-    a = r * np.sqrt(lam)
-    c = r / np.sqrt(lam)
-    b = r
-    R = haar_rotations(rng, n)
-    d2 = np.stack([a * a, b * b, c * c], axis=1)
-    Bf = np.einsum("nij,nj,nkj->nik", R, d2, R)
-    return np.stack([Bf[:, 0, 0], Bf[:, 0, 1], Bf[:, 0, 2], Bf[:, 1, 1], Bf[:, 1, 2]], axis=1)
+    q = rng.standard_normal((n, 4))
+    out = np.empty((n, 5))
+
+    def run(lo, hi):
+        qq = q[lo:hi] / np.linalg.norm(q[lo:hi], axis=1, keepdims=True)
+        R = _rotation(qq)
+        r = (3.0 * volume[lo:hi] / (4.0 * math.pi)) ** (1.0 / 3.0)
+        a = r * np.sqrt(lam[lo:hi])
+        c = r / np.sqrt(lam[lo:hi])
+        b = r
+        d2 = np.stack([a * a, b * b, c * c], axis=1)
+        Bf = np.einsum("nij,nj,nkj->nik", R, d2, R)
+        out[lo:hi] = np.stack([Bf[:, 0, 0], Bf[:, 0, 1], Bf[:, 0, 2], Bf[:, 1, 1], Bf[:, 1, 2]], axis=1)
+
+    _chunked(run, n)
+    return out
 
 
 def gaussian_random_field(rng, nx, ny, nz, slope=-3.0, rms=0.1):
@@ -223,10 +273,8 @@ def rising_thermal(n=128, seed=DEFAULT_SEED, ny=None, nz=None, humidity_plume=Fa
     B = triaxial_B(rng, vol)
     say("shape matrices")
     grf = [gaussian_random_field(rng, n, ny, nz) for _ in range(3)]
-    vor = np.empty((N, 3))
-    for c, f in enumerate(grf):
-        vor[:, c] = trilinear_at(f, pos, lower, dx, n, ny)
-        say(f"vorticity component {c}")
+    vor = np.stack(trilinear_at(grf, pos, lower, dx, n, ny), axis=1)
+    say("vorticity")
     L = np.asarray(extent)
     rel = (pos - np.asarray(lower)) / L
     if n_blobs:

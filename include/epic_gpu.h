@@ -269,6 +269,16 @@ int64_t epic_nearest_ncell(epic_handle h);
 
 int64_t epic_num_parcels(epic_handle h);
 
+/* scherzinger_diagonalise(A, D, V) (scherzinger.f90:31-112), as called by
+ * diagonalise (parcel_ellipsoid.f90:359-376), on the device in FP64 for n
+ * symmetric matrices A(3,3,n): D(3,n) eigenvalues in descending order,
+ * V(3,3,n) with V(:,k,i) the unit eigenvector of D(k,i).  This is the solver
+ * every par2grid, push, split and merge applies to the parcels' B.  Without
+ * a handle; the message of a failure is epic_diagonalise_last_error()
+ * (this thread). */
+int epic_diagonalise(int64_t n, const double *A, double *D, double *V);
+const char *epic_diagonalise_last_error(void);
+
 /* Per-phase GPU timers, the device counterpart of the reference's named
  * wall-clock timers (mpi_timer.f90:53-104, names registered in
  * epic3d.f90:79-95).  With profiling on, every phase is bracketed by HIP
