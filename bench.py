@@ -2,25 +2,34 @@
 batched vs served.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-cpu]
-                    [--members 16] [--requests 16] [--forms plain,batched,served]
+                    [--forms plain,batched,served] [--members 64] [--ens-steps 20]
+                    [--requests 12] [--rates 1,8,64] [--req-steps 50,500]
 
 plain (the headline ``value``): rising thermal, 128x128x128 grid, 8 parcels
 per cell (16,777,216 parcels), fixed dt = 0.01, synthetic seeded inputs,
 state resident in HBM.  One step = 5 stages of {par2grid, vor2vel,
-vorticity_tendency, fused grid2par + dB/dt + push} + reflective BC.
+vorticity_tendency, fused grid2par + dB/dt + push} + reflective BC.  The
+first step from the generated state runs with eager launches bracketed by
+HIP events (per-kernel times for the roofline, the same step the counter
+summaries under profiles/ come from); then W warm-up and K timed steps of
+the captured step graph.
 
-batched: an ensemble of members with grids drawn from {32,48,64,96}^2 x
-{32,64}, 8 parcels per cell, each with its own seed, advanced by one batched
-step (one launch per kernel over all members, one captured HIP graph)
-against the same members stepped one after another with the plain step.
+batched: 64 members with grids drawn from {32,48,64,96}^2 x {32,64}, 8
+parcels per cell, each with its own seed, 20 steps of one batched step (one
+launch per kernel over all members, one captured HIP graph) against the same
+members stepped one after another with the plain step; each member's
+max-relative error after its first batched step against the FP64 oracle.
 
-served: requests with grids drawn from {32,64,96,128}^2 x {32,64}, Poisson
-arrivals, 5-20 steps each, admitted against a 200 GB HBM budget and
-continuously batched; per-tick step latency p50/p99 and parcel-steps/s.
+served: requests with grids drawn from {32,64,96,128}^2 x {32,64}, 50-500
+steps each, Poisson arrivals at each rate of the sweep, admitted against a
+200 GB HBM budget and continuously batched; per-tick step latency p50/p99 and
+parcel-steps/s (busy and wall clock).
 
-Prints one JSON line (the driver's contract) with the step throughput,
-p50/p99 step latency, the dominant kernel's roofline and the CPU baseline
-(the FP64 oracle timed on the host cores on a bounded sample).
+cpu_baseline: the FP64 oracle (a C restatement of the reference's algorithm)
+on the host cores: step 1 of the same member is the warm-up and gives the
+GPU's first step its per-field error table, step 2 is timed.
+
+Prints one JSON line (the driver's contract).
 """
 from __future__ import annotations
 
@@ -28,6 +37,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -44,44 +54,126 @@ METRIC = "parcel-steps/s per GPU (RK4 step) + p50/p99 step latency, batched vs p
 # + h 4; the fused grid2par + push reads 104 and writes 100.
 BYTES_P2G = 68
 BYTES_PUSH = 204
+# HBM traffic per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs
+# on the 128^3 rising thermal (the first step from the generated state, the
+# same step this bench's event timing covers), scaled by the plain-copy
+# factors recorded in the same file
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_rising_thermal128.txt")
 
-# HBM traffic of k_g2p_tiles per launch on the 128^3 rising thermal, from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by what the same
-# counter reads for a 1 GiB device-to-device copy (FETCH x 2.0, WRITE x 1.0):
-# 6.336e9 B per launch over 16,777,216 parcels, averaged over the five stage
-# launches (stages 1-4 also store the next stage's half axes and read
-# buoyancy and humidity for the next par2grid's fixed-point maxima).
-TRAFFIC_BYTES_PER_PARCEL = 377.7
+DEFAULT_SEED = 20261202
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mem, max_threads=16):
-    """One LS-RK4 step of the FP64 oracle on this host (bounded sample)."""
+def host_threads():
+    """The host cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where set (a GPU box grants each GPU a share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def traffic_from_summary(kernel, path=TRAFFIC_SUMMARY):
+    """Bytes per launch of `kernel` from the committed counter summary
+    (lines "# traffic <kernel> <bytes per launch> <bytes per parcel>")."""
+    try:
+        with open(path) as fh:
+            for line in fh:
+                m = re.match(r"#\s*traffic\s+(\S+)\s+([0-9.eE+]+)\s+([0-9.eE+]+)", line)
+                if m and m.group(1) == kernel:
+                    return float(m.group(2)), float(m.group(3))
+    except OSError:
+        pass
+    return None, None
+
+
+def maxrel(a, ref):
+    a = np.asarray(a, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    den = np.abs(ref).max()
+    return float(np.abs(a - ref).max() / den) if den > 0 else float(np.abs(a).max())
+
+
+FIELDS = ("volg", "vortg", "tbuoyg", "dbuoyg", "humg", "velog", "velgradg", "vtend")
+ATTRS = ("vorticity", "B", "volume", "buoyancy", "humidity")
+
+
+def gpu_snapshot(gm):
+    """Parcels and the last stage's fields of a member (host FP64)."""
+    snap = {"parcels": gm.get_parcels()}
+    for f in FIELDS + ("nparg", "nsparg"):
+        snap[f] = gm.get_field(f)
+    return snap
+
+
+def error_table(mem, snap, st):
+    """max|gpu - fp64| / max|fp64| per field and attribute after the same
+    step from the same inputs (positions relative to the cell size)."""
+    nx, ny = mem.nx, mem.ny
+    sl = (Ellipsis, slice(1, nx + 1), slice(1, ny + 1), slice(None))
+    err = {}
+    p = snap["parcels"]
+    err["position/dx"] = float(np.abs(p["position"] - st.position).max() / max(mem.dx))
+    for k in ATTRS:
+        err[k] = maxrel(p[k], getattr(st, k))
+    for f in FIELDS:
+        err[f] = maxrel(snap[f][sl], getattr(st, f)[sl])
+    exact = all(np.array_equal(snap[f][sl], getattr(st, f)[sl]) for f in ("nparg", "nsparg"))
+    return {k: float(f"{v:.3e}") for k, v in err.items()}, exact
+
+
+def cpu_baseline(mem, snap):
+    """Two LS-RK4 steps of the FP64 oracle on this host: step 1 (warm-up)
+    gives the GPU's first step its error table, step 2 is timed."""
     from tests._util import oracle_state
 
-    nthreads = min(max_threads, os.cpu_count() or 1)
+    nthreads = host_threads()
     st = oracle_state(mem, nthreads=nthreads)
     t0 = time.perf_counter()
     st.ls_rk_step(0.0, 0.01)
+    t_warm = time.perf_counter() - t0
+    errors, exact = error_table(mem, snap, st) if snap is not None else (None, None)
+    t0 = time.perf_counter()
+    st.ls_rk_step(0.01, 0.01)
     dt = time.perf_counter() - t0
-    return {"value": mem.n / dt, "unit": "parcel-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"1 RK4 step of the same {mem.nx}^3 member ({mem.n} parcels), FP64 oracle, "
-                      f"OpenMP {nthreads} threads, {dt:.1f} s"}, dt
+    cb = {"value": mem.n / dt, "unit": "parcel-steps/s", "cores": nthreads, "kind": "port",
+          "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+          "sample": f"step 2 of the same {mem.nx}x{mem.ny}x{mem.nz} member ({mem.n} parcels), FP64 oracle "
+                    f"(C, OpenMP {nthreads} threads, static schedule), {dt:.1f} s; step 1 ({t_warm:.1f} s) "
+                    f"was the warm-up"}
+    return cb, dt, errors, exact
 
 
 def device_copy_calibration(nbytes=1 << 30, reps=3):
     """Plain device-to-device copies of a known size through the HIP runtime
     (the reference kernel that scales the HBM byte counters)."""
     hip = ctypes.CDLL("libamdhip64.so")
-    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+    hip.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+    hip.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
+    hip.hipFree.argtypes = [vp]
+    src, dst = vp(), vp()
+    assert hip.hipMalloc(ctypes.byref(src), nbytes) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), nbytes) == 0
+    assert hip.hipMemset(src, 1, nbytes) == 0
     for _ in range(reps):
-        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+        assert hip.hipMemcpy(dst, src, nbytes, 3) == 0   # hipMemcpyDeviceToDevice
     assert hip.hipDeviceSynchronize() == 0
     hip.hipFree(src)
     hip.hipFree(dst)
@@ -100,8 +192,9 @@ def ensemble_workloads(n, seed0, sizes_xy, sizes_z):
     return out
 
 
-def bench_batched(n_members, steps, seed0=20261202):
+def bench_batched(n_members, steps, errors=True, seed0=DEFAULT_SEED):
     from epic_amd.core import Batch, Member
+    from tests._util import oracle_state
 
     t0 = time.perf_counter()
     ws = ensemble_workloads(n_members, seed0, (32, 48, 64, 96), (32, 64))
@@ -121,24 +214,40 @@ def bench_batched(n_members, steps, seed0=20261202):
     plain_s = (time.perf_counter() - a) / steps
     for m in plain:
         m.close()
+    log(f"bench: batched: plain reference {plain_s * 1e3:.1f} ms per ensemble step")
     # batched: the same members in one batch
     mem = [Member.from_workload(w) for w in ws]
     b = Batch(mem)
     tb = np.zeros(n_members)
-    tb, _ = b.step(tb, dt)   # warm-up
+    tb, st = b.step(tb, dt)   # warm-up, step 1
+    assert (st == 0).all(), st
+    member_err = None
+    if errors:
+        t1 = time.perf_counter()
+        member_err = []
+        for w, m in zip(ws, mem):
+            got = m.get_parcels()
+            o = oracle_state(w, nthreads=host_threads())
+            o.ls_rk_step(0.0, dt)
+            e = max([float(np.abs(got["position"] - o.position).max() / max(w.dx))] +
+                    [maxrel(got[k], getattr(o, k)) for k in ATTRS])
+            member_err.append(float(f"{e:.3e}"))
+            del o
+        log(f"bench: batched: per-member errors after step 1 in {time.perf_counter() - t1:.1f} s, "
+            f"max {max(member_err):.2e}")
     lat = []
     for _ in range(steps):
         a = time.perf_counter()
         tb, st = b.step(tb, dt)
         lat.append(time.perf_counter() - a)
-        assert (st == 0).all()
+        assert (st == 0).all(), st
     b.close()
     for m in mem:
         m.close()
     lat = np.array(lat)
     out = {
         "workload": f"{n_members} rising-thermal members, grids from {{32,48,64,96}}^2 x {{32,64}}, "
-                    f"8 parcels/cell, seeds {seed0 + 1}..{seed0 + n_members}",
+                    f"8 parcels/cell, seeds {seed0 + 1}..{seed0 + n_members}, {steps} steps",
         "members": n_members, "parcels": npar, "steps": steps,
         "plain_parcel_steps_per_s": npar / plain_s,
         "batched_parcel_steps_per_s": npar / float(lat.mean()),
@@ -146,33 +255,47 @@ def bench_batched(n_members, steps, seed0=20261202):
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "plain_ms_per_ensemble_step": plain_s * 1e3,
     }
+    if member_err is not None:
+        out["max_rel_error_vs_fp64_step1"] = {"per_member": member_err, "max": max(member_err),
+                                              "fields": "position/dx, " + ", ".join(ATTRS)}
     log(f"bench: batched {out['batched_parcel_steps_per_s']:.3e} vs plain {out['plain_parcel_steps_per_s']:.3e} "
         f"parcel-steps/s (x{out['ratio']:.2f})")
     return out
 
 
-def bench_served(n_requests, rate, seed0=20261203):
+def bench_served(n_requests, rates, req_steps, seed0=DEFAULT_SEED + 1):
     from epic_amd.serve import Server, poisson_requests
 
     t0 = time.perf_counter()
     ws = ensemble_workloads(n_requests, seed0, (32, 64, 96, 128), (32, 64))
     log(f"bench: served: {n_requests} requests, {sum(w.n for w in ws)} parcels, generated in "
         f"{time.perf_counter() - t0:.1f} s")
-    reqs = poisson_requests(ws, rate=rate, steps=(5, 20), dt=0.01, seed=seed0)
     budget = 200 * 10**9
-    _, st = Server(budget, max_members=64).run(reqs)
-    out = {
+    sweep = []
+    for rate in rates:
+        reqs = poisson_requests(ws, rate=rate, steps=req_steps, dt=0.01, seed=seed0)
+        results, st = Server(budget, max_members=64).run(reqs)
+        failed = sum(1 for r in results.values() if r.status != 0)
+        sweep.append({"rate_per_s": rate, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "ticks": st["ticks"],
+                      "parcel_steps": st["parcel_steps"],
+                      "parcel_steps_per_s_busy": st["parcel_steps_per_s_busy"],
+                      "parcel_steps_per_s_wall": st["parcel_steps_per_s_wall"],
+                      "wall_s": st["wall_s"], "mean_members_per_tick": st["mean_members_per_tick"],
+                      "queue_wait_s_p50": st["queue_wait_s_p50"], "failed": failed})
+        log(f"bench: served at {rate}/s: {st['parcel_steps_per_s_busy']:.3e} parcel-steps/s busy, "
+            f"{st['parcel_steps_per_s_wall']:.3e} wall, p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms, "
+            f"{st['mean_members_per_tick']:.1f} members per tick")
+    top = max(sweep, key=lambda s: s["rate_per_s"])
+    return {
         "workload": f"{n_requests} requests, grids from {{32,64,96,128}}^2 x {{32,64}}, 8 parcels/cell, "
-                    f"Poisson arrivals at {rate}/s, 5-20 RK4 steps each, 200 GB HBM budget, continuous batching",
-        "requests": n_requests, "rate_per_s": rate,
-        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "ticks": st["ticks"],
-        "parcel_steps_per_s": st["parcel_steps_per_s_busy"],
-        "parcel_steps_per_s_wall": st["parcel_steps_per_s_wall"],
-        "mean_members_per_tick": st["mean_members_per_tick"],
+                    f"{req_steps[0]}-{req_steps[1]} RK4 steps each, Poisson arrivals, 200 GB HBM budget, "
+                    f"continuous batching",
+        "requests": n_requests, "rates_per_s": list(rates),
+        "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
+        "parcel_steps_per_s": top["parcel_steps_per_s_busy"],
+        "parcel_steps_per_s_wall": top["parcel_steps_per_s_wall"],
+        "sweep": sweep,
     }
-    log(f"bench: served {out['parcel_steps_per_s']:.3e} parcel-steps/s, p50 {out['p50_ms']:.1f} ms, "
-        f"p99 {out['p99_ms']:.1f} ms")
-    return out
 
 
 def main():
@@ -185,11 +308,15 @@ def main():
                     help="the large moist member's statistics (humidity plume, 32 buoyancy blobs) instead of the "
                          "rising thermal; with --size 256: 134M parcels")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--profile-steps", type=int, default=1,
+                    help="eager steps with per-kernel HIP-event timing, from the generated state, before warm-up")
     ap.add_argument("--forms", default="plain,batched,served")
-    ap.add_argument("--members", type=int, default=16)
-    ap.add_argument("--requests", type=int, default=16)
-    ap.add_argument("--rate", type=float, default=8.0)
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=20)
+    ap.add_argument("--no-member-errors", action="store_true")
+    ap.add_argument("--requests", type=int, default=12)
+    ap.add_argument("--rates", default="1,8,64")
+    ap.add_argument("--req-steps", default="50,500")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain device copies of 1 GiB (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -208,6 +335,23 @@ def main():
     gm = Member.from_workload(mem)
     dt = 0.01
     t = 0.0
+    want_err = not args.no_cpu
+
+    # step 1 (and any further profile steps) with eager launches and HIP
+    # events on the member's stream: per-kernel device times
+    gm.set_profiling(True)
+    gm.timers(reset=True)
+    snap = None
+    for i in range(args.profile_steps):
+        t = gm.ls_rk_step(t, dt)
+        if i == 0 and want_err:
+            snap = gpu_snapshot(gm)
+    tm = gm.timers(reset=True)
+    gm.set_profiling(False)
+    nprof = max(1, args.profile_steps)
+    phases = {k: round(v[0] / nprof, 4) for k, v in tm.items() if v[1]}
+    log(f"bench: {args.profile_steps} profiled step(s): {phases}")
+
     for _ in range(args.warmup):
         t = gm.ls_rk_step(t, dt)
     log(f"bench: {args.warmup} warm-up steps done")
@@ -219,26 +363,31 @@ def main():
     times = np.array(times) if times else np.full(1, np.nan)
     ms = float(times.mean() * 1e3)
     log(f"bench: {args.steps} timed steps, mean {ms:.2f} ms")
-
-    # per-kernel timing with HIP events on the member's stream (eager launches)
-    gm.set_profiling(True)
-    gm.timers(reset=True)
-    for _ in range(args.profile_steps):
-        t = gm.ls_rk_step(t, dt)
-    tm = gm.timers(reset=True)
-    gm.set_profiling(False)
-    push_ms, push_calls = tm["push"]
-    phases = {k: round(v[0] / max(1, args.profile_steps), 4) for k, v in tm.items() if v[1]}
-    push_avg = push_ms / max(1, push_calls)
-    achieved = BYTES_PUSH * mem.n / (push_avg * 1e-3) / 1e9 if push_avg > 0 else 0.0
-    # the counter figure belongs to the 128^3 rising thermal it was measured on
-    measured = args.size == 128 and not args.moist
-    traffic = TRAFFIC_BYTES_PER_PARCEL * mem.n if measured and TRAFFIC_BYTES_PER_PARCEL is not None else None
-    roof = {"kernel": "k_g2p_tiles (fused grid2par + dB/dt + RK push)", "bound": "hbm", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "avg_launch_ms": round(push_avg, 4), "bytes_per_parcel": BYTES_PUSH,
-            "units_per_launch": mem.n}
     gm.close()
+
+    # roofline of the tile kernels from their event times (average launch)
+    measured = args.size == 128 and not args.moist
+    kern = {}
+    for name, timer, nbytes in (("k_g2p_tiles", "push", BYTES_PUSH), ("k_p2g_tiles", "p2g_tiles", BYTES_P2G)):
+        tot, calls = tm.get(timer, (0.0, 0))
+        if not calls:
+            continue
+        avg = tot / calls
+        achieved = nbytes * mem.n / (avg * 1e-3) / 1e9
+        traffic, per_parcel = traffic_from_summary(name) if measured else (None, None)
+        kern[name] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                      "traffic_bytes_per_parcel": per_parcel, "avg_launch_ms": round(avg, 4), "launches": calls,
+                      "total_ms_per_step": round(tot / nprof, 4), "bytes_per_parcel": nbytes,
+                      "units_per_launch": mem.n}
+    dominant = max(kern, key=lambda k: kern[k]["total_ms_per_step"]) if kern else None
+    roof = dict(kern[dominant]) if dominant else {}
+    if dominant:
+        roof["kernel"] = dominant
+        roof["measured_on"] = ("step 1 from the generated state, eager launches, HIP events on the member's "
+                               "stream; traffic from rocprofv3 counters on the same step (" +
+                               os.path.relpath(TRAFFIC_SUMMARY, ROOT) + ")")
+        roof["limiter"] = "VALU issue (per-kernel counters in the profiles), below the HBM bound"
     if args.copy_calibration:
         device_copy_calibration()
     line = {
@@ -260,18 +409,24 @@ def main():
                                f"8 parcels/cell ({mem.n} parcels), "
                                f"plain LS-RK4 step, dt=0.01; FP32 storage, FP64 positions/eigen/z-transforms",
                    "grid": [mem.nx, mem.ny, mem.nz], "parcels": mem.n, "form": "plain"},
-        "phases_ms_per_step": phases,
+        "phases_ms_step1": phases,
         "roofline": roof,
+        "roofline_kernels": kern,
     }
     if "batched" in forms and args.members > 0:
-        line["batched"] = bench_batched(args.members, max(1, min(args.steps, 5)))
+        line["batched"] = bench_batched(args.members, max(1, args.ens_steps), errors=not args.no_member_errors)
     if "served" in forms and args.requests > 0:
-        line["served"] = bench_served(args.requests, args.rate)
+        rates = [float(r) for r in args.rates.split(",") if r]
+        lo, hi = (int(v) for v in args.req_steps.split(","))
+        line["served"] = bench_served(args.requests, rates, (lo, hi))
     if not args.no_cpu:
-        cb, cdt = cpu_baseline(mem)
+        cb, cdt, errors, exact = cpu_baseline(mem, snap)
         line["cpu_baseline"] = cb
         line["speedup_vs_cpu"] = round(cdt / (ms * 1e-3), 1)
-        log(f"bench: cpu baseline {cb['value']:.3e} parcel-steps/s")
+        if errors is not None:
+            line["max_rel_error_vs_fp64_step1"] = errors
+            line["counts_bit_exact_step1"] = exact
+        log(f"bench: cpu baseline {cb['value']:.3e} parcel-steps/s; step-1 errors {errors}")
     print(json.dumps(line), flush=True)
 
 

@@ -118,6 +118,9 @@ def load_library(path=LIB_PATH):
         "epic_batch_destroy": ([h], i),
         "epic_batch_last_error": ([h], C.c_char_p),
         "epic_batch_step": ([h, dp, dp, dp, ip], i),
+        "epic_batch_prepare": ([h], i),
+        "epic_batch_launch": ([h, dp, dp, dp], i),
+        "epic_batch_wait": ([h, dp, ip], i),
         "epic_diagonalise": ([i64, dp, dp, dp], i),
         "epic_diagonalise_last_error": ([], C.c_char_p),
     }
@@ -168,8 +171,11 @@ class Member:
 
     def _chk(self, rc):
         if rc != 0:
-            msg = load_library().epic_last_error(self._h)
-            raise EpicError(rc, msg.decode() if msg else "")
+            raise EpicError(rc, self.last_error())
+
+    def last_error(self):
+        msg = load_library().epic_last_error(self._h)
+        return msg.decode() if msg else ""
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -276,7 +282,7 @@ class Member:
         self._chk(load_library().epic_field_diagnostics(self._h, _dp(out)))
         return dict(zip(self.FIELD_STATS, out))
 
-    TIMERS = ("par2grid", "cell_sort", "vor2vel", "vorticity_tendency", "push", "grid2par", "axes")
+    TIMERS = ("par2grid", "cell_sort", "vor2vel", "vorticity_tendency", "push", "grid2par", "axes", "p2g_tiles")
 
     def set_profiling(self, on=True):
         self._chk(load_library().epic_set_profiling(self._h, int(bool(on))))
@@ -473,6 +479,35 @@ class Batch:
         if rc != 0:
             raise EpicError(rc, (L.epic_batch_last_error(self._h) or b"").decode())
         return tt, st
+
+    def prepare(self):
+        """Builds and captures the step graph without running it."""
+        L = load_library()
+        rc = L.epic_batch_prepare(self._h)
+        if rc != 0:
+            raise EpicError(rc, (L.epic_batch_last_error(self._h) or b"").decode())
+
+    def launch(self, t, dt, t_initial=None):
+        """Queues one step of every member and returns at once; wait()
+        completes it.  The members must not be used in between."""
+        nm = len(self.members)
+        self._t = np.ascontiguousarray(t, dtype=np.float64).copy()
+        dd = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (nm,)))
+        ti = np.zeros(nm) if t_initial is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(t_initial, dtype=np.float64), (nm,)))
+        L = load_library()
+        rc = L.epic_batch_launch(self._h, _dp(self._t), _dp(dd), _dp(ti))
+        if rc != 0:
+            raise EpicError(rc, (L.epic_batch_last_error(self._h) or b"").decode())
+
+    def wait(self):
+        """Completes the launched step: (new t array, status array)."""
+        st = np.zeros(len(self.members), dtype=np.int32)
+        L = load_library()
+        rc = L.epic_batch_wait(self._h, _dp(self._t), st.ctypes.data_as(C.POINTER(C.c_int)))
+        if rc != 0:
+            raise EpicError(rc, (L.epic_batch_last_error(self._h) or b"").decode())
+        return self._t, st
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -87,6 +87,16 @@ struct epic_member {
     // physical re-sort into cell order every resort_every steps (0: never)
     int resort_every = 3, steps_since_sort = 0;
     double *sort_tmp = nullptr;
+    // slot order: true while slot s holds caller index s (after set_parcels,
+    // split, merge); a re-sort makes it the cell order, which split and
+    // merge must not see (their pairings and child order follow the index)
+    bool caller_order = true;
+    // the stage tendencies (dpos, dvor, dB, strain) are not moved by a
+    // re-sort: while tend_displaced, the tendencies of the parcel in slot s
+    // are in slot tend_src[s], until settle_tendencies gathers them
+    bool tend_displaced = false;
+    int *tend_src = nullptr;
+    Plan pl_settle, pl_unsort;
     // p2g's fixed-point maxima (P2G_NF + 1)
     float *pmax_out = nullptr;
     Plan pl_resort;
@@ -106,6 +116,8 @@ struct epic_member {
         pl_lap[0].release();
         pl_lap[1].release();
         pl_resort.release();
+        pl_settle.release();
+        pl_unsort.release();
     }
 };
 
@@ -125,12 +137,14 @@ int fail(epic_member *m, int code, const std::string &what)
         if (e_ != hipSuccess) return fail(m, EPIC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-template <typename T> hipError_t dalloc(T **p, int64_t count)
+// zero-filled device array; the fill is queued on the owner's stream (the
+// streams are non-blocking, so nothing here waits for other members' work)
+template <typename T> hipError_t dalloc(T **p, int64_t count, hipStream_t s)
 {
     *p = nullptr;
     if (count <= 0) count = 1;
     hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)count);
-    if (e == hipSuccess) e = hipMemset(*p, 0, sizeof(T) * (size_t)count);
+    if (e == hipSuccess) e = hipMemsetAsync(*p, 0, sizeof(T) * (size_t)count, s);
     return e;
 }
 
@@ -266,6 +280,11 @@ void collect_timers(epic_member *m)
                 m->timers_ms[EPIC_TIMER_PAR2GRID] += ms;
                 break;
             case PH_P2G: m->timers_ms[EPIC_TIMER_PAR2GRID] += ms; m->timer_calls[EPIC_TIMER_PAR2GRID] += 1; break;
+            case PH_P2G_TILES:
+                m->timers_ms[EPIC_TIMER_P2G_TILES] += ms;
+                m->timer_calls[EPIC_TIMER_P2G_TILES] += 1;
+                m->timers_ms[EPIC_TIMER_PAR2GRID] += ms;
+                break;
             case PH_VOR2VEL: m->timers_ms[EPIC_TIMER_VOR2VEL] += ms; m->timer_calls[EPIC_TIMER_VOR2VEL] += 1; break;
             case PH_VTEND: m->timers_ms[EPIC_TIMER_VTEND] += ms; m->timer_calls[EPIC_TIMER_VTEND] += 1; break;
             case PH_PUSH: m->timers_ms[EPIC_TIMER_PUSH] += ms; m->timer_calls[EPIC_TIMER_PUSH] += 1; break;
@@ -730,6 +749,9 @@ void add_gather_errors(Plan &P, const std::vector<ErrArgs> &items)
 // ===========================================================================
 extern "C" {
 
+static int settle_tendencies(epic_member *m);
+static int restore_caller_order(epic_member *m);
+
 int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_flags_t *flags, int64_t max_parcels,
                 epic_handle *out)
 {
@@ -779,24 +801,24 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
         }
         return false;
     };
-    if (bad(hipStreamCreate(&m->s))) return EPIC_E_HIP;
+    if (bad(hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking))) return EPIC_E_HIP;
     const int64_t c = m->cap, nc = ncells(g);
-    if (bad(dalloc(&m->pos, 3 * c)) || bad(dalloc(&m->attr, (int64_t)NATTR * c)) || bad(dalloc(&m->orig, c)) ||
-        bad(dalloc(&m->axbuf, 12 * c)) || bad(dalloc(&m->keys, c)) || bad(dalloc(&m->perm, c + 1)) ||
-        bad(dalloc(&m->cnt[0], nc)) || bad(dalloc(&m->scnt[0], nc)) || bad(dalloc(&m->cnt[1], nc)) ||
-        bad(dalloc(&m->scnt[1], nc)) || bad(dalloc(&m->cnt[2], nc)) || bad(dalloc(&m->scnt[2], nc)) ||
-        bad(dalloc(&m->cell_start, nc + 1)) || bad(dalloc(&m->cursor, nc)) ||
+    if (bad(dalloc(&m->pos, 3 * c, m->s)) || bad(dalloc(&m->attr, (int64_t)NATTR * c, m->s)) || bad(dalloc(&m->orig, c, m->s)) ||
+        bad(dalloc(&m->axbuf, 12 * c, m->s)) || bad(dalloc(&m->keys, c, m->s)) || bad(dalloc(&m->perm, c + 1, m->s)) ||
+        bad(dalloc(&m->cnt[0], nc, m->s)) || bad(dalloc(&m->scnt[0], nc, m->s)) || bad(dalloc(&m->cnt[1], nc, m->s)) ||
+        bad(dalloc(&m->scnt[1], nc, m->s)) || bad(dalloc(&m->cnt[2], nc, m->s)) || bad(dalloc(&m->scnt[2], nc, m->s)) ||
+        bad(dalloc(&m->cell_start, nc + 1, m->s)) || bad(dalloc(&m->cursor, nc, m->s)) ||
         bad(dalloc(&m->bsum, scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nc, c),
-                                                                  (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2))))))
+                                                                  (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2))), m->s)))
         return EPIC_E_NOMEM;
     const int64_t FS = field_stride(g);
-    if (bad(dalloc(&m->fld, (int64_t)F_COUNT * FS)) || bad(dalloc(&m->nparg, FS)) || bad(dalloc(&m->nsparg, FS)) ||
-        bad(dalloc(&m->staging, staging_floats_max(g))))
+    if (bad(dalloc(&m->fld, (int64_t)F_COUNT * FS, m->s)) || bad(dalloc(&m->nparg, FS, m->s)) || bad(dalloc(&m->nsparg, FS, m->s)) ||
+        bad(dalloc(&m->staging, staging_floats_max(g), m->s)))
         return EPIC_E_NOMEM;
     for (int i = 0; i < NSPEC; ++i)
-        if (bad(dalloc(&m->spec[i], spec_size(g)))) return EPIC_E_NOMEM;
-    if (bad(dalloc(&m->err_d, 4)) || bad(dalloc(&m->flags_d, 4)) || bad(dalloc(&m->tm_d, 1))) return EPIC_E_NOMEM;
-    if (bad(dalloc(&m->pmax_out, P2G_NF + 1))) return EPIC_E_NOMEM;
+        if (bad(dalloc(&m->spec[i], spec_size(g), m->s))) return EPIC_E_NOMEM;
+    if (bad(dalloc(&m->err_d, 4, m->s)) || bad(dalloc(&m->flags_d, 4, m->s)) || bad(dalloc(&m->tm_d, 1, m->s))) return EPIC_E_NOMEM;
+    if (bad(dalloc(&m->pmax_out, P2G_NF + 1, m->s))) return EPIC_E_NOMEM;
     if (bad(hipHostMalloc((void **)&m->err_h, sizeof(int) * 4, hipHostMallocDefault)) ||
         bad(hipHostMalloc((void **)&m->tm_h, sizeof(StepTime), hipHostMallocDefault)))
         return EPIC_E_NOMEM;
@@ -811,10 +833,11 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     for (int t = 0; t < g.nx; ++t) tx[t] = float2{(float)std::cos(tp * t / g.nx), (float)-std::sin(tp * t / g.nx)};
     for (int t = 0; t < g.ny; ++t) ty[t] = float2{(float)std::cos(tp * t / g.ny), (float)-std::sin(tp * t / g.ny)};
     for (int t = 0; t < 2 * g.nz; ++t) tz[t] = double2{std::cos(tp * t / (2 * g.nz)), -std::sin(tp * t / (2 * g.nz))};
-    if (bad(dalloc(&m->twx, g.nx)) || bad(dalloc(&m->twy, g.ny)) || bad(dalloc(&m->twz, 2 * g.nz))) return EPIC_E_NOMEM;
-    if (bad(hipMemcpy(m->twx, tx.data(), sizeof(float2) * tx.size(), hipMemcpyHostToDevice)) ||
-        bad(hipMemcpy(m->twy, ty.data(), sizeof(float2) * ty.size(), hipMemcpyHostToDevice)) ||
-        bad(hipMemcpy(m->twz, tz.data(), sizeof(double2) * tz.size(), hipMemcpyHostToDevice)))
+    if (bad(dalloc(&m->twx, g.nx, m->s)) || bad(dalloc(&m->twy, g.ny, m->s)) || bad(dalloc(&m->twz, 2 * g.nz, m->s))) return EPIC_E_NOMEM;
+    if (bad(hipMemcpyAsync(m->twx, tx.data(), sizeof(float2) * tx.size(), hipMemcpyHostToDevice, m->s)) ||
+        bad(hipMemcpyAsync(m->twy, ty.data(), sizeof(float2) * ty.size(), hipMemcpyHostToDevice, m->s)) ||
+        bad(hipMemcpyAsync(m->twz, tz.data(), sizeof(double2) * tz.size(), hipMemcpyHostToDevice, m->s)) ||
+        bad(hipStreamSynchronize(m->s)))
         return EPIC_E_HIP;
     for (int i = 0; i < 6; ++i) {
         m->ax_cur[i] = m->axbuf + i * c;
@@ -833,7 +856,7 @@ int epic_destroy(epic_handle m)
     void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
                     m->cnt[2], m->scnt[2], m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
                     m->twx, m->twy, m->twz, m->err_d, m->flags_d, m->vw_d, m->tm_d, m->diag_d,
-                    m->sort_tmp, m->pmax_out};
+                    m->sort_tmp, m->pmax_out, m->tend_src};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < NSPEC; ++i)
@@ -883,6 +906,8 @@ int epic_set_parcels(epic_handle m, int64_t n, const double *position, const dou
     if (n) HIPOK(m, hipMemcpyAsync(m->orig, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice, m->s));
     HIPOK(m, hipStreamSynchronize(m->s));
     m->keys_valid = false;
+    m->caller_order = true;
+    m->tend_displaced = false;
     return EPIC_OK;
 }
 
@@ -920,6 +945,10 @@ int epic_get_parcel_tendencies(epic_handle m, int64_t n, double *delta_pos, doub
 {
     if (!m) return EPIC_E_INVALID;
     if (n != m->n) return fail(m, EPIC_E_INVALID, "epic_get_parcel_tendencies: n differs from the parcel count");
+    {
+        const int rc = settle_tendencies(m);
+        if (rc) return rc;
+    }
     const int64_t c = m->cap;
     std::vector<float> h((size_t)14 * n);
     std::vector<int> ho(n);
@@ -946,9 +975,14 @@ int epic_set_parcel_tendencies(epic_handle m, int64_t n, const double *delta_pos
 {
     if (!m) return EPIC_E_INVALID;
     if (n != m->n) return fail(m, EPIC_E_INVALID, "epic_set_parcel_tendencies: n differs from the parcel count");
+    {   // dB and strain stay with their parcels
+        const int rc = settle_tendencies(m);
+        if (rc) return rc;
+    }
     const int64_t c = m->cap;
     std::vector<int> ho(n);
-    if (n) HIPOK(m, hipMemcpy(ho.data(), m->orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (n) HIPOK(m, hipMemcpyAsync(ho.data(), m->orig, sizeof(int) * n, hipMemcpyDeviceToHost, m->s));
+    HIPOK(m, hipStreamSynchronize(m->s));
     std::vector<float> h((size_t)6 * n);
     for (int64_t q = 0; q < n; ++q)
         for (int d = 0; d < 3; ++d) {
@@ -1043,6 +1077,11 @@ int epic_vorticity_tendency(epic_handle m)
 int epic_grid2par(epic_handle m, int add)
 {
     if (!m) return EPIC_E_INVALID;
+    if (add == 1) {   // adds onto the tendencies of the same parcels
+        const int rc = settle_tendencies(m);
+        if (rc) return rc;
+    }
+    m->tend_displaced = false;   // rewritten (add) or added onto in place
     Plan &P = m->pl_g2p[add == 1 ? 1 : 0];
     if (P.empty()) {
         G2PArgs a{};
@@ -1092,17 +1131,113 @@ static void build_resort(epic_member *m)
     add_resort(P, PH_SORT, {a});
 }
 
+static int ensure_sort_scratch(epic_member *m)
+{
+    if (!m->sort_tmp) HIPOK(m, dalloc(&m->sort_tmp, resort_tmp_doubles(m->cap), m->s));
+    if (!m->tend_src) HIPOK(m, dalloc(&m->tend_src, m->cap, m->s));
+    return EPIC_OK;
+}
+
+// the 19 tendency arrays gathered by src (dst[s] = a[src[s]]) through sort_tmp
+static void build_tendency_gather(epic_member *m, Plan &P, const int *src)
+{
+    const int64_t c = m->cap;
+    ResortArgs a{};
+    a.perm = src;
+    a.n = m->n;
+    for (int i = 0; i < RESORT_F; ++i) {   // attr slots 11..21 (dpos, dvor, dB) and 28..35 (strain)
+        const int slot = i < 11 ? 11 + i : 28 + (i - 11);
+        a.f[i] = m->attr + slot * c;
+        a.tf[i] = (float *)(m->sort_tmp) + i * c;
+    }
+    a.gather = 1;
+    add_resort(P, PH_SORT, {a}, false);
+    a.gather = 0;
+    add_resort(P, PH_SORT, {a}, false);
+}
+
+// puts every parcel's stage tendencies back into its own slot
+static int settle_tendencies(epic_member *m)
+{
+    if (!m->tend_displaced) return EPIC_OK;
+    m->tend_displaced = false;
+    if (m->n == 0) return EPIC_OK;
+    int rc = ensure_sort_scratch(m);
+    if (rc) return rc;
+    if (m->pl_settle.empty()) build_tendency_gather(m, m->pl_settle, m->tend_src);
+    rc = run_plan(m, m->pl_settle);
+    if (rc) return rc;
+    return check(m, "settle_tendencies");
+}
+
+// after the persistent arrays were gathered by perm (n entries): the
+// tendencies stay where they were, so the parcel now in slot s finds them in
+// slot perm[s]
+static int displace_tendencies(epic_member *m, const int *perm)
+{
+    if (m->n) HIPOK(m, hipMemcpyAsync(m->tend_src, perm, sizeof(int) * m->n, hipMemcpyDeviceToDevice, m->s));
+    m->tend_displaced = true;
+    return EPIC_OK;
+}
+
 static int resort(epic_member *m)
 {
     m->steps_since_sort = 0;
     if (m->n == 0) return EPIC_OK;
-    int rc = ensure_keys(m);   // keys and the histogram cnt[0] of the current positions
+    int rc = settle_tendencies(m);   // one displacement at a time
     if (rc) return rc;
-    if (!m->sort_tmp) HIPOK(m, dalloc(&m->sort_tmp, resort_tmp_doubles(m->cap)));
+    rc = ensure_keys(m);   // keys and the histogram cnt[0] of the current positions
+    if (rc) return rc;
+    rc = ensure_sort_scratch(m);
+    if (rc) return rc;
     if (m->pl_resort.empty()) build_resort(m);
     rc = run_plan(m, m->pl_resort);
     if (rc) return rc;
+    rc = displace_tendencies(m, m->perm);
+    if (rc) return rc;
+    m->caller_order = false;
     return check(m, "epic_resort");
+}
+
+// gathers the persistent arrays back into the caller's order (slot s holds
+// caller index s again), the inverse of every re-sort since the last one
+static int restore_caller_order(epic_member *m)
+{
+    if (m->caller_order) return EPIC_OK;
+    m->caller_order = true;
+    if (m->n == 0) return EPIC_OK;
+    int rc = settle_tendencies(m);
+    if (rc) return rc;
+    rc = ensure_sort_scratch(m);
+    if (rc) return rc;
+    if (m->pl_unsort.empty()) {
+        Plan &P = m->pl_unsort;
+        const int64_t n = m->n, c = m->cap;
+        add_invert(P, PH_SORT, {InvertArgs{m->orig, n, m->perm}});
+        ResortArgs a{};
+        a.perm = m->perm;
+        a.n = n;
+        for (int d = 0; d < RESORT_W; ++d) {
+            a.w[d] = m->pos + d * c;
+            a.tw[d] = m->sort_tmp + d * c;
+        }
+        int k = 0;
+        for (int i = 0; i < 11; ++i) a.f[k++] = m->attr + i * c;
+        a.f[k++] = (float *)m->orig;
+        a.f[k++] = (float *)m->keys;
+        for (int i = 0; i < 6; ++i) a.f[k++] = m->ax_stale[i];
+        float *tf = (float *)(m->sort_tmp + RESORT_W * c);
+        for (int i = 0; i < RESORT_F; ++i) a.tf[i] = tf + i * c;
+        a.gather = 1;
+        add_resort(P, PH_SORT, {a});
+        a.gather = 0;
+        add_resort(P, PH_SORT, {a});
+    }
+    rc = run_plan(m, m->pl_unsort);
+    if (rc) return rc;
+    rc = displace_tendencies(m, m->perm);
+    if (rc) return rc;
+    return check(m, "restore_caller_order");
 }
 
 // one RK step of a member: the plan (fixed or adaptive) and its StepTime
@@ -1127,6 +1262,7 @@ static int run_step(epic_member *m, double *t, double t_initial, bool adaptive, 
         HIPOK(m, P.run_graph(m->s));
     }
     HIPOK(m, hipMemcpyAsync(m->tm_h, m->tm_d, sizeof(StepTime), hipMemcpyDeviceToHost, m->s));
+    m->tend_displaced = false;   // every stage rewrote the tendencies in place
     rc = check(m, where);   // synchronises the stream
     if (rc) return rc;
     *dt_taken = m->tm_h->dt;
@@ -1240,6 +1376,9 @@ struct epic_batch {
     Plan plan;
     std::vector<uint64_t> gens;
     std::string msg;
+    // a launched step not yet waited for: each member's dt
+    bool in_flight = false;
+    std::vector<double> dt_pending;
 };
 
 namespace {
@@ -1410,8 +1549,8 @@ int epic_batch_create(const epic_handle *members, int count, epic_batch_handle *
         }
         return false;
     };
-    if (bad(hipStreamCreate(&b->s))) return EPIC_E_HIP;
-    if (bad(dalloc(&b->flags_d, count)) || bad(dalloc(&b->errs_d, count)) || bad(dalloc(&b->tms_d, count)))
+    if (bad(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking))) return EPIC_E_HIP;
+    if (bad(dalloc(&b->flags_d, count, b->s)) || bad(dalloc(&b->errs_d, count, b->s)) || bad(dalloc(&b->tms_d, count, b->s)))
         return EPIC_E_NOMEM;
     if (bad(hipHostMalloc((void **)&b->flags_h, sizeof(int) * count, hipHostMallocDefault)) ||
         bad(hipHostMalloc((void **)&b->errs_h, sizeof(int) * count, hipHostMallocDefault)) ||
@@ -1439,12 +1578,11 @@ int epic_batch_destroy(epic_batch_handle b)
 
 const char *epic_batch_last_error(epic_batch_handle b) { return b ? b->msg.c_str() : "null handle"; }
 
-int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const double *t_initial, int *status)
+// builds (when membership or a member changed) and captures the batch's
+// step graph; keys of fresh members are computed on their own streams
+static int batch_prepare(epic_batch *b)
 {
-    if (!b || !t || !dt || !t_initial) return EPIC_E_INVALID;
     const int nm = (int)b->ms.size();
-    for (int i = 0; i < nm; ++i)
-        if (!(dt[i] > 0.0)) return batch_fail(b, EPIC_E_DT, "epic_batch_step: time step must be positive");
     for (int i = 0; i < nm; ++i) {
         epic_member *m = b->ms[i];
         if (!m->keys_valid) {
@@ -1468,6 +1606,26 @@ int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const doub
         b->gens.clear();
         for (int i = 0; i < nm; ++i) b->gens.push_back(b->ms[i]->gen);
     }
+    BHIPOK(b, b->plan.prepare_graph(b->s));
+    return EPIC_OK;
+}
+
+int epic_batch_prepare(epic_batch_handle b)
+{
+    if (!b) return EPIC_E_INVALID;
+    if (b->in_flight) return batch_fail(b, EPIC_E_INVALID, "epic_batch_prepare: a step is in flight");
+    return batch_prepare(b);
+}
+
+int epic_batch_launch(epic_batch_handle b, const double *t, const double *dt, const double *t_initial)
+{
+    if (!b || !t || !dt || !t_initial) return EPIC_E_INVALID;
+    if (b->in_flight) return batch_fail(b, EPIC_E_INVALID, "epic_batch_launch: the previous step was not waited for");
+    const int nm = (int)b->ms.size();
+    for (int i = 0; i < nm; ++i)
+        if (!(dt[i] > 0.0)) return batch_fail(b, EPIC_E_DT, "epic_batch_step: time step must be positive");
+    int rc = batch_prepare(b);
+    if (rc) return rc;
     // the first par2grid of a step reuses the cached axes (ls_rk.f90:93)
     for (int i = 0; i < nm; ++i) {
         b->flags_h[i] = (t[i] > t_initial[i]) ? 1 : 0;
@@ -1479,14 +1637,26 @@ int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const doub
     BHIPOK(b, hipMemcpyAsync(b->tms_d, b->tms_h, sizeof(StepTime) * nm, hipMemcpyHostToDevice, b->s));
     BHIPOK(b, b->plan.run_graph(b->s));
     BHIPOK(b, hipMemcpyAsync(b->errs_h, b->errs_d, sizeof(int) * nm, hipMemcpyDeviceToHost, b->s));
+    b->dt_pending.assign(dt, dt + nm);
+    b->in_flight = true;
+    return EPIC_OK;
+}
+
+int epic_batch_wait(epic_batch_handle b, double *t, int *status)
+{
+    if (!b || !t) return EPIC_E_INVALID;
+    if (!b->in_flight) return batch_fail(b, EPIC_E_INVALID, "epic_batch_wait: no step in flight");
+    b->in_flight = false;
     BHIPOK(b, hipStreamSynchronize(b->s));
     BHIPOK(b, hipGetLastError());
+    const int nm = (int)b->ms.size();
     int first = EPIC_OK;
     for (int i = 0; i < nm; ++i) {
         epic_member *m = b->ms[i];
         int rc = error_code(m, b->errs_h[i], "epic_batch_step");
+        m->tend_displaced = false;   // every stage rewrote the tendencies in place
         if (rc == EPIC_OK) {
-            t[i] += dt[i];
+            t[i] += b->dt_pending[i];
             if (m->resort_every > 0 && ++m->steps_since_sort >= m->resort_every) rc = resort(m);
         }
         if (status) status[i] = rc;
@@ -1496,6 +1666,13 @@ int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const doub
         }
     }
     return first;
+}
+
+int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const double *t_initial, int *status)
+{
+    const int rc = epic_batch_launch(b, t, dt, t_initial);
+    if (rc) return rc;
+    return epic_batch_wait(b, t, status);
 }
 
 int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_split)
@@ -1508,6 +1685,10 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
         amax = std::cbrt(0.75 / 3.14159265358979323846) * std::min(g.dx[0], std::min(g.dx[1], g.dx[2]));
     const int64_t n0 = m->n;
     if (n0 == 0) return EPIC_OK;
+    {   // children follow their parents' caller index (parcel_split.f90:87-160)
+        const int rc = restore_caller_order(m);
+        if (rc) return rc;
+    }
     // flags in keys, child slots (exclusive prefix, total at [n0]) in perm
     SplitArgs a{};
     a.g = g;
@@ -1554,6 +1735,10 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
 {
     if (!m) return EPIC_E_INVALID;
     if (n_merge) *n_merge = 0;
+    {   // pairings, tie-breaks and the hole filling follow the caller index
+        const int rc = restore_caller_order(m);
+        if (rc) return rc;
+    }
     MergeCtx c{};
     c.g = m->g;
     c.p = m->pd;
@@ -1624,7 +1809,7 @@ int epic_apply_gradient(epic_handle m, double prefactor, double max_compression,
 static int volume_weighted_sums(epic_member *m, double out[4])
 {
     if (!m->vw_d) {
-        HIPOK(m, dalloc(&m->vw_d, vw_partials(m->cap) + 4));
+        HIPOK(m, dalloc(&m->vw_d, vw_partials(m->cap) + 4, m->s));
         HIPOK(m, hipHostMalloc((void **)&m->vw_h, sizeof(double) * 4, hipHostMallocDefault));
     }
     double *tot = m->vw_d + vw_partials(m->cap);
@@ -1670,7 +1855,7 @@ static int diag_run(epic_member *m, bool parcels, double *out, int nv, const cha
 {
     const int64_t tot = diag_scratch_doubles(m->g, m->cap);
     if (!m->diag_d) {
-        HIPOK(m, dalloc(&m->diag_d, tot));
+        HIPOK(m, dalloc(&m->diag_d, tot, m->s));
         HIPOK(m, hipHostMalloc((void **)&m->diag_h, sizeof(double) * 16, hipHostMallocDefault));
     }
     double *res = m->diag_d + tot - 16;
@@ -1688,6 +1873,10 @@ int epic_parcel_diagnostics(epic_handle m, double *stats, int reset_counters)
 {
     if (!m || !stats) return EPIC_E_INVALID;
     for (int i = 0; i < 18; ++i) stats[i] = 0.0;
+    {   // the velocity is each parcel's delta_pos
+        const int rc = settle_tendencies(m);
+        if (rc) return rc;
+    }
     const Grid &g = m->g;
     if (m->n > 0) {
         double s[12];
@@ -1772,12 +1961,12 @@ int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)
     const Grid &g = m->g;
     const int64_t ncell = (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2), n = m->n;
     int *keys = nullptr, *cnt = nullptr, *start = nullptr, *cur = nullptr, *ptmp = nullptr, *pm = nullptr;
-    HIPOK(m, dalloc(&keys, n));
-    HIPOK(m, dalloc(&cnt, ncell));
-    HIPOK(m, dalloc(&start, ncell + 1));
-    HIPOK(m, dalloc(&cur, ncell));
-    HIPOK(m, dalloc(&ptmp, n));
-    HIPOK(m, dalloc(&pm, n));
+    HIPOK(m, dalloc(&keys, n, m->s));
+    HIPOK(m, dalloc(&cnt, ncell, m->s));
+    HIPOK(m, dalloc(&start, ncell + 1, m->s));
+    HIPOK(m, dalloc(&cur, ncell, m->s));
+    HIPOK(m, dalloc(&ptmp, n, m->s));
+    HIPOK(m, dalloc(&pm, n, m->s));
     hipLaunchKernelGGL(k_nearest_keys, dim3(grid_blocks(std::max<int64_t>(n, 1))), dim3(256), 0, m->s, g, m->pd, keys,
                        cnt, ncell, m->err_d);
     {
@@ -1791,11 +1980,12 @@ int epic_cell_sort(epic_handle m, int *loca, int *nppc, int *kc1, int *node)
     int rc = check(m, "epic_cell_sort");
     std::vector<int> hk(n), hp(n), ho(n), hs(ncell + 1), hc(ncell);
     if (rc == EPIC_OK) {
-        HIPOK(m, hipMemcpy(hk.data(), keys, sizeof(int) * n, hipMemcpyDeviceToHost));
-        HIPOK(m, hipMemcpy(hp.data(), pm, sizeof(int) * n, hipMemcpyDeviceToHost));
-        HIPOK(m, hipMemcpy(ho.data(), m->orig, sizeof(int) * n, hipMemcpyDeviceToHost));
-        HIPOK(m, hipMemcpy(hs.data(), start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToHost));
-        HIPOK(m, hipMemcpy(hc.data(), cnt, sizeof(int) * ncell, hipMemcpyDeviceToHost));
+        HIPOK(m, hipMemcpyAsync(hk.data(), keys, sizeof(int) * n, hipMemcpyDeviceToHost, m->s));
+        HIPOK(m, hipMemcpyAsync(hp.data(), pm, sizeof(int) * n, hipMemcpyDeviceToHost, m->s));
+        HIPOK(m, hipMemcpyAsync(ho.data(), m->orig, sizeof(int) * n, hipMemcpyDeviceToHost, m->s));
+        HIPOK(m, hipMemcpyAsync(hs.data(), start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToHost, m->s));
+        HIPOK(m, hipMemcpyAsync(hc.data(), cnt, sizeof(int) * ncell, hipMemcpyDeviceToHost, m->s));
+        HIPOK(m, hipStreamSynchronize(m->s));
         for (int64_t q = 0; q < n; ++q) {
             if (loca) loca[ho[q]] = hk[q] + 1;
             if (node) node[q] = ho[hp[q]];

@@ -232,7 +232,15 @@ struct ResortArgs {
     double *tw[RESORT_W];
     float *tf[RESORT_F];
 };
-void add_resort(Plan &P, int phase, const std::vector<ResortArgs> &items);
+// with_doubles: the RESORT_W double arrays move too; otherwise only the
+// RESORT_F float arrays (the stage tendencies: dpos 3, dvor 3, dB 5, strain 8)
+void add_resort(Plan &P, int phase, const std::vector<ResortArgs> &items, bool with_doubles = true);
+struct InvertArgs {
+    const int *orig;
+    int64_t n;
+    int *inv;
+};
+void add_invert(Plan &P, int phase, const std::vector<InvertArgs> &items);
 
 // ---- utilities (abi.hip) ----
 struct FillArgs {

@@ -718,7 +718,7 @@ void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::v
             nb.push_back(a.ntx * a.nty * a.nchz);
         }
         const void *k = tz == 2 ? (const void *)k_p2g_tiles<2> : tz == 4 ? (const void *)k_p2g_tiles<4> : (const void *)k_p2g_tiles<8>;
-        P.add(k, 64 * tz * P2G_TPC, p2g_lds_bytes(tz), phase, items, nb);
+        P.add(k, 64 * tz * P2G_TPC, p2g_lds_bytes(tz), phase == PH_P2G ? (int)PH_P2G_TILES : phase, items, nb);
     }
     std::vector<int> nf;
     for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny * (f.g.nz + 3) + 255) / 256));

@@ -40,6 +40,7 @@ enum Phase : int {
     PH_VTEND,
     PH_PUSH,
     PH_GRID2PAR,
+    PH_P2G_TILES,   // the tile scatter of par2grid alone (also counted in par2grid)
     PH_OTHER,
     PH_COUNT
 };
@@ -82,13 +83,15 @@ public:
     size_t size() const { return recs_.size(); }
     const std::vector<Rec> &recs() const { return recs_; }
 
-    // copies the arena to the device (once; again after new launches)
-    hipError_t upload();
+    // copies the arena to the device on s (once; again after new launches)
+    hipError_t upload(hipStream_t s);
     // enqueues every launch on s; with ev != nullptr records an event pair
     // around every run of launches of one phase (ev_phase gets the phases)
     hipError_t run(hipStream_t s, std::vector<hipEvent_t> *ev = nullptr, std::vector<int> *ev_phase = nullptr);
     // captures the launches into a graph (once) and launches it on s
     hipError_t run_graph(hipStream_t s);
+    // uploads and captures (once) without launching
+    hipError_t prepare_graph(hipStream_t s);
     void release();
     ~Plan() { release(); }
     Plan() = default;

@@ -3,7 +3,7 @@
 
 namespace epic {
 
-hipError_t Plan::upload()
+hipError_t Plan::upload(hipStream_t s)
 {
     if (dev_ready_) return hipSuccess;
     if (exec_) {
@@ -23,7 +23,8 @@ hipError_t Plan::upload()
         dev_size_ = host_.size();
     }
     if (!host_.empty()) {
-        hipError_t e = hipMemcpy(dev_, host_.data(), host_.size(), hipMemcpyHostToDevice);
+        hipError_t e = hipMemcpyAsync(dev_, host_.data(), host_.size(), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return e;
     }
     dev_ready_ = true;
@@ -32,7 +33,7 @@ hipError_t Plan::upload()
 
 hipError_t Plan::run(hipStream_t s, std::vector<hipEvent_t> *ev, std::vector<int> *ev_phase)
 {
-    hipError_t e = upload();
+    hipError_t e = upload(s);
     if (e != hipSuccess) return e;
     int cur = -1;
     hipEvent_t stop = nullptr;
@@ -60,9 +61,9 @@ hipError_t Plan::run(hipStream_t s, std::vector<hipEvent_t> *ev, std::vector<int
     return hipSuccess;
 }
 
-hipError_t Plan::run_graph(hipStream_t s)
+hipError_t Plan::prepare_graph(hipStream_t s)
 {
-    hipError_t e = upload();
+    hipError_t e = upload(s);
     if (e != hipSuccess) return e;
     if (!exec_) {
         e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -74,6 +75,13 @@ hipError_t Plan::run_graph(hipStream_t s)
         e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
         if (e != hipSuccess) return e;
     }
+    return hipSuccess;
+}
+
+hipError_t Plan::run_graph(hipStream_t s)
+{
+    hipError_t e = prepare_graph(s);
+    if (e != hipSuccess) return e;
     return hipGraphLaunch(exec_, s);
 }
 

@@ -2,21 +2,29 @@
 
 Requests are member runs (a grid, its parcels, a fixed dt and a number of
 RK steps) that arrive over time.  The server keeps every admitted member's
-parcel and field state resident in HBM and advances all of them together
-with one batched LS-RK step per tick (``core.Batch``: one launch per kernel
-over all members, one captured HIP graph).  Between ticks it
+parcel and field state resident in HBM.  Resident members are grouped into
+cohorts; a cohort is one ``core.Batch`` (one launch per kernel over all its
+members, one captured HIP graph) on its own stream.  Every tick launches the
+step of every cohort without waiting (``Batch.launch``), so cohorts run
+concurrently on the GPU, and while they run the server
 
-* retires the members that have taken their requested steps (their final
-  parcel state becomes the result, their device memory is freed),
 * admits waiting requests in arrival order while the HBM budget
-  (``core.member_bytes`` of every resident member) and the slot limit allow,
+  (``core.member_bytes`` of every resident member) and the slot limit allow:
+  it allocates the member, uploads its parcels on the member's own stream
+  and captures the new cohort's step graph, all overlapping the tick (a
+  retired member's memory counts against the budget until it is released);
+* copies out the results of members retired at the previous tick.
 
-so requests arriving while others are on the GPU join the running batch as
-soon as memory frees up.  A member's arithmetic does not depend on the other
-members of its batch, so every result equals the member's run alone.
+Then it waits for the tick (``Batch.wait``).  A member that has taken its
+requested steps, or whose step failed (its status is returned in the
+result), leaves its cohort; only that cohort's graph is rebuilt.  Device
+memory is released after the wait, when no step is in flight.  A member's
+arithmetic does not depend on the other members of its cohort or on the
+cohorts running beside it, so every result equals the member's run alone.
 
 Reported: per-tick step latency (p50/p99), per-request queueing and
-completion times, and the aggregate parcel-steps/s.
+completion times, and the aggregate parcel-steps/s (over the busy ticks and
+over the wall clock).
 """
 from __future__ import annotations
 
@@ -42,11 +50,36 @@ class Request:
 @dataclasses.dataclass
 class Result:
     rid: int
-    parcels: dict
+    parcels: dict             # final parcel state (None when the run failed)
     t: float
     admitted: float
     finished: float
     arrival: float
+    steps_done: int = 0
+    status: int = 0           # 0, or the EPIC_E_* code of the failed step
+    message: str = ""
+
+
+@dataclasses.dataclass
+class _Entry:
+    req: Request
+    member: Member
+    nbytes: int
+    admitted: float
+    t: float = 0.0
+    steps: int = 0
+    status: int = 0
+    message: str = ""
+
+
+class _Cohort:
+    def __init__(self, entries):
+        self.entries = list(entries)
+        self.batch = Batch([e.member for e in self.entries])
+        self.batch.prepare()
+
+    def close(self):
+        self.batch.close()
 
 
 class Server:
@@ -64,16 +97,24 @@ class Server:
         """Serves ``requests`` (any order; admitted by arrival time) to
         completion.  Returns (results by rid, stats)."""
         waiting = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        active = []            # [req, member, t, steps_done, admitted_at]
+        for r in waiting:
+            if self._bytes(r) > self.budget:
+                raise ValueError(f"request {r.rid} does not fit the HBM budget")
+        cohorts = []           # list of _Cohort
+        fresh = []             # admitted entries that join at the next tick
+        to_collect = []        # retired entries whose results are copied out during the next tick
+        to_free = []           # members released after the next wait
         used = 0
         results = {}
         tick_ms, tick_parcels, tick_members = [], [], []
-        batch, batch_key = None, None
         t0 = clock()
-        while waiting or active:
-            now = clock() - t0
-            # admission: arrival order, HBM budget and slot limit
-            while (waiting and waiting[0].arrival <= now and len(active) < self.max_members
+
+        def n_active():
+            return sum(len(c.entries) for c in cohorts) + len(fresh)
+
+        def admit(now):
+            nonlocal used
+            while (waiting and waiting[0].arrival <= now and n_active() < self.max_members
                    and used + self._bytes(waiting[0]) <= self.budget):
                 req = waiting.popleft()
                 m = Member.from_workload(req.workload, rk_order=req.rk_order)
@@ -82,49 +123,91 @@ class Server:
                     if key not in self._tuned:
                         self._tuned[key] = m.autotune()
                     m.set_tiling(*self._tuned[key])
-                used += self._bytes(req)
-                active.append([req, m, 0.0, 0, now])
-            if waiting and not active and self._bytes(waiting[0]) > self.budget:
-                raise ValueError(f"request {waiting[0].rid} does not fit the HBM budget")
-            if not active:
-                time.sleep(idle_sleep)
+                nb = self._bytes(req)
+                used += nb
+                fresh.append(_Entry(req, m, nb, now))
+
+        def collect():
+            for e in to_collect:
+                parcels = e.member.get_parcels() if e.status == 0 else None
+                results[e.req.rid] = Result(e.req.rid, parcels, e.t, e.admitted, clock() - t0, e.req.arrival,
+                                            e.steps, e.status, e.message)
+                to_free.append(e)
+            to_collect.clear()
+
+        def release():
+            nonlocal used
+            for e in to_free:
+                e.member.close()
+                used -= e.nbytes
+            to_free.clear()
+
+        while waiting or cohorts or fresh or to_collect:
+            if fresh:
+                cohorts.append(_Cohort(fresh))
+                fresh = []
+            if not cohorts:
+                collect()
+                release()
+                if not waiting:
+                    break
+                now = clock() - t0
+                if waiting[0].arrival > now:
+                    time.sleep(min(idle_sleep, waiting[0].arrival - now))
+                admit(clock() - t0)
                 continue
-            key = tuple(id(a[1]) for a in active)
-            if key != batch_key:
-                if batch is not None:
-                    batch.close()
-                batch = Batch([a[1] for a in active])
-                batch_key = key
-            ts = np.array([a[2] for a in active])
-            dts = np.array([a[0].dt for a in active])
+            # one tick: every cohort's step in flight together
             a0 = clock()
-            ts, status = batch.step(ts, dts)
+            for c in cohorts:
+                c.batch.launch([e.t for e in c.entries], [e.req.dt for e in c.entries])
+            # overlapping the tick: results of the last retirements, admissions
+            collect()
+            admit(clock() - t0)
+            if fresh:
+                cohorts_new = _Cohort(fresh)
+                fresh = []
+            else:
+                cohorts_new = None
+            stepped = 0
+            nmem = 0
+            retired_any = []
+            for c in cohorts:
+                ts, status = c.batch.wait()
+                keep = []
+                for e, t, st in zip(c.entries, ts, status):
+                    nmem += 1
+                    if st == 0:
+                        e.t = float(t)
+                        e.steps += 1
+                        stepped += e.req.workload.n
+                    else:
+                        e.status = int(st)
+                        e.message = e.member.last_error()
+                    if st != 0 or e.steps >= e.req.steps:
+                        to_collect.append(e)   # its memory counts until released
+                    else:
+                        keep.append(e)
+                if len(keep) != len(c.entries):
+                    retired_any.append((c, keep))
             tick_ms.append((clock() - a0) * 1e3)
-            tick_parcels.append(sum(a[0].workload.n for a in active))
-            tick_members.append(len(active))
-            done = []
-            for a, t in zip(active, ts):
-                a[2] = float(t)
-                a[3] += 1
-                if a[3] >= a[0].steps:
-                    done.append(a)
-            for a in done:
-                req, m = a[0], a[1]
-                results[req.rid] = Result(req.rid, m.get_parcels(), a[2], a[4], clock() - t0, req.arrival)
-                active.remove(a)
-                used -= self._bytes(req)
-            if done:
-                if batch is not None:
-                    batch.close()
-                batch, batch_key = None, None
-                for a in done:
-                    a[1].close()
+            tick_parcels.append(stepped)
+            tick_members.append(nmem)
+            # no step in flight: release memory, rebuild the cohorts that lost members
+            release()
+            for c, keep in retired_any:
+                cohorts.remove(c)
+                c.close()
+                if keep:
+                    cohorts.append(_Cohort(keep))
+            if cohorts_new is not None:
+                cohorts.append(cohorts_new)
         wall = clock() - t0
-        if batch is not None:
-            batch.close()
+        for c in cohorts:
+            c.close()
         lat = np.array(tick_ms) if tick_ms else np.zeros(1)
         parcel_steps = float(np.sum(tick_parcels))
         busy = float(np.sum(tick_ms)) * 1e-3
+        done = [r for r in results.values()]
         stats = {
             "ticks": len(tick_ms),
             "p50_ms": float(np.percentile(lat, 50)),
@@ -134,8 +217,8 @@ class Server:
             "parcel_steps_per_s_wall": parcel_steps / wall if wall > 0 else 0.0,
             "parcel_steps_per_s_busy": parcel_steps / busy if busy > 0 else 0.0,
             "mean_members_per_tick": float(np.mean(tick_members)) if tick_members else 0.0,
-            "queue_wait_s_p50": float(np.percentile([r.admitted - r.arrival for r in results.values()], 50))
-            if results else 0.0,
+            "queue_wait_s_p50": float(np.percentile([r.admitted - r.arrival for r in done], 50)) if done else 0.0,
+            "failed": sum(1 for r in done if r.status != 0),
         }
         return results, stats
 

@@ -294,7 +294,8 @@ enum {
     EPIC_TIMER_PUSH = 4,
     EPIC_TIMER_GRID2PAR = 5,
     EPIC_TIMER_AXES = 6,        /* support-point half axes (FP64 eigen-solve), once per stage */
-    EPIC_NUM_TIMERS = 7
+    EPIC_TIMER_P2G_TILES = 7,   /* the tile scatter kernel alone (inside par2grid) */
+    EPIC_NUM_TIMERS = 8
 };
 int epic_set_profiling(epic_handle h, int on);
 int epic_get_timers(epic_handle h, double *ms, int *calls, int reset);
@@ -321,6 +322,17 @@ const char *epic_batch_last_error(epic_batch_handle b);
 /* t[i] += dt[i] for every member whose step succeeded; status[i] (may be
  * NULL) gets each member's code; returns the first failure or EPIC_OK. */
 int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const double *t_initial, int *status);
+/* epic_batch_step in two halves, so that a server can admit new members
+ * while a step runs: epic_batch_launch queues the step of every member on
+ * the batch's own stream and returns at once; epic_batch_wait waits for it,
+ * then does the rest of epic_batch_step (statuses, t[i] += dt[i], periodic
+ * re-sorts).  Until the wait the batch's members must not be used; other
+ * members and batches may be, and their work overlaps on the GPU.
+ * epic_batch_prepare builds and captures the step graph (done by the first
+ * launch otherwise), e.g. while another batch's step is in flight. */
+int epic_batch_prepare(epic_batch_handle b);
+int epic_batch_launch(epic_batch_handle b, const double *t, const double *dt, const double *t_initial);
+int epic_batch_wait(epic_batch_handle b, double *t, int *status);
 
 #ifdef __cplusplus
 }

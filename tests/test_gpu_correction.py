@@ -108,7 +108,7 @@ def test_step_after_correction_matches_oracle():
     st.ls_rk_step(0.0, 0.01)
     p = gm.get_parcels()
     assert np.abs(p["position"] - st.position).max() / max(mem.dx) <= 1e-5
-    assert maxrel(p["vorticity"], st.vorticity) <= 3e-5
+    assert maxrel(p["vorticity"], st.vorticity) <= 1e-5
 
 
 def test_vortcor_matches_oracle():
@@ -124,7 +124,7 @@ def test_vortcor_matches_oracle():
     gm.apply_vortcor()
     st.apply_vortcor(vb_ref)
     p = gm.get_parcels()
-    assert maxrel(p["vorticity"], st.vorticity) <= 3e-5
+    assert maxrel(p["vorticity"], st.vorticity) <= 1e-5
     w = (p["volume"][:, None] * p["vorticity"]).sum(0) / p["volume"].sum()
     assert np.abs(w - vb_ref).max() <= 1e-6 * np.abs(p["vorticity"]).max()
 

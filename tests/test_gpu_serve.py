@@ -53,3 +53,58 @@ def test_poisson_arrivals_and_autotuned_tiles():
     assert sorted(results) == [r.rid for r in reqs]
     assert stats["p99_ms"] >= stats["p50_ms"] > 0
     assert stats["parcel_steps"] == sum(r.workload.n * r.steps for r in reqs)
+
+
+def test_failed_member_is_retired_with_its_status():
+    """A member whose step fails (a parcel more than one cell below the
+    lower wall: "Wrong total number of parcels", parcel_interpl.f90:248-250)
+    leaves the batch with its status and message; the others finish and
+    equal their runs alone."""
+    import dataclasses
+
+    good = [W.rising_thermal(16, seed=51), W.rising_thermal(16, ny=24, nz=12, seed=52)]
+    bad = W.rising_thermal(16, seed=53)
+    pos = bad.position.copy()
+    pos[17, 2] = bad.lower[2] - 1.5 * bad.dx[2]
+    bad = dataclasses.replace(bad, position=pos)
+    reqs = [Request(0, good[0], 3, 0.01, 0.0), Request(1, bad, 3, 0.01, 0.0), Request(2, good[1], 2, 0.01, 0.0)]
+    results, stats = Server(1 << 34, max_members=8).run(reqs)
+    assert results[1].status == -3 and results[1].parcels is None and results[1].steps_done == 0
+    assert "Wrong total number of parcels" in results[1].message
+    assert stats["failed"] == 1
+    for r in (reqs[0], reqs[2]):
+        p, t = _alone(r)
+        got = results[r.rid]
+        assert got.status == 0 and got.steps_done == r.steps and abs(got.t - t) < 1e-15
+        for k in ("position", "vorticity", "B", "volume"):
+            assert np.array_equal(got.parcels[k], p[k]), (r.rid, k)
+
+
+def test_launch_wait_overlaps_admission():
+    """Two batches in flight together, a third member created and uploaded
+    while they run: each result equals its run alone."""
+    from epic_amd.core import Batch
+
+    ws = [W.rising_thermal(16, seed=61), W.rising_thermal(24, ny=16, nz=16, seed=62), W.beltrami(16)]
+    ms = [Member.from_workload(w) for w in ws[:2]]
+    b1, b2 = Batch([ms[0]]), Batch([ms[1]])
+    t1 = t2 = np.zeros(1)
+    late = None
+    for step in range(3):
+        b1.launch(t1, 0.01)
+        b2.launch(t2, 0.01)
+        if step == 0:
+            late = Member.from_workload(ws[2])
+        t1, s1 = b1.wait()
+        t2, s2 = b2.wait()
+        assert s1[0] == 0 and s2[0] == 0
+    tl = 0.0
+    for _ in range(3):
+        tl = late.ls_rk_step(tl, 0.01)
+    for w, m in zip(ws, ms + [late]):
+        p, _ = _alone(Request(0, w, 3, 0.01, 0.0))
+        got = m.get_parcels()
+        for k in ("position", "vorticity", "B"):
+            assert np.array_equal(got[k], p[k]), (w.name, k)
+    b1.close()
+    b2.close()

@@ -53,5 +53,5 @@ def test_step_after_split_matches_oracle():
     st.ls_rk_step(0.0, 0.01)
     p = gm.get_parcels()
     assert np.abs(p["position"] - st.position).max() / max(mem.dx) <= 1e-5
-    assert maxrel(p["vorticity"], st.vorticity) <= 3e-5
-    assert maxrel(p["B"], st.B) <= 3e-5
+    assert maxrel(p["vorticity"], st.vorticity) <= 1e-5
+    assert maxrel(p["B"], st.B) <= 1e-5
