@@ -58,11 +58,13 @@ class TimeOpts(C.Structure):
 _lib = None
 
 
-def load_library(path=LIB_PATH):
-    """Load libepic_gpu.so; raises if it has not been built."""
+def load_library(path=None):
+    """Load libepic_gpu.so (or the build named by EPIC_GPU_LIB); raises if it
+    has not been built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("EPIC_GPU_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with `python -m epic_amd.build`")
     L = C.CDLL(path)

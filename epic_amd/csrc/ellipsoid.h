@@ -23,12 +23,38 @@ namespace epic {
 #define EPIC_RHO 0.63245553203367586640
 #define EPIC_RT2I 0.70710678118654752440
 
+// FP64 reciprocal and reciprocal square root: the hardware estimates
+// refined by two Newton steps each (within an ulp or so of the correctly
+// rounded values, in a third of the instructions of the IEEE division and
+// square root sequences).  x > 0 for the square roots, x != 0 for rcp_nr.
+__device__ inline double rcp_nr(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double e = fma(-x, y, 1.0);
+        y = fma(y, e, y);
+    }
+    return y;
+}
+__device__ inline double rsqrt_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double r = fma(-x * y, y, 1.0);   // 1 - x y^2
+        y = fma(0.5 * y, r, y);
+    }
+    return y;
+}
+__device__ inline double sqrt_nr(double x) { return x > 0.0 ? x * rsqrt_nr(x) : 0.0; }
+
 __device__ inline bool b33_of(const double B[5], double vol, double &b33)
 {
     const double abc = 0.75 * vol * (1.0 / 3.14159265358979323846);
     const double den = B[0] * B[3] - B[1] * B[1];
     if (fabs(den) <= 2.220446049250313e-16) return false;
-    b33 = (abc * abc - B[2] * (B[1] * B[4] - B[2] * B[3]) + B[0] * B[4] * B[4] - B[1] * B[2] * B[4]) / den;
+    b33 = (abc * abc - B[2] * (B[1] * B[4] - B[2] * B[3]) + B[0] * B[4] * B[4] - B[1] * B[2] * B[4]) * rcp_nr(den);
     return true;
 }
 
@@ -68,8 +94,9 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         const double j2 = 0.5 * (aa00 + aa11 + aa22);
         const double j3 = A[0] * (A[4] * A[8] - A[5] * A[5]) - A[1] * (A[1] * A[8] - A[2] * A[5]) +
                           A[2] * (A[1] * A[5] - A[2] * A[4]);
-        const double q3 = 3.0 / j2;
-        double smp = 0.5 * j3 * (q3 * sqrt(q3));   // (3/j2)^(3/2)
+        // (3/j2)^(3/2) = 3 sqrt(3) j2^(-3/2)
+        const double rj = rsqrt_nr(j2);
+        double smp = (0.5 * 5.1961524227066318806) * j3 * (rj * rj * rj);
         smp = fmin(fmax(-1.0, smp), 1.0);
         // cos(alpha), alpha = acos(smp)/3, shifted by 2 pi/3 when alpha >
         // pi/6 (scherzinger.f90:175-181): by cos(3a) = 4 cos^3 a - 3 cos a it
@@ -77,13 +104,17 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         // of smp (+sqrt(3)/2 at 0).  That root of 4c^3 - 3c = |smp| lies in
         // [sqrt(3)/2, 1]; a degree-4 least-squares fit of cos(acos(s)/3) on
         // [0, 1] starts within 1e-5 of it, and two Newton steps (error
-        // ratio below 1.7 e^2 there) reach it to FP64 rounding.
+        // ratio below 1.7 e^2 there) reach it to FP64 rounding; the
+        // derivative's reciprocal only needs the hardware estimate (a
+        // relative error d in it adds d e to the step's error).
         const double sabs = fabs(smp);
         double cr = 0.8660344784 + sabs * (0.1663758323 + sabs * (-0.04585590254 + sabs * (0.01750308532 + sabs * -0.004064325447)));
 #pragma unroll
-        for (int it = 0; it < 2; ++it) cr = cr - ((4.0 * cr * cr - 3.0) * cr - sabs) / (12.0 * cr * cr - 3.0);
+        for (int it = 0; it < 2; ++it)
+            cr = cr - ((4.0 * cr * cr - 3.0) * cr - sabs) * __builtin_amdgcn_rcp(12.0 * cr * cr - 3.0);
         const double cos_alpha = (smp < 0.0) ? -cr : cr;
-        const double eta1 = 2.0 * sqrt((1.0 / 3.0) * j2) * cos_alpha;
+        // 2 sqrt(j2 / 3) = (2 / sqrt(3)) j2 / sqrt(j2)
+        const double eta1 = 1.1547005383792515290 * (j2 * rj) * cos_alpha;
 
         double r[3][3];   // r[c] = column c of A - eta1 I
 #pragma unroll
@@ -99,7 +130,7 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         if (l1 > lm) { im = 1; lm = l1; }
         if (l2 > lm) { im = 2; lm = l2; }
         double s1[3];
-        const double ilm = 1.0 / sqrt(lm);
+        const double ilm = rsqrt_nr(lm);
 #pragma unroll
         for (int i = 0; i < 3; ++i) s1[i] = r[im][i] * ilm;
         if (im == 1) {
@@ -122,7 +153,7 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         l2 = dsq3(r[2][0], r[2][1], r[2][2]);
         {
             const bool use2 = l2 > l1;
-            const double il = 1.0 / sqrt(use2 ? l2 : l1);
+            const double il = rsqrt_nr(use2 ? l2 : l1);
 #pragma unroll
             for (int i = 0; i < 3; ++i) r[1][i] = (use2 ? r[2][i] : r[1][i]) * il;
         }
@@ -138,7 +169,7 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         t = s1As1 - s2As2;
         s = s1As1 + s2As2;
         const double sg = (t == 0.0) ? 0.0 : copysign(1.0, t);
-        const double eta2 = 0.5 * s - 0.5 * sg * sqrt(t * t + 4.0 * s2As1 * s2As1);
+        const double eta2 = 0.5 * s - 0.5 * sg * sqrt_nr(t * t + 4.0 * s2As1 * s2As1);
         const double eta3 = s - eta2;
         D[0] = eta1 + t3;
         D[1] = eta2 + t3;
@@ -161,7 +192,7 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         const double m1 = dsq3(u1[0], u1[1], u1[2]);
         {
             const bool use1 = m1 > m0;
-            const double im_ = 1.0 / sqrt(use1 ? m1 : m0);
+            const double im_ = rsqrt_nr(use1 ? m1 : m0);
 #pragma unroll
             for (int i = 0; i < 3; ++i) V[2][i] = (use1 ? u1[i] : u0[i]) * im_;
         }
@@ -173,7 +204,7 @@ __device__ inline void scherzinger(const double A_in[9], double D[3], double V[3
         V[2][2] = V[0][0] * V[1][1] - V[0][1] * V[1][0];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double il = 1.0 / dnorm3(V[c][0], V[c][1], V[c][2]);
+            const double il = rsqrt_nr(dsq3(V[c][0], V[c][1], V[c][2]));
 #pragma unroll
             for (int i = 0; i < 3; ++i) V[c][i] *= il;
         }
@@ -203,8 +234,8 @@ __device__ inline bool support_axes(const double B[5], double vol, double veta[3
     const double A[9] = {B[0], B[1], B[2], B[1], B[3], B[4], B[2], B[4], b33};
     double D[3], V[3][3];
     scherzinger(A, D, V);
-    const double se = sqrt(fabs(D[0] - D[2])) * EPIC_RHO;
-    const double st = sqrt(fabs(D[1] - D[2])) * EPIC_RHO;
+    const double se = sqrt_nr(fabs(D[0] - D[2])) * EPIC_RHO;
+    const double st = sqrt_nr(fabs(D[1] - D[2])) * EPIC_RHO;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         veta[i] = se * V[0][i];

@@ -262,13 +262,17 @@ __global__ void k_cell_order(int64_t nc, const int *__restrict__ cell_start, con
 // different cells' parcels rarely add to one node in the same instruction.
 // Every contribution w * (vol/4) * value is scaled by a power of two chosen
 // per member and field from k_p2g_max's maxima (so that no single point's
-// contribution exceeds 2^28),
+// contribution exceeds 2^27),
 // rounded to an integer and added to a 64-bit LDS accumulator of its node;
-// the points of one parcel that fall in the same cell are combined first.  Integer
+// the points of one parcel that fall in the same cell are combined first.
+// Most points stay in their parcel's own cell: there the two threads of the
+// cell combine their (rounded) contributions by a lane exchange and each
+// adds the pair's sum to four of the cell's eight corners, so these atomics
+// are half as many and no two lanes of an instruction hit one node.  Integer
 // sums do not depend on the order of the additions, so the node sums are
 // bitwise reproducible however the waves are scheduled, with no ordering
 // pass and no barriers between parcels; the only rounding is that of each
-// contribution (2^-29 of the member's largest) and of the final conversion.
+// contribution (2^-28 of the member's largest) and of the final conversion.
 // Node box of a tile: x, y in [-1, 9] around the tile, node planes
 // [k0 - 1, k0 + tz + 1]; rows padded to lpitch 64-bit words (12 spreads the
 // 8 x 8 lanes of a wave evenly over the 32 LDS banks).
@@ -278,9 +282,10 @@ __host__ __device__ constexpr int box_planes(int tz) { return tz + 3; }
 // work-groups per CU within the LDS
 __host__ __device__ constexpr int p2g_lpitch(int tz) { return tz >= 8 ? P2G_BOX : 12; }
 
-// power-of-two scale s with max_abs * s < 2^28 (0 when max_abs is 0): the
+// power-of-two scale s with max_abs * s < 2^27 (0 when max_abs is 0): the
 // summed weights of up to four points of one parcel stay below 4, so every
-// rounded contribution fits an int32
+// rounded contribution is below 2^29 and the sum of two (the two threads of
+// a cell) fits an int32
 __device__ inline float fixed_scale(float max_abs, int &e)
 {
     if (!(max_abs > 0.0f)) {
@@ -288,8 +293,12 @@ __device__ inline float fixed_scale(float max_abs, int &e)
         return 0.0f;
     }
     frexpf(max_abs, &e);   // max_abs < 2^e
-    return ldexpf(1.0f, 28 - e);
+    return ldexpf(1.0f, 27 - e);
 }
+
+// the value of the other lane of a lane pair (2k, 2k+1): DPP quad
+// permutation [1, 0, 3, 2]; both lanes must be active
+__device__ inline int pair_swap(int v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false); }
 
 // the gridded quantities of parcel pi, read from its arrays (o4 = 4 pi)
 __device__ inline void p2g_values(const Grid &g, const ParcelsDev &p, unsigned o4, double z, float v[P2G_NF])
@@ -393,9 +402,19 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     __syncthreads();   // the accumulators are cleared
 
     // ---- pass 2: support points, trilinear weights, integer accumulation ----
+    // the two threads of a cell take its parcels in turns and run the same
+    // number of rounds (the odd thread idles in the last round of an odd
+    // count), so that they can exchange their own-cell contributions
+    static_assert(P2G_TPC == 2, "lane pairs");
+    const bool odd = sub != 0;
+    const int cown = (lz + 1) * lplane + (cy + 1) * lp + cx + 1;   // own cell's lower corner
+    const int nown = cown + (odd ? lplane : 0);   // this lane's four corners: the lower (even) or upper (odd) plane
+    const int rounds = (s1 - s0 + 1) >> 1;
     int err = 0;
-    for (int s = s0 + sub; s < s1; s += P2G_TPC) {
-        const int pi = gl(a.perm)[s];
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int s = s0 + 2 * rd + sub;
+        const bool have = s < s1;
+        const int pi = have ? gl(a.perm)[s] : gl(a.perm)[s0];   // an idle lane reads a valid entry and adds nothing
         const unsigned o4 = (unsigned)pi * 4u;   // byte offset (pi < 2^30): scalar base + 32-bit offset addressing
         const double pos[3] = {gl(a.p.x)[pi], gl(a.p.y)[pi], gl(a.p.z)[pi]};
         const double ve[3] = {at(axes[0], o4), at(axes[1], o4), at(axes[2], o4)};
@@ -404,7 +423,7 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
         p2g_values(g, a.p, o4, pos[2], v);
         float cs[P2G_NF];
 #pragma unroll
-        for (int f = 0; f < P2G_NF; ++f) cs[f] = v[f] * sc[f];   // exact: sc is a power of two
+        for (int f = 0; f < P2G_NF; ++f) cs[f] = have ? v[f] * sc[f] : 0.0f;   // exact: sc is a power of two
         int cc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
@@ -426,16 +445,50 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
             }
             tgt[jp] = -1;
             if (off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1 ||
-                cc[2] + off[2] < -1 || cc[2] + off[2] > g.nz)
-                err |= ERR_OUT_OF_GRID;
-            else   // lower corner of the point's cell in the box
+                cc[2] + off[2] < -1 || cc[2] + off[2] > g.nz) {
+                if (have) err |= ERR_OUT_OF_GRID;
+            } else {   // lower corner of the point's cell in the box
                 tgt[jp] = (lz + off[2] + 1) * lplane + (cy + off[1] + 1) * lp + cx + off[0] + 1;
+            }
         }
-        // points in one cell share its 8 corners: their weights are summed
-        // first (in point order) and each distinct cell takes 8 x 7 atomics
+        // own cell: the summed weights (in point order) of the points in it,
+        // on this lane's four corners and on the other lane's four
+        {
+            float wn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, wf[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            bool any = false;
+#pragma unroll
+            for (int jp = 0; jp < 4; ++jp) {
+                if (tgt[jp] != cown) continue;
+                const float *f3 = fr[jp];
+                const float wxy[4] = {(1.0f - f3[1]) * (1.0f - f3[0]), (1.0f - f3[1]) * f3[0], f3[1] * (1.0f - f3[0]),
+                                      f3[1] * f3[0]};
+                const float zl = 1.0f - f3[2], zu = f3[2];
+                const float zn = odd ? zu : zl, zf = odd ? zl : zu;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    wn[c] = any ? wn[c] + zn * wxy[c] : zn * wxy[c];
+                    wf[c] = any ? wf[c] + zf * wxy[c] : zf * wxy[c];
+                }
+                any = true;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int node = nown + ((c >> 1) & 1) * lp + (c & 1);
+#pragma unroll
+                for (int f = 0; f < P2G_NF; ++f) {
+                    if ((f == 5 && !own5) || (f == 6 && !own6)) continue;   // uniform over the work-group
+                    const int qn = (int)rintf(wn[c] * cs[f]);
+                    const int qf = (int)rintf(wf[c] * cs[f]);
+                    const int q = qn + pair_swap(qf);   // |q| < 2^30
+                    atomicAdd(&acc[f * lfield + node], (unsigned long long)(long long)q);
+                }
+            }
+        }
+        // every other cell the points reach: its 8 corners take the summed
+        // weights of its points (in point order), 8 x 7 atomics per cell
 #pragma unroll
         for (int jp = 0; jp < 4; ++jp) {
-            bool lead = tgt[jp] >= 0;
+            bool lead = tgt[jp] >= 0 && tgt[jp] != cown && have;
 #pragma unroll
             for (int jq = 0; jq < jp; ++jq) lead = lead && (tgt[jq] != tgt[jp]);
             if (!lead) continue;
