@@ -123,6 +123,13 @@ def load_library(path=None):
         "epic_batch_prepare": ([h], i),
         "epic_batch_launch": ([h, dp, dp, dp], i),
         "epic_batch_wait": ([h, dp, ip], i),
+        "epic_nc_file_info": ([C.c_char_p, C.c_char_p, dp, P(i64), P(Grid)], i),
+        "epic_nc_read_fields": ([C.c_char_p, i, P(Grid), dp, dp, dp, dp], i),
+        "epic_read_fields_nc": ([h, C.c_char_p, i, dp], i),
+        "epic_parcel_default": ([h, i, d], i),
+        "epic_init_parcels_from_grids": ([h], i),
+        "epic_next_write_time": ([d, d], d),
+        "epic_setup_restart": ([h, C.c_char_p, dp, C.c_char_p], i),
         "epic_diagonalise": ([i64, dp, dp, dp], i),
         "epic_diagonalise_last_error": ([], C.c_char_p),
     }
@@ -360,6 +367,32 @@ class Member:
         """write_netcdf_fields (field_netcdf.f90:182-266)."""
         self._chk(load_library().epic_write_fields_nc(self._h, str(path).encode(), t))
 
+    def read_fields_nc(self, path, step=-1):
+        """read_netcdf_fields (field_netcdf.f90:306-386) into vortg, tbuoyg,
+        humg; returns the record's time."""
+        t = C.c_double(0)
+        self._chk(load_library().epic_read_fields_nc(self._h, str(path).encode(), int(step), C.byref(t)))
+        return t.value
+
+    # ---- parcel set-up and restart (parcel_init.f90, utils.f90) ----
+    def parcel_default(self, n_per_cell=8, lambda_max=4.0):
+        """parcel_default (parcel_init.f90:31-100)."""
+        self._chk(load_library().epic_parcel_default(self._h, int(n_per_cell), float(lambda_max)))
+        self.n = int(load_library().epic_num_parcels(self._h))
+
+    def init_parcels_from_grids(self):
+        """init_parcels_from_grids (parcel_init.f90:174-203)."""
+        self._chk(load_library().epic_init_parcels_from_grids(self._h))
+
+    def setup_restart(self, path):
+        """setup_restart + the restart branch of setup_fields_and_parcels
+        (utils.f90:144-216); returns (t, file_type)."""
+        t = C.c_double(0)
+        ft = C.create_string_buffer(16)
+        self._chk(load_library().epic_setup_restart(self._h, str(path).encode(), C.byref(t), ft))
+        self.n = int(load_library().epic_num_parcels(self._h))
+        return t.value, ft.value.decode()
+
     # ---- physical re-sort ----
     def resort(self):
         """Reorder the parcel arrays into cell order (not observable)."""
@@ -400,6 +433,32 @@ def diagonalise(A):
     if rc != 0:
         raise EpicError(rc, L.epic_diagonalise_last_error().decode())
     return D, np.ascontiguousarray(np.transpose(V, (0, 2, 1)))
+
+
+def next_write_time(freq, t):
+    """get_next_write_time (utils.f90:48-54)."""
+    return load_library().epic_next_write_time(float(freq), float(t))
+
+
+def nc_file_info(path):
+    """(file_type, last time, number of records, (nx, ny, nz, lower, extent))
+    of a file (netcdf_reader.f90 get_file_type / get_time / get_num_steps)."""
+    ft = C.create_string_buffer(16)
+    t, n, g = C.c_double(0), C.c_int64(0), Grid()
+    _nc_chk(load_library().epic_nc_file_info(str(path).encode(), ft, C.byref(t), C.byref(n), C.byref(g)))
+    return ft.value.decode(), t.value, n.value, (g.nx, g.ny, g.nz, tuple(g.lower), tuple(g.extent))
+
+
+def nc_read_fields(path, grid, step=-1):
+    """Field file to halo-layout host arrays (epic_nc_read_fields): returns
+    (dict vortg (3,H), tbuoyg, humg, t)."""
+    nx, ny, nz = grid[0], grid[1], grid[2]
+    g = Grid(nx, ny, nz, (C.c_double * 3)(*grid[3]), (C.c_double * 3)(*grid[4]))
+    H = (nx + 3, ny + 3, nz + 3)
+    vor, tb, hm, t = np.zeros((3,) + H), np.zeros(H), np.zeros(H), C.c_double(0)
+    _nc_chk(load_library().epic_nc_read_fields(str(path).encode(), int(step), C.byref(g), _dp(vor), _dp(tb), _dp(hm),
+                                                C.byref(t)))
+    return {"vortg": vor, "tbuoyg": tb, "humg": hm}, t.value
 
 
 def _nc_chk(rc):

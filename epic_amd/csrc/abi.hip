@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "epic_gpu.h"
@@ -57,8 +58,9 @@ struct epic_member {
     int *nparg = nullptr, *nsparg = nullptr;
     float *staging = nullptr;
     // spectra
-    float2 *spec[NSPEC] = {};
+    float2 *spec[NSPEC] = {};   // NSPEC consecutive spectra of one allocation (spec[0])
     float2 *twx = nullptr, *twy = nullptr;
+    double2 *twx64 = nullptr, *twy64 = nullptr;
     double2 *twz = nullptr;
     FftPlan px{}, py{}, pz{};
     // per-phase GPU timers (events on the member stream)
@@ -146,6 +148,26 @@ template <typename T> hipError_t dalloc(T **p, int64_t count, hipStream_t s)
     hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)count);
     if (e == hipSuccess) e = hipMemsetAsync(*p, 0, sizeof(T) * (size_t)count, s);
     return e;
+}
+
+// fn(lo, hi) over [0, n) on up to 16 host threads (the per-parcel layout
+// conversions of set/get; each element is written by one thread)
+template <class F> void host_parallel_for(int64_t n, F fn)
+{
+    const int64_t grain = 1 << 16;
+    unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)std::min<int64_t>(std::min<unsigned>(hw ? hw : 1, 16u), (n + grain - 1) / grain);
+    if (nt <= 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; ++t) {
+        const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+        th.emplace_back([=, &fn]() { fn(lo, hi); });
+    }
+    for (std::thread &x : th) x.join();
 }
 
 bool factor23(int n)
@@ -428,6 +450,7 @@ FftXArgs xargs(epic_member *m)
     a.g = m->g;
     a.plan = m->px;
     a.tw = m->twx;
+    a.tw64 = m->twx64;
     a.fld = m->fld;
     a.fld_out = m->fld;
     return a;
@@ -439,6 +462,7 @@ FftYArgs yargs(epic_member *m)
     a.g = m->g;
     a.plan = m->py;
     a.tw = m->twy;
+    a.tw64 = m->twy64;
     return a;
 }
 
@@ -496,41 +520,50 @@ void build_vor2vel(Plan &P, const Members &ms)
 }
 
 // vorticity_tendency: six flux transforms, two combined inverse transforms
-// per component (the z derivative is added in the last pass)
+// per component (the z derivative is added in the last pass), all in FP64
+// from the FP32 node fields: the spectral derivative amplifies white
+// rounding noise at the highest wavenumbers, which FP32 products and
+// transforms would add.  The FP64 spectrum q takes the float2 spectra 2q and
+// 2q + 1 (one allocation).
 void build_vtend(Plan &P, const Members &ms)
 {
     std::vector<FftXArgs> xf, xi;
     std::vector<FftYArgs> yf, yi;
     std::vector<FixArgs> fx;
     for (epic_member *m : ms) {
+        auto sp64 = [&](int q) { return (void *)m->spec[2 * q]; };
         for (int c = 0; c < 3; ++c)
             for (int a = 0; a < 2; ++a) {
                 FftXArgs x = xargs(m);
+                x.fp64 = 1;
                 x.mode = 1;
                 x.comp_a = a;
                 x.comp_b = c;
                 // f_x of the y tendency carries -b, f_y of the x tendency +b (inversion.f90:327-335)
                 x.sgn = (a == 0 && c == 1) ? -1 : ((a == 1 && c == 0) ? 1 : 0);
-                x.out = m->spec[2 * c + a];
+                x.out = sp64(2 * c + a);
                 xf.push_back(x);
                 FftYArgs y = yargs(m);
+                y.fp64 = 1;
                 y.mode = 0;
-                y.in = m->spec[2 * c + a];
-                y.out = m->spec[2 * c + a];
+                y.in = sp64(2 * c + a);
+                y.out = sp64(2 * c + a);
                 yf.push_back(y);
             }
         for (int c = 0; c < 3; ++c) {
             FftYArgs y = yargs(m);
+            y.fp64 = 1;
             y.mode = 2;
-            y.in = m->spec[2 * c];
-            y.in2 = m->spec[2 * c + 1];
-            y.out = m->spec[2 * c];
+            y.in = sp64(2 * c);
+            y.in2 = sp64(2 * c + 1);
+            y.out = sp64(2 * c);
             yi.push_back(y);
             FftXArgs x = xargs(m);
+            x.fp64 = 1;
             x.mode = 2;
             x.comp_b = c;
             x.slot = F_VTEND + c;
-            x.in = m->spec[2 * c];
+            x.in = sp64(2 * c);
             xi.push_back(x);
         }
         fx.push_back(FixArgs{m->g, m->fld});
@@ -770,7 +803,7 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     g.ny = grid->ny;
     g.nz = grid->nz;
     g.nxh = g.nx / 2 + 1;
-    g.nzp = ((g.nz + 1 + ZB - 1) / ZB) * ZB;
+    g.nzp = ((g.nz + 3 + ZB - 1) / ZB) * ZB;   // planes 0..nz and two halo planes
     g.zeta_zero[0] = flags->zeta_bndry_zero[0] != 0;
     g.zeta_zero[1] = flags->zeta_bndry_zero[1] != 0;
     const int n3[3] = {g.nx, g.ny, g.nz};
@@ -815,8 +848,9 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     if (bad(dalloc(&m->fld, (int64_t)F_COUNT * FS, m->s)) || bad(dalloc(&m->nparg, FS, m->s)) || bad(dalloc(&m->nsparg, FS, m->s)) ||
         bad(dalloc(&m->staging, staging_floats_max(g), m->s)))
         return EPIC_E_NOMEM;
-    for (int i = 0; i < NSPEC; ++i)
-        if (bad(dalloc(&m->spec[i], spec_size(g), m->s))) return EPIC_E_NOMEM;
+    static_assert(NSPEC >= 12, "the six FP64 spectra of vorticity_tendency take twelve float2 spectra");
+    if (bad(dalloc(&m->spec[0], NSPEC * spec_size(g), m->s))) return EPIC_E_NOMEM;
+    for (int i = 1; i < NSPEC; ++i) m->spec[i] = m->spec[0] + i * spec_size(g);
     if (bad(dalloc(&m->err_d, 4, m->s)) || bad(dalloc(&m->flags_d, 4, m->s)) || bad(dalloc(&m->tm_d, 1, m->s))) return EPIC_E_NOMEM;
     if (bad(dalloc(&m->pmax_out, P2G_NF + 1, m->s))) return EPIC_E_NOMEM;
     if (bad(hipHostMalloc((void **)&m->err_h, sizeof(int) * 4, hipHostMallocDefault)) ||
@@ -833,9 +867,16 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     for (int t = 0; t < g.nx; ++t) tx[t] = float2{(float)std::cos(tp * t / g.nx), (float)-std::sin(tp * t / g.nx)};
     for (int t = 0; t < g.ny; ++t) ty[t] = float2{(float)std::cos(tp * t / g.ny), (float)-std::sin(tp * t / g.ny)};
     for (int t = 0; t < 2 * g.nz; ++t) tz[t] = double2{std::cos(tp * t / (2 * g.nz)), -std::sin(tp * t / (2 * g.nz))};
-    if (bad(dalloc(&m->twx, g.nx, m->s)) || bad(dalloc(&m->twy, g.ny, m->s)) || bad(dalloc(&m->twz, 2 * g.nz, m->s))) return EPIC_E_NOMEM;
+    std::vector<double2> tx64(g.nx), ty64(g.ny);
+    for (int t = 0; t < g.nx; ++t) tx64[t] = double2{std::cos(tp * t / g.nx), -std::sin(tp * t / g.nx)};
+    for (int t = 0; t < g.ny; ++t) ty64[t] = double2{std::cos(tp * t / g.ny), -std::sin(tp * t / g.ny)};
+    if (bad(dalloc(&m->twx, g.nx, m->s)) || bad(dalloc(&m->twy, g.ny, m->s)) || bad(dalloc(&m->twz, 2 * g.nz, m->s)) ||
+        bad(dalloc(&m->twx64, g.nx, m->s)) || bad(dalloc(&m->twy64, g.ny, m->s)))
+        return EPIC_E_NOMEM;
     if (bad(hipMemcpyAsync(m->twx, tx.data(), sizeof(float2) * tx.size(), hipMemcpyHostToDevice, m->s)) ||
         bad(hipMemcpyAsync(m->twy, ty.data(), sizeof(float2) * ty.size(), hipMemcpyHostToDevice, m->s)) ||
+        bad(hipMemcpyAsync(m->twx64, tx64.data(), sizeof(double2) * tx64.size(), hipMemcpyHostToDevice, m->s)) ||
+        bad(hipMemcpyAsync(m->twy64, ty64.data(), sizeof(double2) * ty64.size(), hipMemcpyHostToDevice, m->s)) ||
         bad(hipMemcpyAsync(m->twz, tz.data(), sizeof(double2) * tz.size(), hipMemcpyHostToDevice, m->s)) ||
         bad(hipStreamSynchronize(m->s)))
         return EPIC_E_HIP;
@@ -855,12 +896,11 @@ int epic_destroy(epic_handle m)
     m->invalidate_plans();
     void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
                     m->cnt[2], m->scnt[2], m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
-                    m->twx, m->twy, m->twz, m->err_d, m->flags_d, m->vw_d, m->tm_d, m->diag_d,
+                    m->twx, m->twy, m->twz, m->twx64, m->twy64, m->err_d, m->flags_d, m->vw_d, m->tm_d, m->diag_d,
                     m->sort_tmp, m->pmax_out, m->tend_src};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    for (int i = 0; i < NSPEC; ++i)
-        if (m->spec[i]) (void)hipFree(m->spec[i]);
+    if (m->spec[0]) (void)hipFree(m->spec[0]);
     if (m->err_h) (void)hipHostFree(m->err_h);
     if (m->vw_h) (void)hipHostFree(m->vw_h);
     if (m->tm_h) (void)hipHostFree(m->tm_h);
@@ -887,15 +927,17 @@ int epic_set_parcels(epic_handle m, int64_t n, const double *position, const dou
     std::vector<double> hp(3 * (size_t)n);
     std::vector<float> ha((size_t)11 * n);
     std::vector<int> ho(n);
-    for (int64_t q = 0; q < n; ++q) {
-        for (int d = 0; d < 3; ++d) hp[d * n + q] = position[3 * q + d];
-        for (int d = 0; d < 3; ++d) ha[d * n + q] = (float)vorticity[3 * q + d];
-        for (int d = 0; d < 5; ++d) ha[(3 + d) * n + q] = (float)B[5 * q + d];
-        ha[8 * n + q] = (float)volume[q];
-        ha[9 * n + q] = (float)buoyancy[q];
-        ha[10 * n + q] = (float)humidity[q];
-        ho[q] = (int)q;
-    }
+    host_parallel_for(n, [&](int64_t lo, int64_t hi) {
+        for (int64_t q = lo; q < hi; ++q) {
+            for (int d = 0; d < 3; ++d) hp[d * n + q] = position[3 * q + d];
+            for (int d = 0; d < 3; ++d) ha[d * n + q] = (float)vorticity[3 * q + d];
+            for (int d = 0; d < 5; ++d) ha[(3 + d) * n + q] = (float)B[5 * q + d];
+            ha[8 * n + q] = (float)volume[q];
+            ha[9 * n + q] = (float)buoyancy[q];
+            ha[10 * n + q] = (float)humidity[q];
+            ho[q] = (int)q;
+        }
+    });
     m->n = n;
     bind_parcels(m);
     m->invalidate_plans();   // the plans hold the parcel count
@@ -926,18 +968,20 @@ int epic_get_parcels(epic_handle m, int64_t n, double *position, double *vortici
         HIPOK(m, hipMemcpyAsync(ha.data() + a * n, m->attr + a * c, sizeof(float) * n, hipMemcpyDeviceToHost, m->s));
     if (n) HIPOK(m, hipMemcpyAsync(ho.data(), m->orig, sizeof(int) * n, hipMemcpyDeviceToHost, m->s));
     HIPOK(m, hipStreamSynchronize(m->s));
-    for (int64_t q = 0; q < n; ++q) {
-        const int64_t o = ho[q];
-        if (position)
-            for (int d = 0; d < 3; ++d) position[3 * o + d] = hp[d * n + q];
-        if (vorticity)
-            for (int d = 0; d < 3; ++d) vorticity[3 * o + d] = ha[d * n + q];
-        if (B)
-            for (int d = 0; d < 5; ++d) B[5 * o + d] = ha[(3 + d) * n + q];
-        if (volume) volume[o] = ha[8 * n + q];
-        if (buoyancy) buoyancy[o] = ha[9 * n + q];
-        if (humidity) humidity[o] = ha[10 * n + q];
-    }
+    host_parallel_for(n, [&](int64_t lo, int64_t hi) {   // orig is a permutation: one writer per element
+        for (int64_t q = lo; q < hi; ++q) {
+            const int64_t o = ho[q];
+            if (position)
+                for (int d = 0; d < 3; ++d) position[3 * o + d] = hp[d * n + q];
+            if (vorticity)
+                for (int d = 0; d < 3; ++d) vorticity[3 * o + d] = ha[d * n + q];
+            if (B)
+                for (int d = 0; d < 5; ++d) B[5 * o + d] = ha[(3 + d) * n + q];
+            if (volume) volume[o] = ha[8 * n + q];
+            if (buoyancy) buoyancy[o] = ha[9 * n + q];
+            if (humidity) humidity[o] = ha[10 * n + q];
+        }
+    });
     return EPIC_OK;
 }
 
@@ -1429,7 +1473,7 @@ int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels)
     g.ny = grid->ny;
     g.nz = grid->nz;
     g.nxh = g.nx / 2 + 1;
-    g.nzp = ((g.nz + 1 + ZB - 1) / ZB) * ZB;
+    g.nzp = ((g.nz + 3 + ZB - 1) / ZB) * ZB;   // planes 0..nz and two halo planes
     const int64_t c = std::max<int64_t>(max_parcels, 1), nc = ncells(g), FS = field_stride(g);
     int64_t b = 0;
     b += c * (3 * 8 + (int64_t)NATTR * 4 + 4 + 12 * 4 + 4 + 4) + 4;           // parcels, axes, orig, keys, perm
@@ -1925,6 +1969,166 @@ int epic_field_diagnostics(epic_handle m, double *stats)
     stats[9] = s[8];
     stats[10] = s[9];
     return EPIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// parcel set-up from gridded fields and restart (parcel_init.f90,
+// utils.f90:144-249)
+// ---------------------------------------------------------------------------
+namespace {
+
+// init_parcels_from_grids (parcel_init.f90:174-203): every parcel adds the
+// trilinear interpolation (interpl.f90:17-56, FP64 weights, the sum in the
+// reference's element order) of vortg, tbuoyg and humg at its centre to
+// its vorticity, buoyancy and humidity
+__global__ __launch_bounds__(256) void k_init_from_grids(Grid g, ParcelsDev p, const float *__restrict__ fld, int *err)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p.n) return;
+    const double pos[3] = {p.x[q], p.y[q], p.z[q]};
+    int ijk[3];
+    double fr[3];
+    for (int d = 0; d < 3; ++d) {
+        const double xyz = (pos[d] - g.lower[d]) * g.dxi[d];
+        const double fl = floor(xyz);
+        ijk[d] = (int)fl;
+        fr[d] = xyz - fl;
+    }
+    if (ijk[2] < -1 || ijk[2] > g.nz) {
+        atomicOr(err, ERR_OUT_OF_GRID);
+        return;
+    }
+    const int64_t FS = field_stride(g), PS = plane_stride(g);
+    int64_t off[8];
+    double w[8];
+    // Fortran element order of weights(0:1, 0:1, 0:1): k fastest, then j, then i
+    for (int ii = 0; ii < 2; ++ii)
+        for (int jj = 0; jj < 2; ++jj)
+            for (int kk = 0; kk < 2; ++kk) {
+                const int m = (ii * 2 + jj) * 2 + kk;
+                const int i = wrap(ijk[0] + ii, g.nx), j = wrap(ijk[1] + jj, g.ny);
+                off[m] = (int64_t)(ijk[2] + kk + 1) * PS + (int64_t)j * g.nx + i;
+                w[m] = (ii ? fr[0] : 1.0 - fr[0]) * (jj ? fr[1] : 1.0 - fr[1]) * (kk ? fr[2] : 1.0 - fr[2]);
+            }
+    auto interp = [&](int slot) {
+        double s = 0.0;
+        for (int m = 0; m < 8; ++m) s += w[m] * (double)fld[slot * FS + off[m]];
+        return s;
+    };
+    for (int c = 0; c < 3; ++c) p.vor[c][q] = (float)((double)p.vor[c][q] + interp(F_VORTG + c));
+    p.buoy[q] = (float)((double)p.buoy[q] + interp(F_TBUOYG));
+    p.hum[q] = (float)((double)p.hum[q] + interp(F_HUMG));
+}
+
+}  // namespace
+
+int epic_parcel_default(epic_handle m, int n_per_cell, double lambda_max)
+{
+    if (!m) return EPIC_E_INVALID;
+    const Grid &g = m->g;
+    if (n_per_cell <= 0) n_per_cell = 8;   // options.f90:88
+    if (!(lambda_max > 0.0)) lambda_max = 4.0;   // options.f90:89
+    int npd = (int)std::lround(std::cbrt((double)n_per_cell));
+    if (npd * npd * npd != n_per_cell)
+        return fail(m, EPIC_E_INVALID, "epic_parcel_default: Number of parcels per cell not a cubic.");
+    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz, n = n_per_cell * ncell;
+    if (n > m->cap) return fail(m, EPIC_E_CAPACITY, "epic_parcel_default: Number of parcels exceeds limit of max_parcels.");
+    // init_regular_positions (parcel_init.f90:104-152): cells iz, iy, ix
+    // (ix fastest), then k, j, i in the cell (i fastest) at (i - 1/2)/npd
+    std::vector<double> pos(3 * (size_t)n), vor(3 * (size_t)n, 0.0), B(5 * (size_t)n, 0.0), vol((size_t)n),
+        zero((size_t)n, 0.0);
+    const double im = 1.0 / npd;
+    // aspect ratio lam = a / c from the cell (parcel_init.f90:60-63)
+    const double lam = std::max({g.dx[0] / g.dx[1], g.dx[1] / g.dx[0], g.dx[0] / g.dx[2], g.dx[2] / g.dx[0],
+                                 g.dx[1] / g.dx[2], g.dx[2] / g.dx[1]});
+    const double v = g.vcell / n_per_cell;
+    const double abc = 0.75 * v / 3.14159265358979323846;   // get_abc (parcel_ellipsoid.f90:409-415)
+    const double l23 = std::pow(lam * abc, 2.0 / 3.0);
+    host_parallel_for(ncell, [&](int64_t lo, int64_t hi) {
+        for (int64_t c = lo; c < hi; ++c) {
+            const int ix = (int)(c % g.nx), iy = (int)((c / g.nx) % g.ny), iz = (int)(c / ((int64_t)g.nx * g.ny));
+            const double corner[3] = {g.lower[0] + ix * g.dx[0], g.lower[1] + iy * g.dx[1], g.lower[2] + iz * g.dx[2]};
+            int64_t l = c * n_per_cell;
+            for (int k = 1; k <= npd; ++k)
+                for (int j = 1; j <= npd; ++j)
+                    for (int i = 1; i <= npd; ++i, ++l) {
+                        pos[3 * l + 0] = corner[0] + g.dx[0] * (i - 0.5) * im;
+                        pos[3 * l + 1] = corner[1] + g.dx[1] * (j - 0.5) * im;
+                        pos[3 * l + 2] = corner[2] + g.dx[2] * (k - 0.5) * im;
+                        vol[l] = v;
+                        B[5 * l + 0] = l23;
+                        B[5 * l + 3] = l23;
+                    }
+        }
+    });
+    int rc = epic_set_parcels(m, n, pos.data(), vor.data(), B.data(), vol.data(), zero.data(), zero.data());
+    if (rc) return rc;
+    // init_refine (parcel_init.f90:156-166): split while the aspect ratio
+    // reaches lambda_max; every parcel has the same shape
+    double lr = lam;
+    while (lr >= lambda_max) {
+        int64_t ns = 0;
+        rc = epic_parcel_split(m, lambda_max, -1.0, &ns);
+        if (rc) return rc;
+        if (ns == 0) break;
+        // the eigenvalues of parcel 1's B (get_eigenvalues): from its matrix
+        std::vector<double> all_b(5 * (size_t)m->n), all_p(3 * (size_t)m->n), all_v(3 * (size_t)m->n),
+            all_vol((size_t)m->n), all_bu((size_t)m->n), all_h((size_t)m->n);
+        rc = epic_get_parcels(m, m->n, all_p.data(), all_v.data(), all_b.data(), all_vol.data(), all_bu.data(),
+                              all_h.data());
+        if (rc) return rc;
+        const double *bb = all_b.data();
+        const double a2 = 0.75 * all_vol[0] / 3.14159265358979323846;
+        const double den = bb[0] * bb[3] - bb[1] * bb[1];
+        const double b33 = (a2 * a2 - bb[2] * (bb[1] * bb[4] - bb[2] * bb[3]) + bb[0] * bb[4] * bb[4] -
+                            bb[1] * bb[2] * bb[4]) / den;
+        // largest / smallest eigenvalue of the symmetric 3x3 matrix
+        double A[9] = {bb[0], bb[1], bb[2], bb[1], bb[3], bb[4], bb[2], bb[4], b33}, D[3], V[9];
+        rc = epic_diagonalise(1, A, D, V);
+        if (rc) return fail(m, rc, epic_diagonalise_last_error());
+        lr = std::sqrt(D[0] / D[2]);
+    }
+    return EPIC_OK;
+}
+
+int epic_init_parcels_from_grids(epic_handle m)
+{
+    if (!m) return EPIC_E_INVALID;
+    if (m->n == 0) return EPIC_OK;
+    const int rc = settle_tendencies(m);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_init_from_grids, dim3((unsigned)((m->n + 255) / 256)), dim3(256), 0, m->s, m->g, m->pd,
+                       (const float *)m->fld, m->err_d);
+    HIPOK(m, hipGetLastError());
+    return check(m, "epic_init_parcels_from_grids");
+}
+
+double epic_next_write_time(double freq, double t)
+{
+    // get_next_write_time (utils.f90:48-54): freq (int(t / freq) + 1)
+    return freq * (double)((long long)(t / freq) + 1);
+}
+
+int epic_setup_restart(epic_handle m, const char *path, double *t, char *file_type)
+{
+    if (!m || !path) return EPIC_E_INVALID;
+    char ft[16];
+    double tt = 0.0;
+    // setup_restart (utils.f90:144-160): the file type and the last time
+    int rc = epic_nc_file_info(path, ft, &tt, nullptr, nullptr);
+    if (rc) return fail(m, rc, std::string("epic_setup_restart: ") + epic_nc_last_error());
+    if (file_type) std::memcpy(file_type, ft, sizeof(ft));
+    if (t) *t = tt;
+    // setup_fields_and_parcels (utils.f90:200-216)
+    if (std::strcmp(ft, "fields") == 0) {
+        rc = epic_parcel_default(m, 8, 4.0);
+        if (rc) return rc;
+        rc = epic_read_fields_nc(m, path, -1, nullptr);
+        if (rc) return rc;
+        return epic_init_parcels_from_grids(m);
+    }
+    if (std::strcmp(ft, "parcels") == 0) return epic_read_parcels_nc(m, path, nullptr);
+    return fail(m, EPIC_E_INVALID, "epic_setup_restart: Restart file must be of type \"fields\" or \"parcels\".");
 }
 
 int epic_get_config(epic_handle m, epic_grid_t *grid, epic_phys_t *phys, epic_flags_t *flags)

@@ -62,7 +62,7 @@ enum { DUDX = 0, DUDY, DUDZ, DVDX, DVDY, DVDZ, DWDX, DWDY };
 struct Grid {
     int nx, ny, nz;
     int nxh;          // nx/2 + 1
-    int nzp;          // nz+1 rounded up to ZB
+    int nzp;          // nz+3 rounded up to ZB: planes 0..nz and the two halo planes
     int zeta_zero[2];
     double lower[3], extent[3], upper[3], dx[3], dxi[3], center[3], hli[3];
     double vcell, vmin;

@@ -375,11 +375,16 @@ __global__ __launch_bounds__(256) void k_vtend_fixup(const FixArgs *__restrict__
     const int64_t FS = field_stride(g), PS = plane_stride(g);
     const int nz = g.nz;
     auto F = [&](int slot, int k) -> EPIC_GLOBAL float & { return gl(fld)[slot * FS + (int64_t)(k + 1) * PS + col]; };
-    if (g.zeta_zero[0]) F(F_VTEND + 2, 0) = 0.0f;
-    if (g.zeta_zero[1]) F(F_VTEND + 2, nz) = 0.0f;
-    for (int c = 0; c < 3; ++c) {
-        F(F_VTEND + c, -1) = 2.0f * F(F_VTEND + c, 0) - F(F_VTEND + c, 1);
-        F(F_VTEND + c, nz + 1) = 2.0f * F(F_VTEND + c, nz) - F(F_VTEND + c, nz - 1);
+    // the transforms wrote the halo planes as the extrapolations 2 v(0) -
+    // v(1) and 2 v(nz) - v(nz-1) of the unmodified tendency; where dzeta/dt
+    // is held at zero on a wall, that extrapolation is -v(1) (-v(nz-1))
+    if (g.zeta_zero[0]) {
+        F(F_VTEND + 2, 0) = 0.0f;
+        F(F_VTEND + 2, -1) = -F(F_VTEND + 2, 1);
+    }
+    if (g.zeta_zero[1]) {
+        F(F_VTEND + 2, nz) = 0.0f;
+        F(F_VTEND + 2, nz + 1) = -F(F_VTEND + 2, nz - 1);
     }
 }
 

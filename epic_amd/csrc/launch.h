@@ -114,26 +114,34 @@ void add_grid2par(Plan &P, int phase, const std::vector<G2PArgs> &items);
 void add_rk_stage_tiles(Plan &P, int phase, const std::vector<G2PTileArgs> &items);
 
 // ---- horizontal FFTs (fft.hip) ----
+// fp64: spectra and arithmetic in FP64 (double2 spectra, twiddles tw64);
+// every item of one launch has the same precision.  The FP64 transforms are
+// the vorticity tendency's (modes 1 forward and 2 inverse) and carry the
+// halo planes -1 and nz + 1 at z indices nz + 1 and nz + 2.
 struct FftXArgs {
     Grid g;
     FftPlan plan;          // complex length nx/2
     const float2 *tw;      // e^{-2 pi i t / nx}, t < nx
+    const double2 *tw64;   // the same in FP64
+    int fp64;
     int mode;              // forward: 0 field, 1 tendency flux, 3 volg / vcell - 1;
                            // inverse: 0 store, 2 store + d/dz flux
     int slot;              // field slot read (forward, mode 0) or written (inverse)
     int comp_a, comp_b, sgn;
     const float *fld;      // node fields read
     float *fld_out;        // node fields written (inverse)
-    float2 *out;           // forward: T[j][kx][z]
-    const float2 *in;      // inverse: T[j][kx][z]
+    void *out;             // forward: T[j][kx][z] (float2 or double2)
+    const void *in;        // inverse: T[j][kx][z]
 };
 struct FftYArgs {
     Grid g;
     FftPlan plan;          // length ny
     const float2 *tw;      // e^{-2 pi i t / ny}
+    const double2 *tw64;
+    int fp64;
     int mode;              // 0 forward, 1 inverse, 2 inverse of i kx in + i ky in2
-    const float2 *in, *in2;
-    float2 *out;           // may equal in
+    const void *in, *in2;
+    void *out;             // may equal in
 };
 void add_fft_xf(Plan &P, int phase, const std::vector<FftXArgs> &items);
 void add_fft_y(Plan &P, int phase, const std::vector<FftYArgs> &items);

@@ -18,6 +18,7 @@
 // epic_set_parcels).
 #include "epic_gpu.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -792,6 +793,108 @@ int epic_nc_write_fields(const char *path, const epic_grid_t *grid, const epic_p
     add("humidity", "specific humidity", "kg/kg", NC_DOUBLE, interior(humg, 0));
     add("volume", "volume", "m^3", NC_DOUBLE, interior(volg, 0));
     return write_file(path, w, data);
+}
+
+int epic_nc_file_info(const char *path, char *file_type, double *t, int64_t *n_steps, epic_grid_t *grid)
+{
+    if (!path) return fail("epic_nc_file_info: invalid argument");
+    Reader r;
+    if (!r.open(path)) return fail(std::string("epic_nc_file_info: ") + path + ": " + r.err);
+    if (file_type) {   // get_file_type (netcdf_reader.f90): the global attribute 'file_type'
+        file_type[0] = '\0';
+        const Att *a = r.gatt("file_type");
+        if (a && a->type == NC_CHAR) {
+            const size_t n = std::min<size_t>(a->data.size(), 15);
+            std::memcpy(file_type, a->data.data(), n);
+            file_type[n] = '\0';
+        }
+    }
+    if (n_steps) *n_steps = r.numrecs;   // get_num_steps
+    if (t) {   // get_time: the last record's time
+        *t = 0.0;
+        const int vt = r.find_var("t");
+        std::vector<double> tv;
+        if (vt >= 0 && r.numrecs > 0 && r.read(vt, r.numrecs - 1, tv) && !tv.empty()) *t = tv[0];
+    }
+    if (grid) {   // get_netcdf_box; nx = ny = nz = 0 without the box attributes
+        double nc[3] = {0, 0, 0}, ext[3] = {0, 0, 0}, org[3] = {0, 0, 0};
+        if (!Reader::att_values(r.gatt("ncells"), nc, 3) || !Reader::att_values(r.gatt("extent"), ext, 3) ||
+            !Reader::att_values(r.gatt("origin"), org, 3))
+            nc[0] = nc[1] = nc[2] = ext[0] = ext[1] = ext[2] = org[0] = org[1] = org[2] = 0.0;
+        grid->nx = (int)nc[0];
+        grid->ny = (int)nc[1];
+        grid->nz = (int)nc[2];
+        for (int d = 0; d < 3; ++d) {
+            grid->extent[d] = ext[d];
+            grid->lower[d] = org[d];
+        }
+    }
+    return EPIC_OK;
+}
+
+int epic_nc_read_fields(const char *path, int step, const epic_grid_t *grid, double *vortg, double *tbuoyg,
+                        double *humg, double *t)
+{
+    if (!path || !grid || !vortg || !tbuoyg || !humg) return fail("epic_nc_read_fields: invalid argument");
+    Reader r;
+    if (!r.open(path)) return fail(std::string("epic_nc_read_fields: ") + path + ": " + r.err);
+    const int64_t n_steps = r.numrecs;
+    int64_t st = step;
+    if (st == -1) st = n_steps;
+    else if (st == 0 || st > n_steps) return fail("epic_nc_read_fields: Step number in NetCDF field file is out of bounds.");
+    const int64_t rec = st - 1;
+    const int nx = grid->nx, ny = grid->ny, nz = grid->nz;
+    const int64_t H = (int64_t)(nz + 3) * (ny + 3) * (nx + 3), N = (int64_t)(nz + 1) * ny * nx;
+    if (t) {
+        *t = 0.0;
+        const int vt = r.find_var("t");
+        std::vector<double> tv;
+        if (vt >= 0 && r.read(vt, rec, tv) && !tv.empty()) *t = tv[0];
+    }
+    // a present dataset fills planes 0..nz of its field; an absent one
+    // leaves the field zero (field_default)
+    struct Ds {
+        const char *name;
+        double *dst;
+        int comp;
+    } ds[] = {{"x_vorticity", vortg, 0}, {"y_vorticity", vortg, 1}, {"z_vorticity", vortg, 2},
+              {"buoyancy", tbuoyg, 0}, {"humidity", humg, 0}};
+    std::vector<double> v;
+    for (const Ds &d : ds) {
+        double *f = d.dst + d.comp * H;
+        for (int64_t i = 0; i < H; ++i) f[i] = 0.0;
+        const int vi = r.find_var(d.name);
+        if (vi < 0) continue;
+        if (!r.read(vi, r.vars[vi].rec ? rec : 0, v) || (int64_t)v.size() != N)
+            return fail(std::string("epic_nc_read_fields: ") + (r.err.empty() ? std::string(d.name) + ": wrong size" : r.err));
+        for (int iz = 0; iz <= nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix)
+                    f[((int64_t)(ix + 1) * (ny + 3) + (iy + 1)) * (nz + 3) + (iz + 1)] =
+                        v[((size_t)iz * ny + iy) * nx + ix];
+    }
+    return EPIC_OK;
+}
+
+int epic_read_fields_nc(epic_handle h, const char *path, int step, double *t)
+{
+    if (!h || !path) return EPIC_E_INVALID;
+    epic_grid_t g, gf;
+    epic_get_config(h, &g, nullptr, nullptr);
+    int rc = epic_nc_file_info(path, nullptr, nullptr, nullptr, &gf);
+    if (rc) return epic_internal_fail(h, rc, g_err);
+    for (int d = 0; d < 3 && gf.nx > 0; ++d)   // a file with a box must match the member's grid
+        if ((&gf.nx)[d] != (&g.nx)[d] || std::fabs(gf.extent[d] - g.extent[d]) > 1e-12 * std::fabs(g.extent[d]) ||
+            std::fabs(gf.lower[d] - g.lower[d]) > 1e-12 * (std::fabs(g.lower[d]) + std::fabs(g.extent[d])))
+            return epic_internal_fail(h, EPIC_E_INVALID, "epic_read_fields_nc: the file's box differs from the member's grid");
+    const int64_t H = (int64_t)(g.nz + 3) * (g.ny + 3) * (g.nx + 3);
+    std::vector<double> vor(3 * H), tb(H), hm(H);
+    rc = epic_nc_read_fields(path, step, &g, vor.data(), tb.data(), hm.data(), t);
+    if (rc) return epic_internal_fail(h, rc, g_err);
+    if ((rc = epic_set_field(h, EPIC_FIELD_VORTG, vor.data())) || (rc = epic_set_field(h, EPIC_FIELD_TBUOYG, tb.data())) ||
+        (rc = epic_set_field(h, EPIC_FIELD_HUMG, hm.data())))
+        return rc;
+    return EPIC_OK;
 }
 
 int epic_write_fields_nc(epic_handle h, const char *path, double t)

@@ -258,6 +258,42 @@ int epic_read_parcels_nc(epic_handle h, const char *path, double *t);
 /* write_netcdf_fields(t) (field_netcdf.f90:182-266): the node fields of
  * planes 0..nz with the coordinate axes x, y, z and one time record */
 int epic_write_fields_nc(epic_handle h, const char *path, double t);
+/* get_file_type, get_time (the last record's t), get_num_steps and
+ * get_netcdf_box of a file (netcdf_reader.f90; utils.f90:144-160);
+ * file_type (16 chars) is "fields" or "parcels"; any pointer may be NULL */
+int epic_nc_file_info(const char *path, char *file_type, double *t, int64_t *n_steps, epic_grid_t *grid);
+/* read_netcdf_fields(fname, step) (field_netcdf.f90:306-386) into host
+ * arrays in the halo layout of epic_get_field: x/y/z_vorticity, buoyancy
+ * (total) and humidity of record `step` (1-based, -1 the last) on planes
+ * 0..nz; an absent dataset leaves its field zero */
+int epic_nc_read_fields(const char *path, int step, const epic_grid_t *grid, double *vortg, double *tbuoyg,
+                        double *humg, double *t);
+/* the same into the member's vortg, tbuoyg and humg (a box in the file must
+ * match the member's grid) */
+int epic_read_fields_nc(epic_handle h, const char *path, int step, double *t);
+
+/* ---- parcel set-up and restart (parcel_init.f90, utils.f90:144-249) ---- */
+/* parcel_default (parcel_init.f90:31-100): n_per_cell parcels per cell on
+ * the regular lattice of init_regular_positions (:104-152), volume vcell /
+ * n_per_cell, spheroids B11 = B22 = (lam abc)^(2/3) with lam the largest cell
+ * aspect ratio, split (init_refine, :156-166) while lam >= lambda_max;
+ * vorticity, buoyancy and humidity zero.  n_per_cell <= 0 selects 8 and
+ * lambda_max <= 0 selects 4 (options.f90:88-89); n_per_cell must be a cube
+ * (else EPIC_E_INVALID), the parcels must fit max_parcels (EPIC_E_CAPACITY). */
+int epic_parcel_default(epic_handle h, int n_per_cell, double lambda_max);
+/* init_parcels_from_grids (parcel_init.f90:174-203): every parcel adds the
+ * trilinear interpolation of vortg, tbuoyg and humg at its centre to its
+ * vorticity, buoyancy and humidity */
+int epic_init_parcels_from_grids(epic_handle h);
+/* get_next_write_time (utils.f90:48-54): freq (int(t / freq) + 1) */
+double epic_next_write_time(double freq, double t);
+/* setup_restart + the restart branch of setup_fields_and_parcels
+ * (utils.f90:144-160, 200-216): *t receives the file's last time and
+ * file_type (16 chars, may be NULL) its type; a "parcels" file is read into
+ * the member, a "fields" file gives parcel_default (8 per cell) with the
+ * file's last vorticity, buoyancy and humidity interpolated to the parcels;
+ * any other type fails with EPIC_E_INVALID */
+int epic_setup_restart(epic_handle h, const char *path, double *t, char *file_type);
 
 /* Cell counting sort of the parcels' current positions with the reference's
  * key ijk = 1 + ix + hx iy + hx hy iz on the halo box

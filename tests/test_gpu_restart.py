@@ -1,0 +1,145 @@
+"""Parcel set-up from gridded fields and restart, through the C-ABI:
+
+- parcel_default (parcel_init.f90:31-152): the regular lattice in the
+  reference's order (the same order as workloads.lattice_positions),
+  volume vcell / n_per_cell, spheroids with det B = (3V/4pi)^2, and
+  init_refine's splits when the cell is anisotropic;
+- init_parcels_from_grids (parcel_init.f90:174-203): trilinear
+  interpolation is exact on linear fields, so every parcel receives the
+  field's value at its centre (FP32 node values, FP64 weights);
+- setup_restart (utils.f90:144-216): a parcel file gives back the parcels
+  and its time, a field file gives the lattice with the file's last
+  vorticity and buoyancy interpolated to it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from epic_amd import core
+from epic_amd import workloads as W
+from epic_amd.core import EpicError, Member
+
+pytestmark = pytest.mark.gpu
+
+
+def _node_coords(nx, ny, nz, lower, extent):
+    dx = [extent[d] / (nx, ny, nz)[d] for d in range(3)]
+    x = lower[0] + (np.arange(nx + 3) - 1) * dx[0]
+    y = lower[1] + (np.arange(ny + 3) - 1) * dx[1]
+    z = lower[2] + (np.arange(nz + 3) - 1) * dx[2]
+    return np.meshgrid(x, y, z, indexing="ij")
+
+
+def test_parcel_default_lattice():
+    nx, ny, nz = 16, 8, 12
+    lower, extent = (-1.0, 0.5, -0.25), (2.0, 1.0, 1.5)
+    m = Member(nx, ny, nz, lower, extent, max_parcels=8 * nx * ny * nz)
+    m.parcel_default()
+    p = m.get_parcels()
+    dx = tuple(extent[d] / (nx, ny, nz)[d] for d in range(3))
+    want = W.lattice_positions(nx, ny, nz, lower, dx, 2)
+    assert m.n == want.shape[0] and np.abs(p["position"] - want).max() < 1e-14
+    vcell = dx[0] * dx[1] * dx[2]
+    assert np.allclose(p["volume"], vcell / 8, rtol=1e-7)
+    assert not p["vorticity"].any() and not p["buoyancy"].any() and not p["humidity"].any()
+    # B11 = B22 = (lam abc)^(2/3), the rest zero (lam = 1: cubic cells)
+    lam = max(dx[0] / dx[1], dx[1] / dx[0], dx[0] / dx[2], dx[2] / dx[0], dx[1] / dx[2], dx[2] / dx[1])
+    l23 = (lam * 0.75 * (vcell / 8) / math.pi) ** (2.0 / 3.0)
+    assert np.allclose(p["B"][:, 0], l23, rtol=1e-7) and np.allclose(p["B"][:, 3], l23, rtol=1e-7)
+    assert not p["B"][:, [1, 2, 4]].any()
+    with pytest.raises(EpicError):
+        m.parcel_default(n_per_cell=6)   # not a cube
+
+
+def test_parcel_default_refines_anisotropic_cells():
+    """A cell five times longer in x than in y and z gives lam = 5 >= 4: the
+    oblate spheroids (a = b = 5 c) split along one of their two long axes,
+    leaving the other at aspect 5, then along that one (aspect 2.5 < 4):
+    init_refine ends with four parcels of a quarter volume per lattice
+    parcel."""
+    nx, ny, nz = 8, 8, 8
+    m = Member(nx, ny, nz, (0.0, 0.0, 0.0), (5.0, 1.0, 1.0), max_parcels=4 * 8 * nx * ny * nz)
+    m.parcel_default()
+    n0 = 8 * nx * ny * nz
+    assert m.n == 4 * n0
+    p = m.get_parcels()
+    vcell = (5.0 / nx) * (1.0 / ny) * (1.0 / nz)
+    assert np.allclose(p["volume"], vcell / 32, rtol=1e-6)
+    assert np.isclose(p["volume"].sum(), 5.0, rtol=1e-6)
+
+
+def test_init_parcels_from_grids_linear_fields():
+    nx, ny, nz = 16, 12, 8
+    lower, extent = (-0.5, -1.0, 0.0), (1.0, 1.5, 0.8)
+    m = Member(nx, ny, nz, lower, extent, max_parcels=8 * nx * ny * nz)
+    m.parcel_default()
+    X, Y, Z = _node_coords(nx, ny, nz, lower, extent)
+    # fields linear in z (x and y are periodic, where a linear field would
+    # jump at the seam): trilinear interpolation reproduces them exactly
+    coef = np.array([[0.3, 1.7], [-0.2, 0.9], [1.1, -2.0], [0.05, 0.4], [0.7, 0.0]])
+    fields = [c[0] + c[1] * Z for c in coef]
+    m.set_field("vortg", np.stack(fields[:3]))
+    m.set_field("tbuoyg", fields[3])
+    m.set_field("humg", fields[4])
+    m.init_parcels_from_grids()
+    p = m.get_parcels()
+    z = p["position"][:, 2]
+    got = [p["vorticity"][:, 0], p["vorticity"][:, 1], p["vorticity"][:, 2], p["buoyancy"], p["humidity"]]
+    for g, c in zip(got, coef):
+        want = c[0] + c[1] * z
+        assert np.abs(g - want).max() <= 2e-7 * (abs(c[0]) + abs(c[1]) * 1.0), c
+    # adds to what the parcels hold: a second call doubles
+    m.init_parcels_from_grids()
+    q = m.get_parcels()
+    assert np.allclose(q["buoyancy"], 2 * p["buoyancy"], rtol=1e-6, atol=1e-7)
+
+
+def test_init_parcels_from_grids_matches_interpolation():
+    """A random field: the device interpolation equals trilinear
+    interpolation of the same FP32 node values (interpl.f90:17-56)."""
+    nx, ny, nz = 16, 16, 8
+    lower, extent = (-0.5 * math.pi,) * 3, (math.pi, math.pi, math.pi / 2)
+    m = Member(nx, ny, nz, lower, extent, max_parcels=8 * nx * ny * nz)
+    m.parcel_default()
+    rng = np.random.default_rng(3)
+    f = np.zeros((nx + 3, ny + 3, nz + 3))
+    f[1:nx + 1, 1:ny + 1, 1:nz + 2] = rng.normal(size=(nx, ny, nz + 1)).astype(np.float32)
+    m.set_field("tbuoyg", f)
+    m.init_parcels_from_grids()
+    p = m.get_parcels()
+    dx = tuple(extent[d] / (nx, ny, nz)[d] for d in range(3))
+    want = W.trilinear_at(f[1:nx + 1, 1:ny + 1, 1:nz + 2], p["position"], lower, dx, nx, ny)
+    assert np.abs(p["buoyancy"] - want).max() < 1e-6
+
+
+def test_setup_restart_from_parcel_and_field_files(tmp_path):
+    mem = W.rising_thermal(16, seed=41)
+    a = Member.from_workload(mem)
+    t = a.ls_rk_step(0.0, 0.01)
+    pf = tmp_path / "r_0000000001_parcels.nc"
+    a.write_parcels_nc(pf, t)
+    b = Member(mem.nx, mem.ny, mem.nz, mem.lower, mem.extent, max_parcels=mem.n)
+    tb, ft = b.setup_restart(pf)
+    assert (tb, ft) == (t, "parcels")
+    pa, pb = a.get_parcels(), b.get_parcels()
+    for k in pa:
+        assert np.array_equal(pa[k], pb[k]), k
+    # a field file: the member's own par2grid fields of a linear vorticity
+    nx, ny, nz = mem.nx, mem.ny, mem.nz
+    X, Y, Z = _node_coords(nx, ny, nz, mem.lower, mem.extent)
+    c = Member(nx, ny, nz, mem.lower, mem.extent, max_parcels=8 * nx * ny * nz)
+    c.set_field("vortg", np.stack([0.5 + Z, -Z, 2.0 * Z]))
+    c.set_field("tbuoyg", 1.0 - 0.5 * Z)
+    ff = tmp_path / "r_fields.nc"
+    c.write_fields_nc(ff, 3.5)
+    d = Member(nx, ny, nz, mem.lower, mem.extent, max_parcels=8 * nx * ny * nz)
+    td, ft = d.setup_restart(ff)
+    assert (td, ft) == (3.5, "fields") and d.n == 8 * nx * ny * nz
+    p = d.get_parcels()
+    z = p["position"][:, 2]
+    assert np.abs(p["vorticity"][:, 0] - (0.5 + z)).max() < 1e-6
+    assert np.abs(p["vorticity"][:, 2] - 2.0 * z).max() < 1e-6
+    assert np.abs(p["buoyancy"] - (1.0 - 0.5 * z)).max() < 1e-6
+    info = core.nc_file_info(ff)
+    assert info[0] == "fields" and info[1] == 3.5

@@ -181,3 +181,60 @@ def test_reads_cdf5(tmp_path):
     assert np.array_equal(back["position"][:, 2], data["z_position"])
     assert np.array_equal(back["B"][:, 1], data["B12"])
     assert np.array_equal(back["buoyancy"], data["buoyancy"])
+
+
+def _field_file(path, grid, fields, times, file_type="fields"):
+    """A field file written by scipy (CDF-1): x/y/z_vorticity, buoyancy and
+    humidity on (t, z, y, x), several records."""
+    nx, ny, nz = grid[:3]
+    with netcdf_file(path, "w") as f:
+        f.file_type = file_type
+        f.ncells = np.array([nx, ny, nz], dtype=np.int32)
+        f.extent = np.array(grid[4])
+        f.origin = np.array(grid[3])
+        f.createDimension("t", None)   # scipy wants the unlimited dimension first
+        f.createDimension("x", nx)
+        f.createDimension("y", ny)
+        f.createDimension("z", nz + 1)
+        tv = f.createVariable("t", "d", ("t",))
+        for r, t in enumerate(times):
+            tv[r] = t
+        for name, vals in fields.items():
+            v = f.createVariable(name, "d", ("t", "z", "y", "x"))
+            v[:] = vals
+
+
+def test_field_file_reader_and_file_info(tmp_path):
+    """read_netcdf_fields (field_netcdf.f90:306-386) and get_file_type /
+    get_time / get_num_steps on a file from an independent writer: the
+    requested record lands on planes 0..nz of the halo layout; absent
+    datasets leave zeros; step -1 is the last record, 0 or past the last is
+    the reference's error."""
+    nx, ny, nz = GRID[:3]
+    rng = np.random.default_rng(5)
+    recs = {k: rng.normal(size=(3, nz + 1, ny, nx)) for k in ("x_vorticity", "z_vorticity", "buoyancy")}
+    path = tmp_path / "f_fields.nc"
+    _field_file(path, GRID, recs, [0.0, 0.5, 1.25])
+    ft, t, n, grid = core.nc_file_info(path)
+    assert (ft, t, n) == ("fields", 1.25, 3) and grid[:3] == (nx, ny, nz) and np.allclose(grid[4], GRID[4])
+    for step, rec in ((-1, 2), (2, 1), (1, 0)):
+        f, tt = core.nc_read_fields(path, GRID, step)
+        assert tt == [0.0, 0.5, 1.25][rec]
+        # halo layout [ix+1, iy+1, iz+1] <- file (z, y, x)
+        want = lambda a: np.transpose(a[rec], (2, 1, 0))  # noqa: E731
+        assert np.array_equal(f["vortg"][0][1:nx + 1, 1:ny + 1, 1:nz + 2], want(recs["x_vorticity"]))
+        assert np.array_equal(f["vortg"][2][1:nx + 1, 1:ny + 1, 1:nz + 2], want(recs["z_vorticity"]))
+        assert np.array_equal(f["tbuoyg"][1:nx + 1, 1:ny + 1, 1:nz + 2], want(recs["buoyancy"]))
+        assert not f["vortg"][1].any() and not f["humg"].any()   # absent datasets
+        assert f["vortg"][0][0].sum() == 0 and f["vortg"][0][:, :, 0].sum() == 0   # halo untouched (zero)
+    for bad in (0, 4):
+        with pytest.raises(core.EpicError, match="out of bounds"):
+            core.nc_read_fields(path, GRID, bad)
+
+
+def test_next_write_time():
+    """get_next_write_time (utils.f90:48-54): freq (int(t / freq) + 1)."""
+    assert core.next_write_time(0.5, 0.0) == 0.5
+    assert core.next_write_time(0.5, 1.2) == 1.5
+    assert core.next_write_time(0.25, 1.0) == 1.25   # a time on the grid moves on
+    assert core.next_write_time(2.0, 7.99) == 8.0
