@@ -56,7 +56,7 @@ struct epic_member {
     // fields
     float *fld = nullptr;
     int *nparg = nullptr, *nsparg = nullptr;
-    float *staging = nullptr;
+    double *staging = nullptr;
     // spectra
     float2 *spec[NSPEC] = {};   // NSPEC consecutive spectra of one allocation (spec[0])
     float2 *twx = nullptr, *twy = nullptr;
@@ -332,7 +332,7 @@ P2GLayout layout_tz(const Grid &g, int tz)
     P2GLayout L = p2g_layout(g);
     L.tz = tz;
     L.nchz = (g.nz + 2 + tz - 1) / tz;
-    L.staging_floats = (int64_t)L.ntx * L.nty * L.nchz * (tz + 3) * P2G_BOX * P2G_BOX * P2G_NF;
+    L.staging_elems = (int64_t)L.ntx * L.nty * L.nchz * (tz + 3) * P2G_BOX * P2G_BOX * P2G_NF;
     return L;
 }
 
@@ -342,10 +342,10 @@ P2GLayout layout_tz(const Grid &g, int tz)
 const int TZ_P2G[3] = {2, 4, 8};
 const int TZ_G2P[3] = {2, 4, 8};
 
-int64_t staging_floats_max(const Grid &g)
+int64_t staging_elems_max(const Grid &g)
 {
     int64_t mx = 0;
-    for (int tz : TZ_P2G) mx = std::max(mx, layout_tz(g, tz).staging_floats);
+    for (int tz : TZ_P2G) mx = std::max(mx, layout_tz(g, tz).staging_elems);
     return mx;
 }
 
@@ -846,7 +846,7 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
         return EPIC_E_NOMEM;
     const int64_t FS = field_stride(g);
     if (bad(dalloc(&m->fld, (int64_t)F_COUNT * FS, m->s)) || bad(dalloc(&m->nparg, FS, m->s)) || bad(dalloc(&m->nsparg, FS, m->s)) ||
-        bad(dalloc(&m->staging, staging_floats_max(g), m->s)))
+        bad(dalloc(&m->staging, staging_elems_max(g), m->s)))
         return EPIC_E_NOMEM;
     static_assert(NSPEC >= 12, "the six FP64 spectra of vorticity_tendency take twelve float2 spectra");
     if (bad(dalloc(&m->spec[0], NSPEC * spec_size(g), m->s))) return EPIC_E_NOMEM;
@@ -1479,7 +1479,7 @@ int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels)
     b += c * (3 * 8 + (int64_t)NATTR * 4 + 4 + 12 * 4 + 4 + 4) + 4;           // parcels, axes, orig, keys, perm
     b += 4 * (6 * nc + (nc + 1) + nc);                                        // histograms, cell_start, cursor
     b += 4 * scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nc, c), (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2)));
-    b += 4 * ((int64_t)F_COUNT * FS + 2 * FS) + 4 * staging_floats_max(g);   // fields, counts, staging
+    b += 4 * ((int64_t)F_COUNT * FS + 2 * FS) + 8 * staging_elems_max(g);   // fields, counts, staging
     b += (int64_t)NSPEC * 8 * spec_size(g);                                  // spectra
     return b;
 }

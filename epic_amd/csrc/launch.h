@@ -47,7 +47,7 @@ struct AxesArgs {
 };
 struct P2GLayout {
     int tz, ntx, nty, nchz;
-    int64_t staging_floats;
+    int64_t staging_elems;   // FP64 values
 };
 P2GLayout p2g_layout(const Grid &g);
 struct P2GArgs {
@@ -58,7 +58,7 @@ struct P2GArgs {
     const int *use_alt;         // may be null
     const int *perm;
     const int *cell_start;
-    float *staging;
+    double *staging;            // exact partial node sums (scaled integers)
     int *err;
     const float *tmax;          // P2G_NF + 1 maxima from k_p2g_max (fixed-point scales)
     int tz, ntx, nty, nchz;
@@ -75,7 +75,7 @@ int p2g_max_blocks(int64_t n);
 void add_p2g_max(Plan &P, int phase, const std::vector<P2GMaxArgs> &items);
 struct FinalizeArgs {
     Grid g;
-    const float *staging;
+    const double *staging;
     const int *count, *scount;
     float *fld;     // F_VOLG .. F_HUMG
     int *nparg, *nsparg;

@@ -13,12 +13,14 @@
 //                      each of the 4 x 8 x 7 contributions of a parcel scaled
 //                      by a per-member power of two, rounded and added to a
 //                      64-bit integer LDS accumulator of its node; the box is
-//                      converted back and written to a per-tile staging box
+//                      scaled back exactly and written to a per-tile FP64
+//                      staging box
 //   6. k_p2g_finalize  per node, sums the staging boxes that cover it in a
 //                      fixed tile order (the periodic halo accumulation of
 //                      field_mpi.f90), applies the free-slip folds,
 //                      normalises by volume, the zeta wall condition and the
-//                      z extrapolation (parcel_interpl.f90:235-319)
+//                      z extrapolation (parcel_interpl.f90:235-319) in FP64;
+//                      each field is rounded to FP32 once, on its store
 // Integer node sums are exact and the staging boxes are added in a fixed
 // tile order, so the result depends on the parcel set alone, never on
 // scheduling or parcel order: results are bitwise reproducible.
@@ -520,19 +522,21 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
     if (err) atomicOr(&s_err, err);
     __syncthreads();
 
-    // ---- the box to staging (FP32, node-major, fields fastest) ----
-    float inv[P2G_NF];
+    // ---- the box to staging (FP64, node-major, fields fastest) ----
+    // q / scale is exact in FP64 (|q| < 2^53, the scale a power of two), so
+    // the staged partial sums keep every bit of the integer sums
+    double inv[P2G_NF];
 #pragma unroll
-    for (int f = 0; f < P2G_NF; ++f) inv[f] = (sc[f] > 0.0f) ? 1.0f / sc[f] : 0.0f;   // a power of two: exact
+    for (int f = 0; f < P2G_NF; ++f) inv[f] = (sc[f] > 0.0f) ? 1.0 / (double)sc[f] : 0.0;
     constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
-    EPIC_GLOBAL float *stage = gl(a.staging) + (int64_t)ib.blk * nbox * PLANE;
+    EPIC_GLOBAL double *stage = gl(a.staging) + (int64_t)ib.blk * nbox * PLANE;
     for (int u = tid; u < nbox * PLANE; u += blockDim.x) {
         const int f = u % P2G_NF, nd = u / P2G_NF;
         const int pz = nd / (P2G_BOX * P2G_BOX), yx = nd - pz * (P2G_BOX * P2G_BOX);
         const int y = yx / P2G_BOX, x = yx - y * P2G_BOX;
         const int fs = (f == 5 && !own5) ? 4 : f;   // a skipped field 5 is field 4
         const long long q = (f == 6 && !own6) ? 0ll : (long long)acc[fs * lfield + pz * lplane + y * lp + x];
-        stage[u] = (float)((double)q * (double)inv[fs]);
+        stage[u] = (double)q * inv[fs];
     }
     if (tid == 0 && s_err) atomicOr(a.err, s_err);
 }
@@ -540,12 +544,12 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
 // ---------------------------------------------------------------------------
 // 6. staging sum + folds + normalisation (one thread per (i, j) column)
 // ---------------------------------------------------------------------------
-__device__ inline void stage_sum(const FinalizeArgs &a, int i, int j, int k, float out[P2G_NF])
+__device__ inline void stage_sum(const FinalizeArgs &a, int i, int j, int k, double out[P2G_NF])
 {
     constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
     const Grid &g = a.g;
 #pragma unroll
-    for (int f = 0; f < P2G_NF; ++f) out[f] = 0.0f;
+    for (int f = 0; f < P2G_NF; ++f) out[f] = 0.0;
     const int nbox = a.tz + 3;
     const int cza = (k + 1) / a.tz;
     const int txa = i / P2G_TX, tya = j / P2G_TY;
@@ -564,7 +568,7 @@ __device__ inline void stage_sum(const FinalizeArgs &a, int i, int j, int k, flo
                     const int tx = wrap(txa + dtx, a.ntx);
                     for (int lx = wrap(i - (tx * P2G_TX - 1), g.nx); lx < P2G_BOX; lx += g.nx) {
                         const int wg = (cz * a.nty + ty) * a.ntx + tx;
-                        const EPIC_GLOBAL float *s = gl(a.staging) + ((int64_t)wg * nbox + lp) * PLANE + (ly * P2G_BOX + lx) * P2G_NF;
+                        const EPIC_GLOBAL double *s = gl(a.staging) + ((int64_t)wg * nbox + lp) * PLANE + (ly * P2G_BOX + lx) * P2G_NF;
 #pragma unroll
                         for (int f = 0; f < P2G_NF; ++f) out[f] += s[f];
                     }
@@ -595,13 +599,15 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     const int64_t base = (int64_t)j * g.nx + i;
     auto F = [&](int slot, int kk) -> EPIC_GLOBAL float & { return gl(a.fld)[slot * FS + (int64_t)(kk + 1) * PS + base]; };
 
-    // the node value of plane kk (folds of parcel_interpl.f90:258-287)
-    auto folded = [&](int kk, float r[P2G_NF]) {
-        float x[P2G_NF], y[P2G_NF];
+    // the node value of plane kk (folds of parcel_interpl.f90:258-287); the
+    // staged sums add exactly, the folds and the normalisation run in FP64
+    // and each field is rounded to FP32 once, when it is stored
+    auto folded = [&](int kk, double r[P2G_NF]) {
+        double x[P2G_NF], y[P2G_NF];
         if (kk == 0 || kk == nz) {
             stage_sum(a, i, j, kk, x);
 #pragma unroll
-            for (int f = 0; f < P2G_NF; ++f) r[f] = 2.0f * x[f];
+            for (int f = 0; f < P2G_NF; ++f) r[f] = 2.0 * x[f];
         } else if (kk == 1 || kk == nz - 1) {
             stage_sum(a, i, j, kk, x);
             stage_sum(a, i, j, kk == 1 ? -1 : nz + 1, y);
@@ -614,34 +620,34 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     auto zero_zeta = [&](int kk) { return (kk == 0 && g.zeta_zero[0]) || (kk == nz && g.zeta_zero[1]); };
 
     if (k >= 0 && k <= nz) {
-        float r[P2G_NF];
+        double r[P2G_NF];
         folded(k, r);
-        const float v = r[0];
-        F(F_VOLG, k) = v;
-        F(F_VORTG + 0, k) = r[1] / v;
-        F(F_VORTG + 1, k) = r[2] / v;
-        F(F_VORTG + 2, k) = zero_zeta(k) ? 0.0f : r[3] / v;
-        F(F_TBUOYG, k) = r[4] / v;
-        F(F_DBUOYG, k) = r[5] / v;
-        F(F_HUMG, k) = r[6] / v;
+        const double v = r[0];
+        F(F_VOLG, k) = (float)v;
+        F(F_VORTG + 0, k) = (float)(r[1] / v);
+        F(F_VORTG + 1, k) = (float)(r[2] / v);
+        F(F_VORTG + 2, k) = zero_zeta(k) ? 0.0f : (float)(r[3] / v);
+        F(F_TBUOYG, k) = (float)(r[4] / v);
+        F(F_DBUOYG, k) = (float)(r[5] / v);
+        F(F_HUMG, k) = (float)(r[6] / v);
     } else {
         // halo planes keep the raw sums for vol/dbuoy/hum; vortg and tbuoyg
         // are extrapolated from the normalised boundary planes (:306-318)
-        float raw[P2G_NF], rb[P2G_NF], rn[P2G_NF];
+        double raw[P2G_NF], rb[P2G_NF], rn[P2G_NF];
         stage_sum(a, i, j, k, raw);
         const int kb = (k < 0) ? 0 : nz, kn = (k < 0) ? 1 : nz - 1;
         folded(kb, rb);
         folded(kn, rn);
-        F(F_VOLG, k) = raw[0];
-        F(F_DBUOYG, k) = raw[5];
-        F(F_HUMG, k) = raw[6];
+        F(F_VOLG, k) = (float)raw[0];
+        F(F_DBUOYG, k) = (float)raw[5];
+        F(F_HUMG, k) = (float)raw[6];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const float vb = (c == 2 && zero_zeta(kb)) ? 0.0f : rb[1 + c] / rb[0];
-            const float vn = rn[1 + c] / rn[0];
-            F(F_VORTG + c, k) = 2.0f * vb - vn;
+            const double vb = (c == 2 && zero_zeta(kb)) ? 0.0 : rb[1 + c] / rb[0];
+            const double vn = rn[1 + c] / rn[0];
+            F(F_VORTG + c, k) = (float)(2.0 * vb - vn);
         }
-        F(F_TBUOYG, k) = 2.0f * (rb[4] / rb[0]) - rn[4] / rn[0];
+        F(F_TBUOYG, k) = (float)(2.0 * (rb[4] / rb[0]) - rn[4] / rn[0]);
     }
     // parcel counts (:201-205, folds :239-243)
     auto cnt = [&](const int *c, int kk) -> int {
@@ -747,7 +753,7 @@ P2GLayout p2g_layout(const Grid &g)
     L.ntx = g.nx / P2G_TX;
     L.nty = g.ny / P2G_TY;
     L.nchz = (g.nz + 2 + L.tz - 1) / L.tz;
-    L.staging_floats = (int64_t)L.ntx * L.nty * L.nchz * (L.tz + 3) * P2G_BOX * P2G_BOX * P2G_NF;
+    L.staging_elems = (int64_t)L.ntx * L.nty * L.nchz * (L.tz + 3) * P2G_BOX * P2G_BOX * P2G_NF;
     return L;
 }
 

@@ -70,7 +70,7 @@ def test_parcel_default_refines_anisotropic_cells():
 
 
 def test_init_parcels_from_grids_linear_fields():
-    nx, ny, nz = 16, 12, 8
+    nx, ny, nz = 16, 24, 8
     lower, extent = (-0.5, -1.0, 0.0), (1.0, 1.5, 0.8)
     m = Member(nx, ny, nz, lower, extent, max_parcels=8 * nx * ny * nz)
     m.parcel_default()
