@@ -286,8 +286,8 @@ __host__ __device__ constexpr int p2g_lpitch(int tz) { return tz >= 8 ? P2G_BOX 
 
 // power-of-two scale s with max_abs * s < 2^27 (0 when max_abs is 0): the
 // summed weights of up to four points of one parcel stay below 4, so every
-// rounded contribution is below 2^29 and the sum of two (the two threads of
-// a cell) fits an int32
+// rounded contribution is below 2^29 and a node's sum over its parcels stays
+// far below 2^47
 __device__ inline float fixed_scale(float max_abs, int &e)
 {
     if (!(max_abs > 0.0f)) {
@@ -301,6 +301,25 @@ __device__ inline float fixed_scale(float max_abs, int &e)
 // the value of the other lane of a lane pair (2k, 2k+1): DPP quad
 // permutation [1, 0, 3, 2]; both lanes must be active
 __device__ inline int pair_swap(int v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false); }
+__device__ inline unsigned long long pair_swap64(unsigned long long v)
+{
+    const unsigned lo = (unsigned)pair_swap((int)(unsigned)v), hi = (unsigned)pair_swap((int)(unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// One fixed-point contribution as a 64-bit accumulator term, in one FP64
+// FMA: w * c is exact in FP64 (two FP32 factors), and adding 1.5 * 2^52
+// rounds it to the nearest integer q (ties to even) with q held in the low
+// bits of the result, whose pattern is F_MAGIC + q for |q| < 2^51.  F_MAGIC
+// is zero below bit 48, so a sum of such terms equals the sum of the q
+// modulo 2^48 in its low 48 bits, whatever the number of terms: the node
+// sums (|sum| < 2^47) are the low 48 bits sign-extended (fixed_sum).
+constexpr double F_MAGIC_D = 6755399441055744.0;   // 1.5 * 2^52
+__device__ inline unsigned long long fixed_term(double w, double c)
+{
+    return (unsigned long long)__double_as_longlong(fma(w, c, F_MAGIC_D));
+}
+__device__ inline long long fixed_sum(unsigned long long acc) { return (long long)(acc << 16) >> 16; }
 
 // the gridded quantities of parcel pi, read from its arrays (o4 = 4 pi)
 __device__ inline void p2g_values(const Grid &g, const ParcelsDev &p, unsigned o4, double z, float v[P2G_NF])
@@ -351,7 +370,7 @@ __global__ __launch_bounds__(256) void k_p2g_max(const P2GMaxArgs *__restrict__ 
 // TZ (= a.tz) is a template parameter, so the LDS strides are constants and
 // the 8 x 7 atomics of a cell are its base word plus immediate offsets
 template <int TZ>
-__global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+__global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P2GArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const P2GArgs &a = tab[ib.item];
@@ -423,9 +442,9 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
         const double vt[3] = {at(axes[3], o4), at(axes[4], o4), at(axes[5], o4)};
         float v[P2G_NF];
         p2g_values(g, a.p, o4, pos[2], v);
-        float cs[P2G_NF];
+        double csd[P2G_NF];
 #pragma unroll
-        for (int f = 0; f < P2G_NF; ++f) cs[f] = have ? v[f] * sc[f] : 0.0f;   // exact: sc is a power of two
+        for (int f = 0; f < P2G_NF; ++f) csd[f] = have ? (double)(v[f] * sc[f]) : 0.0;   // exact: sc is a power of two
         int cc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
@@ -476,13 +495,13 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int node = nown + ((c >> 1) & 1) * lp + (c & 1);
+                const double wnd = (double)wn[c], wfd = (double)wf[c];
 #pragma unroll
                 for (int f = 0; f < P2G_NF; ++f) {
                     if ((f == 5 && !own5) || (f == 6 && !own6)) continue;   // uniform over the work-group
-                    const int qn = (int)rintf(wn[c] * cs[f]);
-                    const int qf = (int)rintf(wf[c] * cs[f]);
-                    const int q = qn + pair_swap(qf);   // |q| < 2^30
-                    atomicAdd(&acc[f * lfield + node], (unsigned long long)(long long)q);
+                    // this lane's parcel on its plane plus the other lane's parcel on it
+                    const unsigned long long t = fixed_term(wnd, csd[f]) + pair_swap64(fixed_term(wfd, csd[f]));
+                    atomicAdd(&acc[f * lfield + node], t);
                 }
             }
         }
@@ -510,11 +529,11 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int node = tgt[jp] + (c >> 2) * lplane + ((c >> 1) & 1) * lp + (c & 1);
+                const double wd = (double)w[c];
 #pragma unroll
                 for (int f = 0; f < P2G_NF; ++f) {
                     if ((f == 5 && !own5) || (f == 6 && !own6)) continue;   // uniform over the work-group
-                    const long long q = (long long)(int)rintf(w[c] * cs[f]);
-                    atomicAdd(&acc[f * lfield + node], (unsigned long long)q);
+                    atomicAdd(&acc[f * lfield + node], fixed_term(wd, csd[f]));
                 }
             }
         }
@@ -535,7 +554,7 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
         const int pz = nd / (P2G_BOX * P2G_BOX), yx = nd - pz * (P2G_BOX * P2G_BOX);
         const int y = yx / P2G_BOX, x = yx - y * P2G_BOX;
         const int fs = (f == 5 && !own5) ? 4 : f;   // a skipped field 5 is field 4
-        const long long q = (f == 6 && !own6) ? 0ll : (long long)acc[fs * lfield + pz * lplane + y * lp + x];
+        const long long q = (f == 6 && !own6) ? 0ll : fixed_sum(acc[fs * lfield + pz * lplane + y * lp + x]);
         stage[u] = (double)q * inv[fs];
     }
     if (tid == 0 && s_err) atomicOr(a.err, s_err);
@@ -544,12 +563,11 @@ __global__ __launch_bounds__(1024) void k_p2g_tiles(const P2GArgs *__restrict__ 
 // ---------------------------------------------------------------------------
 // 6. staging sum + folds + normalisation (one thread per (i, j) column)
 // ---------------------------------------------------------------------------
-__device__ inline void stage_sum(const FinalizeArgs &a, int i, int j, int k, double out[P2G_NF])
+// adds the staged sums of node (i, j, k) to out[]
+__device__ inline void stage_add(const FinalizeArgs &a, int i, int j, int k, double out[P2G_NF])
 {
     constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
     const Grid &g = a.g;
-#pragma unroll
-    for (int f = 0; f < P2G_NF; ++f) out[f] = 0.0;
     const int nbox = a.tz + 3;
     const int cza = (k + 1) / a.tz;
     const int txa = i / P2G_TX, tya = j / P2G_TY;
@@ -603,18 +621,14 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     // staged sums add exactly, the folds and the normalisation run in FP64
     // and each field is rounded to FP32 once, when it is stored
     auto folded = [&](int kk, double r[P2G_NF]) {
-        double x[P2G_NF], y[P2G_NF];
+#pragma unroll
+        for (int f = 0; f < P2G_NF; ++f) r[f] = 0.0;
+        stage_add(a, i, j, kk, r);
         if (kk == 0 || kk == nz) {
-            stage_sum(a, i, j, kk, x);
 #pragma unroll
-            for (int f = 0; f < P2G_NF; ++f) r[f] = 2.0 * x[f];
+            for (int f = 0; f < P2G_NF; ++f) r[f] *= 2.0;
         } else if (kk == 1 || kk == nz - 1) {
-            stage_sum(a, i, j, kk, x);
-            stage_sum(a, i, j, kk == 1 ? -1 : nz + 1, y);
-#pragma unroll
-            for (int f = 0; f < P2G_NF; ++f) r[f] = x[f] + y[f];
-        } else {
-            stage_sum(a, i, j, kk, r);
+            stage_add(a, i, j, kk == 1 ? -1 : nz + 1, r);
         }
     };
     auto zero_zeta = [&](int kk) { return (kk == 0 && g.zeta_zero[0]) || (kk == nz && g.zeta_zero[1]); };
@@ -633,21 +647,23 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     } else {
         // halo planes keep the raw sums for vol/dbuoy/hum; vortg and tbuoyg
         // are extrapolated from the normalised boundary planes (:306-318)
-        double raw[P2G_NF], rb[P2G_NF], rn[P2G_NF];
-        stage_sum(a, i, j, k, raw);
-        const int kb = (k < 0) ? 0 : nz, kn = (k < 0) ? 1 : nz - 1;
-        folded(kb, rb);
-        folded(kn, rn);
-        F(F_VOLG, k) = (float)raw[0];
-        F(F_DBUOYG, k) = (float)raw[5];
-        F(F_HUMG, k) = (float)raw[6];
+        double r[P2G_NF];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double vb = (c == 2 && zero_zeta(kb)) ? 0.0 : rb[1 + c] / rb[0];
-            const double vn = rn[1 + c] / rn[0];
-            F(F_VORTG + c, k) = (float)(2.0 * vb - vn);
-        }
-        F(F_TBUOYG, k) = (float)(2.0 * (rb[4] / rb[0]) - rn[4] / rn[0]);
+        for (int f = 0; f < P2G_NF; ++f) r[f] = 0.0;
+        stage_add(a, i, j, k, r);
+        F(F_VOLG, k) = (float)r[0];
+        F(F_DBUOYG, k) = (float)r[5];
+        F(F_HUMG, k) = (float)r[6];
+        const int kb = (k < 0) ? 0 : nz, kn = (k < 0) ? 1 : nz - 1;
+        double e2[4];   // twice the boundary plane's normalised values
+        folded(kb, r);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) e2[c] = (c == 2 && zero_zeta(kb)) ? 0.0 : 2.0 * (r[1 + c] / r[0]);
+        e2[3] = 2.0 * (r[4] / r[0]);
+        folded(kn, r);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) F(F_VORTG + c, k) = (float)(e2[c] - r[1 + c] / r[0]);
+        F(F_TBUOYG, k) = (float)(e2[3] - r[4] / r[0]);
     }
     // parcel counts (:201-205, folds :239-243)
     auto cnt = [&](const int *c, int kk) -> int {
