@@ -389,6 +389,7 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
     constexpr int lfield = nbox * lplane;         // words per field
     extern __shared__ unsigned long long acc[];   // [field][plane][y][x]
     __shared__ int s_err;
+    __shared__ unsigned short s_items[2 * TZ][64];   // per wave: the pass's items (source lane, first point)
 
     for (int u = tid; u < P2G_NF * lfield; u += blockDim.x) acc[u] = 0ull;
     if (tid == 0) s_err = 0;
@@ -430,12 +431,19 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
     const bool odd = sub != 0;
     const int cown = (lz + 1) * lplane + (cy + 1) * lp + cx + 1;   // own cell's lower corner
     const int nown = cown + (odd ? lplane : 0);   // this lane's four corners: the lower (even) or upper (odd) plane
-    const int rounds = (s1 - s0 + 1) >> 1;
+    // every lane of a wave runs the wave's largest number of rounds (the
+    // non-own cells below are shared out over the whole wave), the lanes of
+    // cells with fewer parcels idle
+    int rounds = (s1 - s0 + 1) >> 1;
+    for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o, 64));
+    const int lane = tid & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned short *items = s_items[tid >> 6];
     int err = 0;
     for (int rd = 0; rd < rounds; ++rd) {
         const int s = s0 + 2 * rd + sub;
         const bool have = s < s1;
-        const int pi = have ? gl(a.perm)[s] : gl(a.perm)[s0];   // an idle lane reads a valid entry and adds nothing
+        const int pi = have ? gl(a.perm)[s] : 0;   // an idle lane reads parcel 0 and adds nothing
         const unsigned o4 = (unsigned)pi * 4u;   // byte offset (pi < 2^30): scalar base + 32-bit offset addressing
         const double pos[3] = {gl(a.p.x)[pi], gl(a.p.y)[pi], gl(a.p.z)[pi]};
         const double ve[3] = {at(axes[0], o4), at(axes[1], o4), at(axes[2], o4)};
@@ -506,34 +514,92 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
             }
         }
         // every other cell the points reach: its 8 corners take the summed
-        // weights of its points (in point order), 8 x 7 atomics per cell
+        // weights of its points (in point order), 8 x 7 atomics per cell.
+        // These (lane, cell) items, about one for every second parcel, are
+        // shared out over the wave: item P[k] + (rank of the lane among the
+        // lanes with more than k items) is a lane's k-th; the lanes take 64
+        // items per pass, find their source lane through a per-wave list and
+        // read its fractions, targets and values by ds_bpermute, so the
+        // atomics run with nearly every lane active (one pass per round as a
+        // rule, where a loop over the four points ran up to four sparse ones).
+        // Each item's weights and terms are the same arithmetic as in its
+        // source lane.
+        unsigned leadm = 0;   // bit jp: point jp opens a cell other than its own and earlier points'
 #pragma unroll
         for (int jp = 0; jp < 4; ++jp) {
             bool lead = tgt[jp] >= 0 && tgt[jp] != cown && have;
 #pragma unroll
             for (int jq = 0; jq < jp; ++jq) lead = lead && (tgt[jq] != tgt[jp]);
-            if (!lead) continue;
-            float w[8];
+            leadm |= (lead ? 1u : 0u) << jp;
+        }
+        const int ni = __popc(leadm);
+        unsigned long long lb[4];
+        int pk[5];
+        pk[0] = 0;
 #pragma unroll
-            for (int jq = jp; jq < 4; ++jq) {
-                if (jq > jp && tgt[jq] != tgt[jp]) continue;
-                const float *f3 = fr[jq];
-                const float wxy[4] = {(1.0f - f3[1]) * (1.0f - f3[0]), (1.0f - f3[1]) * f3[0], f3[1] * (1.0f - f3[0]),
-                                      f3[1] * f3[0]};
+        for (int k = 0; k < 4; ++k) {
+            lb[k] = __ballot(ni > k);
+            pk[k + 1] = pk[k] + __popcll(lb[k]);
+        }
+        const int total = pk[4];   // uniform over the wave
+        if (total > 0) {
+            float csf[P2G_NF];
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const float wc = ((c & 4) ? f3[2] : 1.0f - f3[2]) * wxy[c & 3];
-                    w[c] = (jq == jp) ? wc : w[c] + wc;
+            for (int f = 0; f < P2G_NF; ++f) csf[f] = have ? v[f] * sc[f] : 0.0f;
+            const int t01 = (tgt[0] & 0xffff) | (tgt[1] << 16), t23 = (tgt[2] & 0xffff) | (tgt[3] << 16);
+            for (int r = 0; r < total; r += 64) {
+                unsigned lm = leadm;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int jp = __ffs(lm) - 1;   // the lane's k-th item: point jp
+                    lm &= lm - 1u;
+                    const int idx = pk[k] + __popcll(lb[k] & below) - r;
+                    if (k < ni && idx >= 0 && idx < 64) items[idx] = (unsigned short)(lane | (jp << 8));
                 }
-            }
+                __builtin_amdgcn_wave_barrier();
+                const bool mine = r + lane < total;
+                const unsigned e = mine ? (unsigned)items[lane] : 0u;
+                const int sa = (int)(e & 63u) << 2, jp0 = (int)(e >> 8) & 3;
+                float g[4][3];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int node = tgt[jp] + (c >> 2) * lplane + ((c >> 1) & 1) * lp + (c & 1);
-                const double wd = (double)w[c];
+                for (int jq = 0; jq < 4; ++jq)
 #pragma unroll
-                for (int f = 0; f < P2G_NF; ++f) {
-                    if ((f == 5 && !own5) || (f == 6 && !own6)) continue;   // uniform over the work-group
-                    atomicAdd(&acc[f * lfield + node], fixed_term(wd, csd[f]));
+                    for (int d = 0; d < 3; ++d) g[jq][d] = __int_as_float(__builtin_amdgcn_ds_bpermute(sa, __float_as_int(fr[jq][d])));
+                float cf[P2G_NF];
+#pragma unroll
+                for (int f = 0; f < P2G_NF; ++f)
+                    cf[f] = ((f == 5 && !own5) || (f == 6 && !own6)) ? 0.0f
+                            : __int_as_float(__builtin_amdgcn_ds_bpermute(sa, __float_as_int(csf[f])));
+                const int u01 = __builtin_amdgcn_ds_bpermute(sa, t01), u23 = __builtin_amdgcn_ds_bpermute(sa, t23);
+                __builtin_amdgcn_wave_barrier();   // the list is read before the next pass writes it
+                if (mine) {
+                    const int tg[4] = {(int)(short)(u01 & 0xffff), u01 >> 16, (int)(short)(u23 & 0xffff), u23 >> 16};
+                    const int t = jp0 == 0 ? tg[0] : jp0 == 1 ? tg[1] : jp0 == 2 ? tg[2] : tg[3];
+                    float w[8];
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) w[c] = 0.0f;
+#pragma unroll
+                    for (int jq = 0; jq < 4; ++jq) {
+                        if (jq < jp0 || (jq > jp0 && tg[jq] != t)) continue;
+                        const float *f3 = g[jq];
+                        const float wxy[4] = {(1.0f - f3[1]) * (1.0f - f3[0]), (1.0f - f3[1]) * f3[0],
+                                              f3[1] * (1.0f - f3[0]), f3[1] * f3[0]};
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) {
+                            const float wc = ((c & 4) ? f3[2] : 1.0f - f3[2]) * wxy[c & 3];
+                            w[c] = (jq == jp0) ? wc : w[c] + wc;
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int node = t + (c >> 2) * lplane + ((c >> 1) & 1) * lp + (c & 1);
+                        const double wd = (double)w[c];
+#pragma unroll
+                        for (int f = 0; f < P2G_NF; ++f) {
+                            if ((f == 5 && !own5) || (f == 6 && !own6)) continue;   // uniform over the work-group
+                            atomicAdd(&acc[f * lfield + node], fixed_term(wd, (double)cf[f]));
+                        }
+                    }
                 }
             }
         }
