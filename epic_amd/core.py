@@ -91,6 +91,7 @@ def load_library(path=None):
         "epic_get_time_step": ([h, d, P(TimeOpts), dp, dp, dp], i),
         "epic_ls_rk_step_adaptive": ([h, dp, P(TimeOpts), d, dp], i),
         "epic_parcel_diagnostics": ([h, dp, i], i),
+        "epic_parcel_diagnostics_noalloc": ([h, dp, i], i),
         "epic_resort": ([h], i),
         "epic_get_config": ([h, P(Grid), P(Phys), P(Flags)], i),
         "epic_write_parcels_nc": ([h, C.c_char_p, d], i),
@@ -279,10 +280,14 @@ class Member:
     FIELD_STATS = ("rms_v", "avg_npar", "avg_nspar", "keg", "apeg", "eng", "abserr_v", "max_npar", "max_buoy",
                    "min_npar", "min_buoy")
 
-    def parcel_diagnostics(self, reset_counters=False):
-        """calculate_parcel_diagnostics (parcel_diagnostics.f90:51-159)."""
+    def parcel_diagnostics(self, reset_counters=False, noalloc=False):
+        """calculate_parcel_diagnostics (parcel_diagnostics.f90:51-159);
+        noalloc: through epic_parcel_diagnostics_noalloc (no device or pinned
+        allocation, a fixed per-GPU scratch array)."""
         out = np.zeros(18)
-        self._chk(load_library().epic_parcel_diagnostics(self._h, _dp(out), int(bool(reset_counters))))
+        L = load_library()
+        fn = L.epic_parcel_diagnostics_noalloc if noalloc else L.epic_parcel_diagnostics
+        self._chk(fn(self._h, _dp(out), int(bool(reset_counters))))
         return dict(zip(self.PARCEL_STATS, out))
 
     def field_diagnostics(self):

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1894,6 +1895,34 @@ int epic_apply_vortcor(epic_handle m)
     return check(m, "epic_apply_vortcor");
 }
 
+// the reported parcel diagnostics from the 12 reduced totals
+// (parcel_diagnostics.f90:51-159)
+static void parcel_stats_of(const epic_member *m, const double s[12], double *stats)
+{
+    const Grid &g = m->g;
+    const double vdomaini = 1.0 / (g.extent[0] * g.extent[1] * g.extent[2]);
+    const double ntoti = 1.0 / (double)m->n;
+    stats[1] = 0.5 * s[0] * vdomaini;
+    stats[2] = s[6];
+    stats[3] = s[2] * ntoti;
+    stats[4] = s[4] * ntoti;
+    stats[5] = std::sqrt(std::fabs(s[3] * ntoti - stats[3] * stats[3]));
+    stats[6] = std::sqrt(std::fabs(s[5] * ntoti - stats[4] * stats[4]));
+    stats[7] = s[4];
+    for (int c = 0; c < 3; ++c) stats[8 + c] = std::sqrt(s[7 + c] / s[4]);
+    stats[12] = 0.5 * s[1] * vdomaini;
+    stats[16] = std::min(0.0, s[11]);   // the extremes start from zero (parcel_diagnostics.f90:60-61)
+    stats[17] = std::max(0.0, s[10]);
+}
+
+static void parcel_counts_of(epic_member *m, double *stats, int reset_counters)
+{
+    stats[11] = (double)m->n;
+    stats[13] = (double)m->n_splits;
+    stats[15] = (double)m->n_merges;
+    if (reset_counters) m->n_splits = m->n_merges = 0;   // as after a write (parcel_diagnostics_netcdf.f90:181-183)
+}
+
 // run a diagnostics reduction and copy its nv totals to the host
 static int diag_run(epic_member *m, bool parcels, double *out, int nv, const char *where)
 {
@@ -1903,7 +1932,7 @@ static int diag_run(epic_member *m, bool parcels, double *out, int nv, const cha
         HIPOK(m, hipHostMalloc((void **)&m->diag_h, sizeof(double) * 16, hipHostMallocDefault));
     }
     double *res = m->diag_d + tot - 16;
-    if (parcels) launch_parcel_diagnostics(m->g, m->pd, m->diag_d, res, m->err_d, m->s);
+    if (parcels) launch_parcel_diagnostics(m->g, m->pd, nullptr, m->diag_d, res, m->err_d, m->s);
     else launch_field_diagnostics(m->g, m->fld, m->nparg, m->nsparg, m->diag_d, res, m->s);
     HIPOK(m, hipGetLastError());
     HIPOK(m, hipMemcpyAsync(m->diag_h, res, sizeof(double) * nv, hipMemcpyDeviceToHost, m->s));
@@ -1921,29 +1950,78 @@ int epic_parcel_diagnostics(epic_handle m, double *stats, int reset_counters)
         const int rc = settle_tendencies(m);
         if (rc) return rc;
     }
-    const Grid &g = m->g;
     if (m->n > 0) {
         double s[12];
         const int rc = diag_run(m, true, s, 12, "epic_parcel_diagnostics");
         if (rc) return rc;
-        const double vdomaini = 1.0 / (g.extent[0] * g.extent[1] * g.extent[2]);
-        const double ntoti = 1.0 / (double)m->n;
-        stats[1] = 0.5 * s[0] * vdomaini;
-        stats[2] = s[6];
-        stats[3] = s[2] * ntoti;
-        stats[4] = s[4] * ntoti;
-        stats[5] = std::sqrt(std::fabs(s[3] * ntoti - stats[3] * stats[3]));
-        stats[6] = std::sqrt(std::fabs(s[5] * ntoti - stats[4] * stats[4]));
-        stats[7] = s[4];
-        for (int c = 0; c < 3; ++c) stats[8 + c] = std::sqrt(s[7 + c] / s[4]);
-        stats[12] = 0.5 * s[1] * vdomaini;
-        stats[16] = std::min(0.0, s[11]);   // the extremes start from zero (parcel_diagnostics.f90:60-61)
-        stats[17] = std::max(0.0, s[10]);
+        parcel_stats_of(m, s, stats);
     }
-    stats[11] = (double)m->n;
-    stats[13] = (double)m->n_splits;
-    stats[15] = (double)m->n_merges;
-    if (reset_counters) m->n_splits = m->n_merges = 0;   // as after a write (parcel_diagnostics_netcdf.f90:181-183)
+    parcel_counts_of(m, stats, reset_counters);
+    return EPIC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Parcel diagnostics without allocation: the block partials and totals live
+// in one file-scope device array (each GPU holds its own copy until the
+// process ends), whose address on the current device is looked up once per
+// device with hipGetSymbolAddress.  Every call writes every partial it then
+// reads, so nothing needs clearing between calls; calls that overlap on one
+// GPU share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int NOALLOC_BLOCKS = 1024;                                  // partial blocks (grid-stride over the parcels)
+constexpr int NOALLOC_TOTALS = 16;                                    // the totals after the partials
+constexpr int NOALLOC_DOUBLES = NOALLOC_BLOCKS * 12 + NOALLOC_TOTALS;  // 12 partials per block: 98,432 bytes
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_dev[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+// the device address of scratch on the current device (host code never
+// passes the array's name itself: that is a host-side stand-in)
+static int scratch_on_device(double **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return EPIC_E_HIP;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return EPIC_E_HIP;
+        scratch_dev[dev] = (double *)p;
+    }
+    *out = scratch_dev[dev];
+    return EPIC_OK;
+}
+
+extern "C" {
+
+int epic_parcel_diagnostics_noalloc(epic_handle m, double *stats, int reset_counters)
+{
+    if (!m || !stats) return EPIC_E_INVALID;
+    if (m->n > EPIC_MAX_PARCELS)
+        return fail(m, EPIC_E_INVALID, "epic_parcel_diagnostics_noalloc: more parcels than EPIC_MAX_PARCELS");
+    for (int i = 0; i < 18; ++i) stats[i] = 0.0;
+    if (m->n > 0) {
+        double *sp = nullptr;
+        if (scratch_on_device(&sp) != EPIC_OK)
+            return fail(m, EPIC_E_HIP, "epic_parcel_diagnostics_noalloc: no device address for the scratch array");
+        // tendencies a re-sort left in their slots are read through tend_src
+        // (in place of epic_parcel_diagnostics' gather, which allocates)
+        const int *src = m->tend_displaced ? m->tend_src : nullptr;
+        double *res = sp + NOALLOC_BLOCKS * 12;
+        launch_parcel_diagnostics(m->g, m->pd, src, sp, res, m->err_d, m->s, NOALLOC_BLOCKS);
+        HIPOK(m, hipGetLastError());
+        double s[12];
+        HIPOK(m, hipMemcpyAsync(s, res, sizeof(double) * 12, hipMemcpyDeviceToHost, m->s));
+        const int rc = check(m, "epic_parcel_diagnostics_noalloc");   // synchronises
+        if (rc) return rc;
+        parcel_stats_of(m, s, stats);
+    }
+    parcel_counts_of(m, stats, reset_counters);
     return EPIC_OK;
 }
 

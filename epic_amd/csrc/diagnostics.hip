@@ -44,7 +44,10 @@ __device__ inline double identity(int op)
 constexpr int NP = 12;
 __constant__ int c_pop[NP] = {SUM, SUM, SUM, SUM, SUM, SUM, SUM, SUM, SUM, SUM, MAX, MIN};
 
-__global__ __launch_bounds__(DT) void k_pdiag_partial(Grid g, ParcelsDev p, double *part, int *err)
+// src (may be null): the slot holding each parcel's tendencies (after a
+// re-sort that left them in place); a grid-stride loop, so any number of
+// blocks covers the parcels (one parcel per thread when nblocks(n) blocks)
+__global__ __launch_bounds__(DT) void k_pdiag_partial(Grid g, ParcelsDev p, const int *src, double *part, int *err)
 {
     __shared__ double s[NP][DT];
     int op[NP];
@@ -53,10 +56,10 @@ __global__ __launch_bounds__(DT) void k_pdiag_partial(Grid g, ParcelsDev p, doub
     double v[NP];
 #pragma unroll
     for (int c = 0; c < NP; ++c) v[c] = identity(op[c]);
-    const int64_t n = (int64_t)blockIdx.x * DT + threadIdx.x;
-    if (n < p.n) {
+    for (int64_t n = (int64_t)blockIdx.x * DT + threadIdx.x; n < p.n; n += (int64_t)gridDim.x * DT) {
+        const int64_t t = src ? (int64_t)src[n] : n;
         const double vol = p.vol[n];
-        const double u0 = p.dpos[0][n], u1 = p.dpos[1][n], u2 = p.dpos[2][n];
+        const double u0 = p.dpos[0][t], u1 = p.dpos[1][t], u2 = p.dpos[2][t];
         const double w0 = p.vor[0][n], w1 = p.vor[1][n], w2 = p.vor[2][n];
         const double B[5] = {p.B[0][n], p.B[1][n], p.B[2][n], p.B[3][n], p.B[4][n]};
         double b33;
@@ -68,18 +71,20 @@ __global__ __launch_bounds__(DT) void k_pdiag_partial(Grid g, ParcelsDev p, doub
         double D[3], V[3][3];
         scherzinger(A, D, V);
         const double lam = sqrt(D[0] / D[2]);   // get_aspect_ratio (parcel_ellipsoid.f90:423-429)
-        v[0] = (u0 * u0 + u1 * u1 + u2 * u2) * vol;
-        v[1] = (w0 * w0 + w1 * w1 + w2 * w2) * vol;
-        v[2] = lam;
-        v[3] = lam * lam;
-        v[4] = vol;
-        v[5] = vol * vol;
-        v[6] = (vol <= g.vmin) ? 1.0 : 0.0;
-        v[7] = vol * w0 * w0;
-        v[8] = vol * w1 * w1;
-        v[9] = vol * w2 * w2;
-        v[10] = p.buoy[n];
-        v[11] = p.buoy[n];
+        const double x[NP] = {(u0 * u0 + u1 * u1 + u2 * u2) * vol,
+                              (w0 * w0 + w1 * w1 + w2 * w2) * vol,
+                              lam,
+                              lam * lam,
+                              vol,
+                              vol * vol,
+                              (vol <= g.vmin) ? 1.0 : 0.0,
+                              vol * w0 * w0,
+                              vol * w1 * w1,
+                              vol * w2 * w2,
+                              (double)p.buoy[n],
+                              (double)p.buoy[n]};
+#pragma unroll
+        for (int c = 0; c < NP; ++c) v[c] = (op[c] == SUM) ? v[c] + x[c] : (op[c] == MAX) ? fmax(v[c], x[c]) : fmin(v[c], x[c]);
     }
     block_reduce<NP>(v, op, s);
     if (threadIdx.x < NP) part[(int64_t)blockIdx.x * NP + threadIdx.x] = s[threadIdx.x][0];
@@ -167,11 +172,11 @@ int64_t diag_scratch_doubles(const Grid &g, int64_t cap)
     return std::max<int64_t>((int64_t)nblocks(cap) * NP, (int64_t)nblocks(nn) * NF) + NP + NF;
 }
 
-void launch_parcel_diagnostics(const Grid &g, const ParcelsDev &p, double *scratch, double *out, int *err,
-                               hipStream_t s)
+void launch_parcel_diagnostics(const Grid &g, const ParcelsDev &p, const int *src, double *scratch, double *out, int *err,
+                               hipStream_t s, int max_blocks)
 {
-    const int nb = nblocks(p.n);
-    hipLaunchKernelGGL(k_pdiag_partial, dim3(nb), dim3(DT), 0, s, g, p, scratch, err);
+    const int nb = max_blocks > 0 ? std::min(nblocks(p.n), max_blocks) : nblocks(p.n);
+    hipLaunchKernelGGL(k_pdiag_partial, dim3(nb), dim3(DT), 0, s, g, p, src, scratch, err);
     hipLaunchKernelGGL(k_diag_total<NP>, dim3(1), dim3(DT), 0, s, scratch, nb, out);
 }
 

@@ -220,8 +220,10 @@ void add_time_step(Plan &P, int phase, const std::vector<TsArgs> &items);
 int64_t diag_scratch_doubles(const Grid &g, int64_t cap);
 // out[0..11]: sums of vol |u|^2, vol |w|^2, lam, lam^2, vol, vol^2, small,
 // vol xi^2, vol eta^2, vol zeta^2; max and min buoyancy
-void launch_parcel_diagnostics(const Grid &g, const ParcelsDev &p, double *scratch, double *out, int *err,
-                               hipStream_t s);
+// parcel diagnostics partials (at most max_blocks blocks when positive) and
+// totals into out; src (may be null) maps a slot to the slot of its tendencies
+void launch_parcel_diagnostics(const Grid &g, const ParcelsDev &p, const int *src, double *scratch, double *out, int *err,
+                               hipStream_t s, int max_blocks = 0);
 // out[0..9]: sums of (volg - vcell)^2, nparg, nsparg, weighted volg |u|^2,
 // volg |w|^2; max |volg - vcell|, nparg, tbuoyg; min nparg, tbuoyg
 void launch_field_diagnostics(const Grid &g, const float *fld, const int *nparg, const int *nsparg, double *scratch,

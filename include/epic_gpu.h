@@ -204,6 +204,18 @@ int epic_apply_vortcor(epic_handle h);
  * reset_counters != 0 clears them (parcel_diagnostics_netcdf.f90:181-183);
  * nbig_iclo is not tracked (0).  Sums are FP64 in a fixed order. */
 int epic_parcel_diagnostics(epic_handle h, double *stats, int reset_counters);
+/* No counterpart in the reference: epic_parcel_diagnostics for callers that
+ * cannot allocate device or pinned memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same stats[18] and
+ * counters; calls no hipMalloc, hipFree, hipHostMalloc or other allocating
+ * routine.  Its reduction partials live in one fixed device array per GPU
+ * (1,024 blocks x 12 doubles plus the totals, 98,432 bytes, held until the
+ * process ends; a grid-stride loop covers any parcel count up to
+ * EPIC_MAX_PARCELS, beyond which it returns EPIC_E_INVALID).  The sums are
+ * FP64 in a fixed order, which differs from epic_parcel_diagnostics' above
+ * 262,144 parcels (agreement to FP64 rounding).  Calls that overlap on one
+ * GPU share that array and race. */
+int epic_parcel_diagnostics_noalloc(epic_handle h, double *stats, int reset_counters);
 /* calculate_field_diagnostics (field_diagnostics.f90:40-154) on one rank
  * from volg, nparg, nsparg, tbuoyg, velog and vortg (after epic_par2grid and
  * epic_vor2vel): stats[11] = rms_v, avg_npar, avg_nspar, keg, apeg (0),
