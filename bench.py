@@ -1,18 +1,20 @@
 """Benchmark: parcel-steps/s of the full LS-RK4 step on one MI355X, plain vs
 batched vs served.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-cpu] [--eager]
                     [--forms plain,batched,served] [--members 64] [--ens-steps 20]
                     [--requests 12] [--rates 1,8,64] [--req-steps 50,500]
 
 plain (the headline ``value``): rising thermal, 128x128x128 grid, 8 parcels
 per cell (16,777,216 parcels), fixed dt = 0.01, synthetic seeded inputs,
 state resident in HBM.  One step = 5 stages of {par2grid, vor2vel,
-vorticity_tendency, fused grid2par + dB/dt + push} + reflective BC.  The
-first step from the generated state runs with eager launches bracketed by
-HIP events (per-kernel times for the roofline, the same step the counter
-summaries under profiles/ come from); then W warm-up and K timed steps of
-the captured step graph.
+vorticity_tendency, fused grid2par + dB/dt + push} + reflective BC.  Step
+1 from the generated state (its result gives the error table), W warm-up
+steps, then K timed steps: replays of the captured step graph, captured
+with an event pair around every phase, so the per-kernel times of the
+roofline are those of the timed steps themselves (device wall-clock stamps
+around every phase inside the graph; --eager: the timed steps launch
+eagerly between HIP event pairs, as the counter runs under profiles/ do).
 
 batched: 64 members with grids drawn from {32,48,64,96}^2 x {32,64}, 8
 parcels per cell, each with its own seed, 20 steps of one batched step (one
@@ -308,8 +310,9 @@ def main():
                     help="the large moist member's statistics (humidity plume, 32 buoyancy blobs) instead of the "
                          "rising thermal; with --size 256: 134M parcels")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=1,
-                    help="eager steps with per-kernel HIP-event timing, from the generated state, before warm-up")
+    ap.add_argument("--eager", action="store_true",
+                    help="the timed steps launch their kernels eagerly between event pairs instead of replaying the "
+                         "captured step graph (for counter collection, which is not run on graph replays)")
     ap.add_argument("--forms", default="plain,batched,served")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=20)
@@ -337,35 +340,34 @@ def main():
     t = 0.0
     want_err = not args.no_cpu
 
-    # step 1 (and any further profile steps) with eager launches and HIP
-    # events on the member's stream: per-kernel device times
-    gm.set_profiling(True)
-    gm.timers(reset=True)
-    snap = None
-    for i in range(args.profile_steps):
-        t = gm.ls_rk_step(t, dt)
-        if i == 0 and want_err:
-            snap = gpu_snapshot(gm)
-    tm = gm.timers(reset=True)
-    gm.set_profiling(False)
-    nprof = max(1, args.profile_steps)
-    phases = {k: round(v[0] / nprof, 4) for k, v in tm.items() if v[1]}
-    log(f"bench: {args.profile_steps} profiled step(s): {phases}")
-
+    # step 1 from the generated state: its result gives the error table
+    # against the FP64 oracle's step 1
+    t = gm.ls_rk_step(t, dt)
+    snap = gpu_snapshot(gm) if want_err else None
     for _ in range(args.warmup):
         t = gm.ls_rk_step(t, dt)
-    log(f"bench: {args.warmup} warm-up steps done")
+    log(f"bench: step 1 and {args.warmup} warm-up steps done")
+    # the timed steps: graph replays whose capture holds device wall-clock
+    # stamps around every phase (or eager launches between event pairs with
+    # --eager), so the per-kernel times come from these very steps
+    gm.set_profiling(True, graph=not args.eager)
+    gm.timers(reset=True)
     times = []
     for _ in range(args.steps):
         a = time.perf_counter()
         t = gm.ls_rk_step(t, dt)   # returns after the step's work has completed
         times.append(time.perf_counter() - a)
+    tm = gm.timers(reset=True)
+    gm.set_profiling(False)
     times = np.array(times) if times else np.full(1, np.nan)
     ms = float(times.mean() * 1e3)
-    log(f"bench: {args.steps} timed steps, mean {ms:.2f} ms")
+    nprof = max(1, args.steps)
+    phases = {k: round(v[0] / nprof, 4) for k, v in tm.items() if v[1]}
+    log(f"bench: {args.steps} timed steps, mean {ms:.2f} ms; per step {phases}")
     gm.close()
 
-    # roofline of the tile kernels from their event times (average launch)
+    # roofline of the tile kernels from their event times over the timed
+    # steps (average launch)
     measured = args.size == 128 and not args.moist
     kern = {}
     for name, timer, nbytes in (("k_g2p_tiles", "push", BYTES_PUSH), ("k_p2g_tiles", "p2g_tiles", BYTES_P2G)):
@@ -384,10 +386,11 @@ def main():
     roof = dict(kern[dominant]) if dominant else {}
     if dominant:
         roof["kernel"] = dominant
-        roof["measured_on"] = ("step 1 from the generated state, eager launches, HIP events on the member's "
-                               "stream; traffic from rocprofv3 counters on the same step (" +
-                               os.path.relpath(TRAFFIC_SUMMARY, ROOT) + ")")
-        roof["limiter"] = "VALU issue (per-kernel counters in the profiles), below the HBM bound"
+        roof["measured_on"] = ("the timed steps: device wall-clock stamps around every phase captured into the "
+                               "replayed step graph on the member's stream" +
+                               (" (eager launches between HIP events)" if args.eager else "") +
+                               "; traffic from rocprofv3 counters (" + os.path.relpath(TRAFFIC_SUMMARY, ROOT) + ")")
+        roof["limiter"] = "memory latency and VALU issue (per-kernel counters in the profiles), below the HBM bound"
     if args.copy_calibration:
         device_copy_calibration()
     line = {
@@ -409,7 +412,7 @@ def main():
                                f"8 parcels/cell ({mem.n} parcels), "
                                f"plain LS-RK4 step, dt=0.01; FP32 storage, FP64 positions/eigen/z-transforms",
                    "grid": [mem.nx, mem.ny, mem.nz], "parcels": mem.n, "form": "plain"},
-        "phases_ms_step1": phases,
+        "phases_ms_per_step": phases,
         "roofline": roof,
         "roofline_kernels": kern,
     }

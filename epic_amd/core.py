@@ -298,8 +298,11 @@ class Member:
 
     TIMERS = ("par2grid", "cell_sort", "vor2vel", "vorticity_tendency", "push", "grid2par", "axes", "p2g_tiles")
 
-    def set_profiling(self, on=True):
-        self._chk(load_library().epic_set_profiling(self._h, int(bool(on))))
+    def set_profiling(self, on=True, graph=False):
+        """Per-phase GPU timers: eager launches between event pairs, or (graph)
+        the captured step graph replayed with device wall-clock stamps
+        around every phase."""
+        self._chk(load_library().epic_set_profiling(self._h, (2 if graph else 1) if on else 0))
 
     def timers(self, reset=True):
         ms = np.zeros(len(self.TIMERS))

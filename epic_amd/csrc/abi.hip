@@ -65,7 +65,7 @@ struct epic_member {
     double2 *twz = nullptr;
     FftPlan px{}, py{}, pz{};
     // per-phase GPU timers (events on the member stream)
-    bool profiling = false;
+    int profiling = 0;   // 1: eager launches between event pairs; 2: graph replays with wall-clock stamps per phase
     std::vector<hipEvent_t> ev;        // pairs (start, stop) recorded in order
     std::vector<int> ev_phase;
     double timers_ms[EPIC_NUM_TIMERS] = {};
@@ -290,12 +290,9 @@ int grid_blocks(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192
 
 // after a stream synchronisation: fold the recorded event pairs into the
 // public timers (a par2grid is its cell sort plus the tile scatter)
-void collect_timers(epic_member *m)
+static void add_phase_time(epic_member *m, int phase, float ms)
 {
-    for (size_t k = 0; k < m->ev_phase.size(); ++k) {
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, m->ev[2 * k], m->ev[2 * k + 1]) == hipSuccess) {
-            switch (m->ev_phase[k]) {
+            switch (phase) {
             case PH_AXES: m->timers_ms[EPIC_TIMER_AXES] += ms; m->timer_calls[EPIC_TIMER_AXES] += 1; break;
             case PH_SORT:
                 m->timers_ms[EPIC_TIMER_CELL_SORT] += ms;
@@ -314,12 +311,27 @@ void collect_timers(epic_member *m)
             case PH_GRID2PAR: m->timers_ms[EPIC_TIMER_GRID2PAR] += ms; m->timer_calls[EPIC_TIMER_GRID2PAR] += 1; break;
             default: break;
             }
-        }
+}
+
+void collect_timers(epic_member *m)
+{
+    for (size_t k = 0; k < m->ev_phase.size(); ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, m->ev[2 * k], m->ev[2 * k + 1]) == hipSuccess) add_phase_time(m, m->ev_phase[k], ms);
         (void)hipEventDestroy(m->ev[2 * k]);
         (void)hipEventDestroy(m->ev[2 * k + 1]);
     }
     m->ev.clear();
     m->ev_phase.clear();
+}
+
+// after a synchronised replay of a graph captured with timers
+static int collect_graph_timers(epic_member *m, Plan &P)
+{
+    std::vector<std::pair<int, float>> t;
+    HIPOK(m, P.graph_phase_times(t));
+    for (const auto &pt : t) add_phase_time(m, pt.first, pt.second);
+    return EPIC_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1300,16 +1312,20 @@ static int run_step(epic_member *m, double *t, double t_initial, bool adaptive, 
     m->tm_h->gmax_bits = m->tm_h->b2max_bits = 0ull;
     HIPOK(m, hipMemcpyAsync(m->flags_d, m->err_h + 1, sizeof(int), hipMemcpyHostToDevice, m->s));
     HIPOK(m, hipMemcpyAsync(m->tm_d, m->tm_h, sizeof(StepTime), hipMemcpyHostToDevice, m->s));
-    if (m->profiling) {
+    if (m->profiling == 1) {
         rc = run_plan(m, P);
         if (rc) return rc;
     } else {
-        HIPOK(m, P.run_graph(m->s));
+        HIPOK(m, P.run_graph(m->s, m->profiling == 2));
     }
     HIPOK(m, hipMemcpyAsync(m->tm_h, m->tm_d, sizeof(StepTime), hipMemcpyDeviceToHost, m->s));
     m->tend_displaced = false;   // every stage rewrote the tendencies in place
     rc = check(m, where);   // synchronises the stream
     if (rc) return rc;
+    if (m->profiling == 2) {
+        rc = collect_graph_timers(m, P);
+        if (rc) return rc;
+    }
     *dt_taken = m->tm_h->dt;
     *t = *t + m->tm_h->dt;
     if (m->resort_every > 0 && ++m->steps_since_sort >= m->resort_every) return resort(m);
@@ -1388,8 +1404,8 @@ int epic_get_time_step(epic_handle m, double t, const epic_time_t *opts, double 
 
 int epic_set_profiling(epic_handle m, int on)
 {
-    if (!m) return EPIC_E_INVALID;
-    m->profiling = (on != 0);
+    if (!m || on < 0 || on > 2) return EPIC_E_INVALID;
+    m->profiling = on;
     return EPIC_OK;
 }
 

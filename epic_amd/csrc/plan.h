@@ -88,10 +88,15 @@ public:
     // enqueues every launch on s; with ev != nullptr records an event pair
     // around every run of launches of one phase (ev_phase gets the phases)
     hipError_t run(hipStream_t s, std::vector<hipEvent_t> *ev = nullptr, std::vector<int> *ev_phase = nullptr);
-    // captures the launches into a graph (once) and launches it on s
-    hipError_t run_graph(hipStream_t s);
+    // captures the launches into a graph (once) and launches it on s; a
+    // timed capture puts a one-thread kernel that stores the device wall
+    // clock before and after every run of launches of one phase, so a replay
+    // times its phases (a change of `timed` captures again)
+    hipError_t run_graph(hipStream_t s, bool timed = false);
     // uploads and captures (once) without launching
-    hipError_t prepare_graph(hipStream_t s);
+    hipError_t prepare_graph(hipStream_t s, bool timed = false);
+    // after a timed replay has completed: (phase, ms) of every phase run
+    hipError_t graph_phase_times(std::vector<std::pair<int, float>> &out);
     void release();
     ~Plan() { release(); }
     Plan() = default;
@@ -123,6 +128,13 @@ private:
     bool dev_ready_ = false;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
+    bool timed_ = false;
+    unsigned long long *stamps_ = nullptr;   // device: two wall-clock stamps per phase run
+    std::vector<int> stamp_phase_;
+    std::vector<unsigned long long> stamps_h_;
+    double clock_khz_ = 0.0;
+    void drop_graph();
+    hipError_t run_stamped(hipStream_t s);
 };
 
 }  // namespace epic

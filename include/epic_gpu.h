@@ -329,8 +329,12 @@ const char *epic_diagonalise_last_error(void);
 
 /* Per-phase GPU timers, the device counterpart of the reference's named
  * wall-clock timers (mpi_timer.f90:53-104, names registered in
- * epic3d.f90:79-95).  With profiling on, every phase is bracketed by HIP
- * events on the handle's stream; epic_get_timers returns the accumulated
+ * epic3d.f90:79-95).  With profiling on (on = 1), every phase is bracketed
+ * by HIP events on the handle's stream and epic_ls_rk_step launches its
+ * kernels eagerly; with on = 2 the step stays one graph replay, captured
+ * with a one-thread kernel storing the device wall clock before and after
+ * every phase, so the timers measure the replayed step itself (other entry
+ * points time as with on = 1).  epic_get_timers returns the accumulated
  * milliseconds and call counts per phase (ms/calls may be NULL) and resets
  * them when reset != 0.  "push" is the fused grid2par + dB/dt + RK push
  * kernel of ls_rk_step; "grid2par" is the stand-alone grid2par. */
