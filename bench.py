@@ -24,8 +24,9 @@ max-relative error after its first batched step against the FP64 oracle.
 
 served: requests with grids drawn from {32,64,96,128}^2 x {32,64}, 50-500
 steps each, Poisson arrivals at each rate of the sweep, admitted against a
-200 GB HBM budget and continuously batched; per-tick step latency p50/p99 and
-parcel-steps/s (busy and wall clock).
+200 GB HBM budget and continuously batched, each member merging and
+splitting its parcels after every step (EPIC's run loop); per-tick step
+latency p50/p99 and parcel-steps/s (busy and wall clock).
 
 cpu_baseline: the FP64 oracle (a C restatement of the reference's algorithm)
 on the host cores: step 1 of the same member is the warm-up and gives the
@@ -63,6 +64,11 @@ BYTES_PUSH = 204
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_rising_thermal128.txt")
 
 DEFAULT_SEED = 20261202
+# the served runs take EPIC's run-loop parcel operations (merge, then split)
+# after every step of a member, as its run loop does: long runs need the
+# splits (with them every 10 steps, ellipsoids outgrow their neighbour cells
+# after about 300 steps)
+PARCEL_OPS_EVERY = 1
 
 
 def log(*a):
@@ -276,14 +282,15 @@ def bench_served(n_requests, rates, req_steps, seed0=DEFAULT_SEED + 1):
     sweep = []
     for rate in rates:
         reqs = poisson_requests(ws, rate=rate, steps=req_steps, dt=0.01, seed=seed0)
-        results, st = Server(budget, max_members=64).run(reqs)
+        results, st = Server(budget, max_members=64, parcel_ops_every=PARCEL_OPS_EVERY).run(reqs)
         failed = sum(1 for r in results.values() if r.status != 0)
+        failures = [f"request {r.rid} after {r.steps_done} steps: {r.message}" for r in results.values() if r.status != 0]
         sweep.append({"rate_per_s": rate, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "ticks": st["ticks"],
                       "parcel_steps": st["parcel_steps"],
                       "parcel_steps_per_s_busy": st["parcel_steps_per_s_busy"],
                       "parcel_steps_per_s_wall": st["parcel_steps_per_s_wall"],
                       "wall_s": st["wall_s"], "mean_members_per_tick": st["mean_members_per_tick"],
-                      "queue_wait_s_p50": st["queue_wait_s_p50"], "failed": failed})
+                      "queue_wait_s_p50": st["queue_wait_s_p50"], "failed": failed, "failures": failures[:4]})
         log(f"bench: served at {rate}/s: {st['parcel_steps_per_s_busy']:.3e} parcel-steps/s busy, "
             f"{st['parcel_steps_per_s_wall']:.3e} wall, p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms, "
             f"{st['mean_members_per_tick']:.1f} members per tick")
@@ -291,7 +298,7 @@ def bench_served(n_requests, rates, req_steps, seed0=DEFAULT_SEED + 1):
     return {
         "workload": f"{n_requests} requests, grids from {{32,64,96,128}}^2 x {{32,64}}, 8 parcels/cell, "
                     f"{req_steps[0]}-{req_steps[1]} RK4 steps each, Poisson arrivals, 200 GB HBM budget, "
-                    f"continuous batching",
+                    f"continuous batching, parcel merge + split after every {PARCEL_OPS_EVERY} step(s) of a member",
         "requests": n_requests, "rates_per_s": list(rates),
         "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
         "parcel_steps_per_s": top["parcel_steps_per_s_busy"],

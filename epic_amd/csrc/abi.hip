@@ -58,6 +58,11 @@ struct epic_member {
     float *fld = nullptr;
     int *nparg = nullptr, *nsparg = nullptr;
     double *staging = nullptr;
+    // par2grid support points beyond the neighbour cells: 64-bit fixed-point
+    // sums [field][plane][j][i], their flag and the fields' inverse scales
+    unsigned long long *far_acc = nullptr;
+    int *far_flag = nullptr;
+    double *far_scale = nullptr;
     // spectra
     float2 *spec[NSPEC] = {};   // NSPEC consecutive spectra of one allocation (spec[0])
     float2 *twx = nullptr, *twy = nullptr;
@@ -406,6 +411,7 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
     std::vector<PlaceArgs> pl;
     std::vector<P2GArgs> ti;
     std::vector<FinalizeArgs> fi;
+    std::vector<FarClearArgs> far;
     for (size_t i = 0; i < ms.size(); ++i) {
         epic_member *m = ms[i];
         const P2GSel &q = sel[i];
@@ -426,6 +432,9 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
         a.staging = m->staging;
         a.err = m->err_d;
         a.tmax = m->pmax_out;
+        a.far_acc = m->far_acc;
+        a.far_flag = m->far_flag;
+        a.far_scale = m->far_scale;
         a.tz = L.tz;
         a.ntx = L.ntx;
         a.nty = L.nty;
@@ -442,6 +451,10 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
         f.clear_count = q.clr >= 0 ? m->cnt[q.clr] : nullptr;
         f.clear_scount = q.clr >= 0 ? m->scnt[q.clr] : nullptr;
         f.clear_tmax = m->pmax_out;
+        f.far_acc = m->far_acc;
+        f.far_flag = m->far_flag;
+        f.far_scale = m->far_scale;
+        far.push_back(FarClearArgs{m->far_acc, m->far_flag, (int64_t)P2G_NF * field_stride(m->g)});
         f.tz = L.tz;
         f.ntx = L.ntx;
         f.nty = L.nty;
@@ -454,7 +467,7 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
     if (!mx.empty()) add_p2g_max(P, PH_P2G, mx);
     add_exclusive_scan(P, PH_SORT, sc);
     add_place(P, PH_SORT, pl);
-    add_p2g(P, PH_P2G, ti, fi);
+    add_p2g(P, PH_P2G, ti, fi, far);
 }
 
 FftXArgs xargs(epic_member *m)
@@ -859,7 +872,9 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
         return EPIC_E_NOMEM;
     const int64_t FS = field_stride(g);
     if (bad(dalloc(&m->fld, (int64_t)F_COUNT * FS, m->s)) || bad(dalloc(&m->nparg, FS, m->s)) || bad(dalloc(&m->nsparg, FS, m->s)) ||
-        bad(dalloc(&m->staging, staging_elems_max(g), m->s)))
+        bad(dalloc(&m->staging, staging_elems_max(g), m->s)) ||
+        bad(dalloc(&m->far_acc, (int64_t)P2G_NF * FS, m->s)) || bad(dalloc(&m->far_flag, 1, m->s)) ||
+        bad(dalloc(&m->far_scale, P2G_NF, m->s)))
         return EPIC_E_NOMEM;
     static_assert(NSPEC >= 12, "the six FP64 spectra of vorticity_tendency take twelve float2 spectra");
     if (bad(dalloc(&m->spec[0], NSPEC * spec_size(g), m->s))) return EPIC_E_NOMEM;
@@ -910,7 +925,7 @@ int epic_destroy(epic_handle m)
     void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
                     m->cnt[2], m->scnt[2], m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
                     m->twx, m->twy, m->twz, m->twx64, m->twy64, m->err_d, m->flags_d, m->vw_d, m->tm_d, m->diag_d,
-                    m->sort_tmp, m->pmax_out, m->tend_src};
+                    m->sort_tmp, m->pmax_out, m->tend_src, m->far_acc, m->far_flag, m->far_scale};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->spec[0]) (void)hipFree(m->spec[0]);
@@ -1497,6 +1512,7 @@ int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels)
     b += 4 * (6 * nc + (nc + 1) + nc);                                        // histograms, cell_start, cursor
     b += 4 * scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nc, c), (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2)));
     b += 4 * ((int64_t)F_COUNT * FS + 2 * FS) + 8 * staging_elems_max(g);   // fields, counts, staging
+    b += 8 * (int64_t)P2G_NF * FS + 4 + 8 * P2G_NF;                          // far-point sums
     b += (int64_t)NSPEC * 8 * spec_size(g);                                  // spectra
     return b;
 }

@@ -276,11 +276,14 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
                     off[d] = (int)fl - cc[d];
                     w[d] = (float)(xyz - fl);
                 }
-                if (off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1 ||
-                    k + off[2] < -1 || k + off[2] > g.nz) {
+                if (k + off[2] < -1 || k + off[2] > g.nz) {   // beyond the halo layers (the reference fails too)
                     e |= ERR_OUT_OF_GRID;
                     continue;
                 }
+                // a point beyond the neighbour cells (strongly stretched
+                // parcels, rare) gathers its corners from the global fields,
+                // in the same order, so the sums are the same bits
+                const bool far = off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1;
                 // box coordinates of the point's lower corner
                 const int lx = cx + off[0] + 1, ly = cy + off[1] + 1, lp = k + off[2] - (k0 - 1);
                 const float pxc = 1.0f - w[0], pyc = 1.0f - w[1], pzc = 1.0f - w[2];
@@ -291,16 +294,28 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
                 float s[14];
 #pragma unroll
                 for (int f = 0; f < 14; ++f) s[f] = 0.0f;
+                if (!far) {
 #pragma unroll
-                for (int m = 0; m < 8; ++m) {
-                    const int dz = m & 1, dy = (m >> 1) & 1, dx = (m >> 2) & 1;
-                    const int nd = (lp + dz) * NN + (ly + dy) * P2G_BOX + (lx + dx);
-                    const float4 q0 = box[nd], q1 = box[NB + nd], q2 = box[2 * NB + nd];
-                    const float2 q3 = box2[nd];
-                    const float vv[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
+                    for (int m = 0; m < 8; ++m) {
+                        const int dz = m & 1, dy = (m >> 1) & 1, dx = (m >> 2) & 1;
+                        const int nd = (lp + dz) * NN + (ly + dy) * P2G_BOX + (lx + dx);
+                        const float4 q0 = box[nd], q1 = box[NB + nd], q2 = box[2 * NB + nd];
+                        const float2 q3 = box2[nd];
+                        const float vv[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                              q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
 #pragma unroll
-                    for (int f = 0; f < 14; ++f) s[f] += wt[m] * vv[f];
+                        for (int f = 0; f < 14; ++f) s[f] += wt[m] * vv[f];
+                    }
+                } else {
+#pragma unroll 1
+                    for (int m = 0; m < 8; ++m) {
+                        const int dz = m & 1, dy = (m >> 1) & 1, dx = (m >> 2) & 1;
+                        const int ii = wrap(cc[0] + off[0] + dx, g.nx), jj = wrap(cc[1] + off[1] + dy, g.ny);
+                        const int64_t o = (int64_t)(k + off[2] + dz + 1) * PS + (int64_t)jj * g.nx + ii;
+                        const EPIC_GLOBAL float *f0 = gl(a.fld) + (int64_t)F_VELOG * FS + o;
+#pragma unroll
+                        for (int f = 0; f < 14; ++f) s[f] += wt[m] * f0[(int64_t)f * FS];
+                    }
                 }
 #pragma unroll
                 for (int c = 0; c < 3; ++c) dp[c] += 0.25f * s[c];

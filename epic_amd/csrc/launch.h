@@ -61,6 +61,13 @@ struct P2GArgs {
     double *staging;            // exact partial node sums (scaled integers)
     int *err;
     const float *tmax;          // P2G_NF + 1 maxima from k_p2g_max (fixed-point scales)
+    // support points beyond the neighbour cells of their parcel's cell: their
+    // fixed-point terms go to far_acc ([field][plane k+1][j][i], 64-bit) by
+    // global atomics; far_flag is set when there are any, far_scale receives
+    // the fields' inverse scales for the finalize
+    unsigned long long *far_acc;
+    int *far_flag;
+    double *far_scale;
     int tz, ntx, nty, nchz;
 };
 // largest |contribution| per field (and |tbuoy - buoy|) over a member's parcels
@@ -81,10 +88,20 @@ struct FinalizeArgs {
     int *nparg, *nsparg;
     int *clear_count, *clear_scount;   // histograms cleared for the next push (may be null)
     float *clear_tmax;                 // the tile kernel's maxima, zeroed for the next k_p2g_max (may be null)
+    const unsigned long long *far_acc; // the tile kernel's far-point sums (read when *far_flag)
+    const int *far_flag;
+    const double *far_scale;
     int tz, ntx, nty, nchz;
 };
+// clears far_acc and then far_flag after a finalize that read them
+struct FarClearArgs {
+    unsigned long long *far_acc;
+    int *far_flag;
+    int64_t n;                         // P2G_NF * field_stride
+};
 void add_axes(Plan &P, int phase, const std::vector<AxesArgs> &items);
-void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin);
+void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin,
+             const std::vector<FarClearArgs> &far);
 
 // ---- grid2par + push (g2p.hip) ----
 struct G2PArgs {

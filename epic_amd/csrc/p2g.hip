@@ -388,11 +388,11 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
     constexpr int lplane = P2G_BOX * lp;          // words per LDS node plane
     constexpr int lfield = nbox * lplane;         // words per field
     extern __shared__ unsigned long long acc[];   // [field][plane][y][x]
-    __shared__ int s_err;
+    __shared__ int s_err, s_far;
     __shared__ unsigned short s_items[2 * TZ][64];   // per wave: the pass's items (source lane, first point)
 
     for (int u = tid; u < P2G_NF * lfield; u += blockDim.x) acc[u] = 0ull;
-    if (tid == 0) s_err = 0;
+    if (tid == 0) s_err = s_far = 0;
 
     // this thread's cell and its share of the cell's parcels: the P2G_TPC
     // threads of a cell are neighbouring lanes, so one load instruction
@@ -458,8 +458,11 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
         for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
         // the four support points: target cell (box corner index, -1 when
         // outside the neighbour cells) and cell fractions
+        // tgt: >= 0 box corner; -1 none; < -1 a point beyond the neighbour
+        // cells, -2 - its cell offsets (7 bits each, biased by 64)
         int tgt[4];
         float fr[4][3];
+        unsigned farm = 0;   // bit jp: point jp lies beyond the neighbour cells
 #pragma unroll
         for (int jp = 0; jp < 4; ++jp) {
             const double csn = support_cos(jp), snn = support_sin(jp);
@@ -473,11 +476,43 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
                 fr[jp][d] = (float)(xyz - fl);
             }
             tgt[jp] = -1;
-            if (off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1 ||
-                cc[2] + off[2] < -1 || cc[2] + off[2] > g.nz) {
+            if (cc[2] + off[2] < -1 || cc[2] + off[2] > g.nz) {   // beyond the halo layers (the reference fails too)
                 if (have) err |= ERR_OUT_OF_GRID;
+            } else if (off[0] < -1 || off[0] > 1 || off[1] < -1 || off[1] > 1 || off[2] < -1 || off[2] > 1) {
+                if (have && off[0] >= -64 && off[0] < 64 && off[1] >= -64 && off[1] < 64 && off[2] >= -64 && off[2] < 64) {
+                    farm |= 1u << jp;   // beyond the neighbour cells: global accumulator, below
+                    tgt[jp] = -2 - (((off[2] + 64) << 14) | ((off[1] + 64) << 7) | (off[0] + 64));
+                } else if (have) {
+                    err |= ERR_OUT_OF_GRID;   // more than 64 cells away
+                }
             } else {   // lower corner of the point's cell in the box
                 tgt[jp] = (lz + off[2] + 1) * lplane + (cy + off[1] + 1) * lp + cx + off[0] + 1;
+            }
+        }
+        // support points beyond the neighbour cells (strongly stretched
+        // parcels, rare): each point's 8 x 7 fixed-point terms go to the
+        // member's global accumulator by 64-bit atomics (exact integer sums,
+        // so still independent of the order), which the finalize adds
+        if (farm) {
+            s_far = 1;
+            const int64_t FS = field_stride(g);
+#pragma unroll
+            for (int jp = 0; jp < 4; ++jp) {
+                if (!((farm >> jp) & 1u)) continue;
+                const int code = -2 - tgt[jp];
+                const int gi[3] = {cc[0] + (code & 127) - 64, cc[1] + ((code >> 7) & 127) - 64, cc[2] + (code >> 14) - 64};
+                const float *f3 = fr[jp];
+                const float wxy[4] = {(1.0f - f3[1]) * (1.0f - f3[0]), (1.0f - f3[1]) * f3[0], f3[1] * (1.0f - f3[0]),
+                                      f3[1] * f3[0]};
+#pragma unroll 1
+                for (int c = 0; c < 8; ++c) {
+                    const int ii = wrap(gi[0] + (c & 1), g.nx), jj = wrap(gi[1] + ((c >> 1) & 1), g.ny);
+                    const int kk = gi[2] + (c >> 2);   // in [-1, nz + 1]
+                    const int64_t node = ((int64_t)(kk + 1) * g.ny + jj) * g.nx + ii;
+                    const double wd = (double)(((c & 4) ? f3[2] : 1.0f - f3[2]) * wxy[c & 3]);
+#pragma unroll
+                    for (int f = 0; f < P2G_NF; ++f) atomicAdd(a.far_acc + f * FS + node, fixed_term(wd, csd[f]));
+                }
             }
         }
         // own cell: the summed weights (in point order) of the points in it,
@@ -546,7 +581,11 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
             float csf[P2G_NF];
 #pragma unroll
             for (int f = 0; f < P2G_NF; ++f) csf[f] = have ? v[f] * sc[f] : 0.0f;
-            const int t01 = (tgt[0] & 0xffff) | (tgt[1] << 16), t23 = (tgt[2] & 0xffff) | (tgt[3] << 16);
+            // box corners in 16 bits each; no target (also a far point) as -1
+            int tb[4];
+#pragma unroll
+            for (int jq = 0; jq < 4; ++jq) tb[jq] = tgt[jq] >= 0 ? tgt[jq] : -1;
+            const int t01 = (tb[0] & 0xffff) | (tb[1] << 16), t23 = (tb[2] & 0xffff) | (tb[3] << 16);
             for (int r = 0; r < total; r += 64) {
                 unsigned lm = leadm;
 #pragma unroll
@@ -624,16 +663,24 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
         stage[u] = (double)q * inv[fs];
     }
     if (tid == 0 && s_err) atomicOr(a.err, s_err);
+    if (tid == 0 && s_far) atomicOr(a.far_flag, 1);
+    if (ib.blk == 0 && tid < P2G_NF) a.far_scale[tid] = inv[tid];   // the same in every work-group
 }
 
 // ---------------------------------------------------------------------------
 // 6. staging sum + folds + normalisation (one thread per (i, j) column)
 // ---------------------------------------------------------------------------
-// adds the staged sums of node (i, j, k) to out[]
-__device__ inline void stage_add(const FinalizeArgs &a, int i, int j, int k, double out[P2G_NF])
+// adds the staged sums of node (i, j, k) to out[], and its far-point sums
+// when there are any (far)
+__device__ inline void stage_add(const FinalizeArgs &a, int i, int j, int k, double out[P2G_NF], bool far)
 {
     constexpr int PLANE = P2G_BOX * P2G_BOX * P2G_NF;
     const Grid &g = a.g;
+    if (far) {
+        const int64_t FS = field_stride(g), o = ((int64_t)(k + 1) * g.ny + j) * g.nx + i;
+#pragma unroll
+        for (int f = 0; f < P2G_NF; ++f) out[f] += (double)fixed_sum(a.far_acc[f * FS + o]) * a.far_scale[f];
+    }
     const int nbox = a.tz + 3;
     const int cza = (k + 1) / a.tz;
     const int txa = i / P2G_TX, tya = j / P2G_TY;
@@ -676,6 +723,7 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     const int64_t tq = ib.blk * (int64_t)blockDim.x + threadIdx.x;
     if (a.clear_tmax && tq <= P2G_NF) gl(a.clear_tmax)[tq] = 0.0f;   // the tile kernel has read them
     if (tq >= ncol * (nz + 3)) return;
+    const bool far = *a.far_flag != 0;   // k_far_clear resets it after this kernel
     const int k = (int)(tq / ncol) - 1;
     const int64_t col = tq - (int64_t)(k + 1) * ncol;
     const int i = (int)(col % g.nx), j = (int)(col / g.nx);
@@ -689,12 +737,12 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
     auto folded = [&](int kk, double r[P2G_NF]) {
 #pragma unroll
         for (int f = 0; f < P2G_NF; ++f) r[f] = 0.0;
-        stage_add(a, i, j, kk, r);
+        stage_add(a, i, j, kk, r, far);
         if (kk == 0 || kk == nz) {
 #pragma unroll
             for (int f = 0; f < P2G_NF; ++f) r[f] *= 2.0;
         } else if (kk == 1 || kk == nz - 1) {
-            stage_add(a, i, j, kk == 1 ? -1 : nz + 1, r);
+            stage_add(a, i, j, kk == 1 ? -1 : nz + 1, r, far);
         }
     };
     auto zero_zeta = [&](int kk) { return (kk == 0 && g.zeta_zero[0]) || (kk == nz && g.zeta_zero[1]); };
@@ -716,7 +764,7 @@ __global__ __launch_bounds__(256) void k_p2g_finalize(const FinalizeArgs *__rest
         double r[P2G_NF];
 #pragma unroll
         for (int f = 0; f < P2G_NF; ++f) r[f] = 0.0;
-        stage_add(a, i, j, k, r);
+        stage_add(a, i, j, k, r, far);
         F(F_VOLG, k) = (float)r[0];
         F(F_DBUOYG, k) = (float)r[5];
         F(F_HUMG, k) = (float)r[6];
@@ -842,7 +890,26 @@ P2GLayout p2g_layout(const Grid &g)
 
 size_t p2g_lds_bytes(int tz) { return sizeof(unsigned long long) * P2G_NF * box_planes(tz) * P2G_BOX * p2g_lpitch(tz); }
 
-void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin)
+// after a finalize: the far-point accumulator back to zero (only when the
+// tile kernel put terms there), then the flag
+constexpr int FAR_CLEAR_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void k_far_clear(const FarClearArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+{
+    const ItemBlock ib = item_block(bmap);
+    const FarClearArgs &a = tab[ib.item];
+    if (*a.far_flag == 0) return;   // uniform over the launch
+    for (int64_t q = (int64_t)ib.blk * 256 + threadIdx.x; q < a.n; q += (int64_t)FAR_CLEAR_BLOCKS * 256)
+        gl(a.far_acc)[q] = 0ull;
+}
+
+__global__ __launch_bounds__(64) void k_far_reset(const FarClearArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+{
+    const ItemBlock ib = item_block(bmap);
+    if (threadIdx.x == 0) gl(tab[ib.item].far_flag)[0] = 0;
+}
+
+void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin,
+             const std::vector<FarClearArgs> &far)
 {
     // one grouped launch per z chunk (the work-group size is 64 tz P2G_TPC),
     // the kernel specialised on it (epic_set_tiling and the autotuner admit
@@ -864,6 +931,9 @@ void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::v
     std::vector<int> nf;
     for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny * (f.g.nz + 3) + 255) / 256));
     P.add((const void *)k_p2g_finalize, 256, 0, phase, fin, nf);
+    std::vector<int> nc(far.size(), FAR_CLEAR_BLOCKS), nr(far.size(), 1);
+    P.add((const void *)k_far_clear, 256, 0, phase, far, nc);
+    P.add((const void *)k_far_reset, 64, 0, phase, far, nr);
 }
 
 }  // namespace epic

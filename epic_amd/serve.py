@@ -15,9 +15,16 @@ concurrently on the GPU, and while they run the server
   retired member's memory counts against the budget until it is released);
 * copies out the results of members retired at the previous tick.
 
-Then it waits for the tick (``Batch.wait``).  A member that has taken its
-requested steps, or whose step failed (its status is returned in the
-result), leaves its cohort; only that cohort's graph is rebuilt.  Device
+Then it waits for the tick (``Batch.wait``).  With ``parcel_ops_every = K``
+every member runs the parcel operations of EPIC's run loop after each K-th
+of its steps, merge then split (epic3d.f90:143-148), on its own stream;
+members are allocated with room for the children of the splits, and a
+cohort whose members changed their parcel count is captured again at its
+next launch.  Long runs need them: without splitting, ellipsoids stretch
+until their support points leave the neighbouring cells.  A member that
+has taken its requested steps, or whose step or parcel operation failed
+(its status is returned in the result), leaves its cohort; only that
+cohort's graph is rebuilt.  Device
 memory is released after the wait, when no step is in flight.  A member's
 arithmetic does not depend on the other members of its cohort or on the
 cohorts running beside it, so every result equals the member's run alone.
@@ -34,7 +41,7 @@ from collections import deque
 
 import numpy as np
 
-from .core import Batch, Member, member_bytes
+from .core import Batch, EpicError, Member, member_bytes
 
 
 @dataclasses.dataclass
@@ -83,15 +90,21 @@ class _Cohort:
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes: int, max_members: int = 64, autotune: bool = False):
+    def __init__(self, hbm_budget_bytes: int, max_members: int = 64, autotune: bool = False,
+                 parcel_ops_every: int = 0, capacity_factor: float = 2.0):
         self.budget = int(hbm_budget_bytes)
         self.max_members = int(max_members)
         self.autotune = autotune
+        self.parcel_ops_every = int(parcel_ops_every)
+        self.capacity_factor = float(capacity_factor) if parcel_ops_every else 1.0
         self._tuned = {}
+
+    def _capacity(self, req):
+        return int(req.workload.n * self.capacity_factor)
 
     def _bytes(self, req):
         w = req.workload
-        return member_bytes(w.nx, w.ny, w.nz, w.n)
+        return member_bytes(w.nx, w.ny, w.nz, self._capacity(req))
 
     def run(self, requests, clock=time.perf_counter, idle_sleep=1e-3):
         """Serves ``requests`` (any order; admitted by arrival time) to
@@ -117,7 +130,7 @@ class Server:
             while (waiting and waiting[0].arrival <= now and n_active() < self.max_members
                    and used + self._bytes(waiting[0]) <= self.budget):
                 req = waiting.popleft()
-                m = Member.from_workload(req.workload, rk_order=req.rk_order)
+                m = Member.from_workload(req.workload, rk_order=req.rk_order, max_parcels=self._capacity(req))
                 if self.autotune:
                     key = (req.workload.nx, req.workload.ny, req.workload.nz, req.workload.n)
                     if key not in self._tuned:
@@ -179,7 +192,16 @@ class Server:
                     if st == 0:
                         e.t = float(t)
                         e.steps += 1
-                        stepped += e.req.workload.n
+                        stepped += e.member.n
+                        if (self.parcel_ops_every and e.steps % self.parcel_ops_every == 0
+                                and e.steps < e.req.steps):
+                            try:   # epic3d.f90:143-148: merge, then split
+                                e.member.parcel_merge()
+                                e.member.parcel_split()
+                            except EpicError as err:
+                                st = err.code
+                                e.status = int(err.code)
+                                e.message = str(err)
                     else:
                         e.status = int(st)
                         e.message = e.member.last_error()

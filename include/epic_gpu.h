@@ -62,7 +62,7 @@ enum {
     EPIC_E_HIP = -2,           /* HIP runtime failure */
     EPIC_E_PARCEL_COUNT = -3,  /* "par2grid: Wrong total number of parcels!" (parcel_interpl.f90:248-250) */
     EPIC_E_B33 = -4,           /* "get_B33: Division by small number!" (parcel_ellipsoid.f90:391-394) */
-    EPIC_E_OUT_OF_GRID = -5,   /* a support point left the node grid / its neighbour cells */
+    EPIC_E_OUT_OF_GRID = -5,   /* a support point left the z halo layers, or lies more than 64 cells from its parcel */
     EPIC_E_RK_ORDER = -6,      /* "Only third and fourth order RK supported." (ls_rk.f90:76-78) */
     EPIC_E_DT = -7,            /* time step must be positive (rk_utils.f90:179-182) */
     EPIC_E_NOMEM = -8,

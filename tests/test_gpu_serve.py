@@ -46,6 +46,39 @@ def test_served_results_equal_alone():
             assert np.array_equal(got.parcels[k], p[k]), (r.rid, k)
 
 
+def _alone_with_parcel_ops(req, every, capacity):
+    m = Member.from_workload(req.workload, rk_order=req.rk_order, max_parcels=capacity)
+    t = 0.0
+    for s in range(1, req.steps + 1):
+        t = m.ls_rk_step(t, req.dt)
+        if s % every == 0 and s < req.steps:   # epic3d.f90:143-148
+            m.parcel_merge()
+            m.parcel_split()
+    p = m.get_parcels()
+    m.close()
+    return p, t
+
+
+def test_served_parcel_operations_equal_alone():
+    """Merge and split every second step of each member (the run loop's
+    parcel operations): members change their parcel counts, their cohort is
+    captured again, and every result still equals the member run alone."""
+    ws = [W.rising_thermal(16, seed=51), W.rising_thermal(24, ny=16, nz=16, seed=52), W.rising_thermal(16, ny=24, seed=53)]
+    reqs = [Request(0, ws[0], 5, 0.01, 0.0), Request(1, ws[1], 4, 0.01, 0.0), Request(2, ws[2], 6, 0.01, 0.02)]
+    results, stats = Server(1 << 34, max_members=2, parcel_ops_every=2).run(reqs)
+    assert sorted(results) == [0, 1, 2] and stats["failed"] == 0
+    grown = 0
+    for r in reqs:
+        p, t = _alone_with_parcel_ops(r, 2, 2 * r.workload.n)
+        got = results[r.rid]
+        assert got.status == 0, got.message
+        assert abs(got.t - t) < 1e-15
+        grown += got.parcels["volume"].shape[0] != r.workload.n
+        for k in ("position", "vorticity", "B", "volume"):
+            assert np.array_equal(got.parcels[k], p[k]), (r.rid, k)
+    assert grown > 0   # the splits changed parcel counts
+
+
 def test_poisson_arrivals_and_autotuned_tiles():
     ws = [W.rising_thermal(16, seed=40 + i) for i in range(4)]
     reqs = poisson_requests(ws, rate=50.0, steps=(1, 3), dt=0.01, seed=7)
