@@ -108,7 +108,16 @@ struct epic_member {
     // p2g's fixed-point maxima (P2G_NF + 1)
     float *pmax_out = nullptr;
     Plan pl_resort;
+    // the parcel count in device memory (and its pinned host staging word)
+    int64_t *n_dev = nullptr, *n_h = nullptr;
+    // device scratch of the parcel merge (grown to the largest need seen)
+    ScratchArena merge_scratch;
     uint64_t gen = 0;   // bumped whenever plans built over this member go stale
+    // the member generation the step plans were built at: a stale step plan
+    // is rebuilt, and when the rebuilt launch sequence is the same (a new
+    // parcel count, say) only its argument tables are replaced, so the
+    // captured graph is kept
+    uint64_t step_gen[2] = {0, 0};
     void invalidate_plans()
     {
         ++gen;
@@ -119,13 +128,26 @@ struct epic_member {
         pl_vtend.release();
         pl_g2p[0].release();
         pl_g2p[1].release();
-        pl_step.release();
-        pl_step_adaptive.release();
         pl_lap[0].release();
         pl_lap[1].release();
         pl_resort.release();
         pl_settle.release();
         pl_unsort.release();
+    }
+    void release_plans()
+    {
+        invalidate_plans();
+        pl_step.release();
+        pl_step_adaptive.release();
+    }
+    // the parcel count changed (set_parcels, split, merge): the step plans
+    // read the count from n_dev and stay valid; the plans of the single
+    // calls hold it in their tables and are rebuilt on their next use
+    void counts_changed()
+    {
+        const uint64_t keep = gen;
+        invalidate_plans();
+        gen = keep;
     }
 };
 
@@ -176,19 +198,26 @@ template <class F> void host_parallel_for(int64_t n, F fn)
     for (std::thread &x : th) x.join();
 }
 
-bool factor23(int n)
+// lengths the Stockham stages cover: 2^a 3^b 5^c (stafft.f90:128-190
+// factorises 2, 3, 4, 5, 6)
+bool factor235(int n)
 {
     if (n < 1) return false;
     while (n % 2 == 0) n /= 2;
     while (n % 3 == 0) n /= 3;
+    while (n % 5 == 0) n /= 5;
     return n == 1;
 }
 
-void bind_parcels(epic_member *m)
+// the parcel views of the member's arrays and its count, also into device
+// memory (n_dev), which the step plans' kernels read
+hipError_t bind_parcels(epic_member *m)
 {
     ParcelsDev &p = m->pd;
     const int64_t c = m->cap;
     p.n = m->n;
+    p.cap = c;
+    p.n_dev = m->n_dev;
     p.x = m->pos;
     p.y = m->pos + c;
     p.z = m->pos + 2 * c;
@@ -210,6 +239,10 @@ void bind_parcels(epic_member *m)
         p.veta[i] = m->ax_stale[i];
         p.vtau[i] = m->ax_stale[3 + i];
     }
+    *m->n_h = m->n;
+    hipError_t e = hipMemcpyAsync(m->n_dev, m->n_h, sizeof(int64_t), hipMemcpyHostToDevice, m->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->s);
+    return e;
 }
 
 void collect_timers(epic_member *m);
@@ -417,7 +450,7 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
         const P2GSel &q = sel[i];
         const int64_t nc = ncells(m->g);
         sc.push_back(ScanArgs{m->cnt[q.rd], nc, m->cell_start, m->bsum, m->cursor});
-        pl.push_back(PlaceArgs{m->n, m->keys, m->cell_start, m->cursor, m->perm});
+        pl.push_back(PlaceArgs{m->n, m->keys, m->cell_start, m->cursor, m->perm, m->cap, m->n_dev});
         const P2GLayout L = layout_tz(m->g, m->tz_p2g);
         P2GArgs a{};
         a.g = m->g;
@@ -463,7 +496,7 @@ void build_p2g(Plan &P, const Members &ms, const std::vector<P2GSel> &sel)
     }
     std::vector<P2GMaxArgs> mx;
     for (size_t i = 0; i < ms.size(); ++i)
-        if (!sel[i].max_from_push) mx.push_back(P2GMaxArgs{ms[i]->g, ms[i]->pd, p2g_max_blocks(ms[i]->n), ms[i]->pmax_out});
+        if (!sel[i].max_from_push) mx.push_back(P2GMaxArgs{ms[i]->g, ms[i]->pd, p2g_max_blocks(ms[i]->cap), ms[i]->pmax_out});
     if (!mx.empty()) add_p2g_max(P, PH_P2G, mx);
     add_exclusive_scan(P, PH_SORT, sc);
     add_place(P, PH_SORT, pl);
@@ -816,8 +849,8 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
 {
     if (!grid || !phys || !flags || !out || max_parcels < 0) return EPIC_E_INVALID;
     *out = nullptr;
-    if (grid->nx < 8 || grid->ny < 8 || grid->nx % 8 || grid->ny % 8 || grid->nz < 4) return EPIC_E_INVALID;
-    if (!factor23(grid->nx / 2) || !factor23(grid->ny) || !factor23(2 * grid->nz)) return EPIC_E_INVALID;
+    if (grid->nx < 8 || grid->ny < 8 || grid->nx % 2 || grid->nz < 4) return EPIC_E_INVALID;
+    if (!factor235(grid->nx / 2) || !factor235(grid->ny) || !factor235(2 * grid->nz)) return EPIC_E_INVALID;
     if (flags->rk_order != 3 && flags->rk_order != 4) return EPIC_E_RK_ORDER;
     for (int i = 0; i < 3; ++i)
         if (!(grid->extent[i] > 0.0)) return EPIC_E_INVALID;
@@ -912,7 +945,9 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
         m->ax_cur[i] = m->axbuf + i * c;
         m->ax_stale[i] = m->axbuf + (6 + i) * c;
     }
-    bind_parcels(m);
+    if (bad(dalloc(&m->n_dev, 1, m->s)) || bad(hipHostMalloc((void **)&m->n_h, sizeof(int64_t), hipHostMallocDefault)))
+        return EPIC_E_NOMEM;
+    if (bad(bind_parcels(m))) return EPIC_E_HIP;
     *out = m;
     return EPIC_OK;
 }
@@ -921,11 +956,12 @@ int epic_destroy(epic_handle m)
 {
     if (!m) return EPIC_OK;
     if (m->s) (void)hipStreamSynchronize(m->s);
-    m->invalidate_plans();
+    m->release_plans();   // before the stream goes: released arenas wait for it
+    if (m->merge_scratch.base) (void)hipFree(m->merge_scratch.base);
     void *ptrs[] = {m->pos, m->attr, m->orig, m->axbuf, m->keys, m->cnt[0], m->scnt[0], m->cnt[1], m->scnt[1],
                     m->cnt[2], m->scnt[2], m->cell_start, m->cursor, m->perm, m->bsum, m->fld, m->nparg, m->nsparg, m->staging,
                     m->twx, m->twy, m->twz, m->twx64, m->twy64, m->err_d, m->flags_d, m->vw_d, m->tm_d, m->diag_d,
-                    m->sort_tmp, m->pmax_out, m->tend_src, m->far_acc, m->far_flag, m->far_scale};
+                    m->sort_tmp, m->pmax_out, m->tend_src, m->far_acc, m->far_flag, m->far_scale, m->n_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->spec[0]) (void)hipFree(m->spec[0]);
@@ -933,6 +969,7 @@ int epic_destroy(epic_handle m)
     if (m->vw_h) (void)hipHostFree(m->vw_h);
     if (m->tm_h) (void)hipHostFree(m->tm_h);
     if (m->diag_h) (void)hipHostFree(m->diag_h);
+    if (m->n_h) (void)hipHostFree(m->n_h);
     if (m->s) (void)hipStreamDestroy(m->s);
     delete m;
     return EPIC_OK;
@@ -967,8 +1004,8 @@ int epic_set_parcels(epic_handle m, int64_t n, const double *position, const dou
         }
     });
     m->n = n;
-    bind_parcels(m);
-    m->invalidate_plans();   // the plans hold the parcel count
+    HIPOK(m, bind_parcels(m));
+    m->counts_changed();
     for (int d = 0; d < 3 && n; ++d)
         HIPOK(m, hipMemcpyAsync(m->pos + d * c, hp.data() + d * n, sizeof(double) * n, hipMemcpyHostToDevice, m->s));
     for (int a = 0; a < 11 && n; ++a)
@@ -1319,7 +1356,16 @@ static int run_step(epic_member *m, double *t, double t_initial, bool adaptive, 
     int rc = ensure_keys(m);
     if (rc) return rc;
     Plan &P = adaptive ? m->pl_step_adaptive : m->pl_step;
+    uint64_t &pg = m->step_gen[adaptive ? 1 : 0];
+    if (!P.empty() && pg != m->gen) {   // stale: new tables into the same graph, or a new plan
+        Plan Q;
+        build_step(Q, {StepItem{m, m->tm_d, m->flags_d, adaptive}});
+        bool same = false;
+        HIPOK(m, P.refresh_from(Q, m->s, &same));
+        if (!same) P.swap(Q);   // Q now holds the old plan and releases it
+    }
     if (P.empty()) build_step(P, {StepItem{m, m->tm_d, m->flags_d, adaptive}});
+    pg = m->gen;
     // the first par2grid of a step reuses the cached axes (ls_rk.f90:93)
     m->err_h[1] = (*t > t_initial) ? 1 : 0;
     m->tm_h->t = *t;
@@ -1513,6 +1559,7 @@ int64_t epic_member_bytes(const epic_grid_t *grid, int64_t max_parcels)
     b += 4 * scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nc, c), (int64_t)(g.nx + 3) * (g.ny + 3) * (g.nz + 2)));
     b += 4 * ((int64_t)F_COUNT * FS + 2 * FS) + 8 * staging_elems_max(g);   // fields, counts, staging
     b += 8 * (int64_t)P2G_NF * FS + 4 + 8 * P2G_NF;                          // far-point sums
+    b += 20 * c + 4096;                                                       // merge scratch (its first phase)
     b += (int64_t)NSPEC * 8 * spec_size(g);                                  // spectra
     return b;
 }
@@ -1671,15 +1718,21 @@ static int batch_prepare(epic_batch *b)
     bool same = b->gens.size() == (size_t)nm;
     for (int i = 0; same && i < nm; ++i) same = (b->gens[i] == b->ms[i]->gen);
     if (b->plan.empty() || !same) {
-        b->plan.release();
+        // a member changed: the step rebuilt; when its launch sequence is the
+        // same (new parcel counts, say) the captured graph keeps running on
+        // the new argument tables
+        Plan Q;
         std::vector<StepItem> items;
         std::vector<ErrArgs> errs;
         for (int i = 0; i < nm; ++i) {
             items.push_back(StepItem{b->ms[i], b->tms_d + i, b->flags_d + i, false});
             errs.push_back(ErrArgs{b->ms[i]->err_d, b->errs_d + i});
         }
-        build_step(b->plan, items);
-        add_gather_errors(b->plan, errs);
+        build_step(Q, items);
+        add_gather_errors(Q, errs);
+        bool kept = false;
+        if (!b->plan.empty()) BHIPOK(b, b->plan.refresh_from(Q, b->s, &kept));
+        if (!kept) b->plan.swap(Q);
         b->gens.clear();
         for (int i = 0; i < nm; ++i) b->gens.push_back(b->ms[i]->gen);
     }
@@ -1793,8 +1846,8 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
         add_split_apply(P, PH_OTHER, {a});
         HIPOK(m, P.run(m->s));
         m->n = n0 + total;
-        bind_parcels(m);
-        m->invalidate_plans();
+        HIPOK(m, bind_parcels(m));
+        m->counts_changed();
         // the cached support-point axes of every parcel from its current B
         // (what the next correction's vol2grid stores, parcel_interpl.f90:92-106)
         Plan Q;
@@ -1803,6 +1856,12 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
         rc = check(m, "epic_parcel_split");
         if (rc) return rc;
     }
+    // the cell keys and histogram of the new parcel set now, in the same
+    // wait, rather than before the next step
+    rc = ensure_keys(m);
+    if (rc) return rc;
+    rc = check(m, "epic_parcel_split");
+    if (rc) return rc;
     if (n_split) *n_split = total;
     m->n_splits += total;
     return EPIC_OK;
@@ -1826,6 +1885,7 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
     c.attr = m->attr;
     c.cap = m->cap;
     c.nattr = NATTR;
+    c.scratch = &m->merge_scratch;
     int64_t nm = 0;
     const int r = run_merge(c, &nm);
     if (r < 0) return fail(m, EPIC_E_HIP, "epic_parcel_merge: HIP failure");
@@ -1833,12 +1893,14 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
     if (rc) return rc;
     if (nm > 0) {
         m->n -= nm;
-        bind_parcels(m);
-        m->invalidate_plans();
+        HIPOK(m, bind_parcels(m));
+        m->counts_changed();
         m->keys_valid = false;
         Plan Q;
         build_axes(Q, {m}, {m->ax_stale});
         HIPOK(m, Q.run(m->s));
+        rc = ensure_keys(m);   // keys and histogram of the new parcel set
+        if (rc) return rc;
         rc = check(m, "epic_parcel_merge");
         if (rc) return rc;
     }

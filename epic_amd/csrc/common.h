@@ -72,6 +72,10 @@ struct Grid {
 // parcel attribute arrays (n entries each)
 struct ParcelsDev {
     int64_t n;
+    int64_t cap;                // allocated slots: grids over the parcels are sized by it (0: by n), so
+                                // a plan stays valid when the count changes
+    const int64_t *n_dev;       // the count in device memory (may be null): the kernels of the step
+                                // plans read it, so their tables do not hold the count
     double *x, *y, *z;
     float *vor[3];
     float *B[5];
@@ -81,6 +85,10 @@ struct ParcelsDev {
     float *strain[8];           // only written by the stand-alone grid2par
     int *orig;                  // caller index of each physical slot
 };
+
+// the parcel count a kernel works on: the device copy when the argument
+// tables leave it there
+__device__ inline int64_t parcel_count(const ParcelsDev &p) { return p.n_dev ? *p.n_dev : p.n; }
 
 struct FftPlan {
     int n;          // complex length

@@ -1,4 +1,4 @@
-// Batched complex FFT over LDS (Stockham autosort, radix 4/2/3) shared by
+// Batched complex FFT over LDS (Stockham autosort, radix 4/2/3/5) shared by
 // the horizontal FP32 transforms and the FP64 vertical transforms.
 #pragma once
 
@@ -18,7 +18,7 @@ template <typename T2> __device__ inline T2 cconj(T2 a) { return T2{a.x, -a.y}; 
 template <typename T2> __device__ inline T2 cmuli(T2 a, bool inv) { return inv ? T2{-a.y, a.x} : T2{a.y, -a.x}; }
 template <typename T2, typename R> __device__ inline T2 cscale(T2 a, R s) { return T2{a.x * s, a.y * s}; }
 
-// In-place R-point DFT, R in {2, 3, 4}; forward uses e^{-2 pi i/R}.
+// In-place R-point DFT, R in {2, 3, 4, 5}; forward uses e^{-2 pi i/R}.
 template <int R, typename T2> __device__ inline void butterfly(T2 *v, bool inv)
 {
     using Rl = typename Cx<T2>::R;
@@ -33,7 +33,7 @@ template <int R, typename T2> __device__ inline void butterfly(T2 *v, bool inv)
         v[2] = csub(t0, t2);
         v[1] = cadd(t1, t3);
         v[3] = csub(t1, t3);
-    } else {   // R == 3
+    } else if constexpr (R == 3) {
         const Rl h = (Rl)0.5, s3 = (Rl)0.86602540378443864676;
         const T2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
         const T2 m = T2{v[0].x - h * s.x, v[0].y - h * s.y};
@@ -41,6 +41,22 @@ template <int R, typename T2> __device__ inline void butterfly(T2 *v, bool inv)
         v[0] = cadd(v[0], s);
         v[1] = cadd(m, rd);
         v[2] = csub(m, rd);
+    } else {   // R == 5: pairs (1,4) and (2,3) share cosines; the sines
+               // enter with -i (forward) on X1, X2 and +i on X4, X3
+        static_assert(R == 5, "radix 2, 3, 4 or 5");
+        const Rl c1 = (Rl)0.30901699437494742410, c2 = (Rl)-0.80901699437494742410;   // cos(2 pi/5), cos(4 pi/5)
+        const Rl s1 = (Rl)0.95105651629515357212, s2 = (Rl)0.58778525229247312917;    // sin(2 pi/5), sin(4 pi/5)
+        const T2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        const T2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        const T2 a1 = T2{v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+        const T2 a2 = T2{v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+        const T2 b1 = cmuli(T2{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, inv);
+        const T2 b2 = cmuli(T2{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, inv);
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(a1, b1);
+        v[4] = csub(a1, b1);
+        v[2] = cadd(a2, b2);
+        v[3] = csub(a2, b2);
     }
 }
 
@@ -101,7 +117,8 @@ __device__ T2 *lds_fft_tw(T2 *a, T2 *b, int L, const FftPlan &pl, const TW &tw, 
         const int R = pl.radix[st];
         if (R == 4) fft_stage<4>(in, out, L, n, Ns, tw, inv);
         else if (R == 2) fft_stage<2>(in, out, L, n, Ns, tw, inv);
-        else fft_stage<3>(in, out, L, n, Ns, tw, inv);
+        else if (R == 3) fft_stage<3>(in, out, L, n, Ns, tw, inv);
+        else fft_stage<5>(in, out, L, n, Ns, tw, inv);
         __syncthreads();
         T2 *tmp = in;
         in = out;
@@ -126,6 +143,7 @@ inline FftPlan make_plan(int n)
     while (m % 4 == 0) { p.radix[p.nst++] = 4; m /= 4; }
     while (m % 2 == 0) { p.radix[p.nst++] = 2; m /= 2; }
     while (m % 3 == 0) { p.radix[p.nst++] = 3; m /= 3; }
+    while (m % 5 == 0) { p.radix[p.nst++] = 5; m /= 5; }
     if (m != 1) p.nst = -1;   // unsupported length
     return p;
 }

@@ -194,11 +194,17 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
         }
     }
     const int nrow = nl * P2G_TY;
-    if (tid < nrow) {
+    if (tid < nrow) {   // a partial tile's rows are shorter (wx cells), or beyond the grid (empty)
         const int k = k0 + tid / P2G_TY, r = tid % P2G_TY;
-        const int c0 = ((k + 1) * g.ny + (j0 + r)) * g.nx + i0;
-        row_slot[tid] = gl(a.cell_start)[c0];
-        row_off[tid + 1] = gl(a.cell_start)[c0 + P2G_TX] - gl(a.cell_start)[c0];
+        const int wx = min(P2G_TX, g.nx - i0);
+        if (j0 + r < g.ny) {
+            const int c0 = ((k + 1) * g.ny + (j0 + r)) * g.nx + i0;
+            row_slot[tid] = gl(a.cell_start)[c0];
+            row_off[tid + 1] = gl(a.cell_start)[c0 + wx] - gl(a.cell_start)[c0];
+        } else {
+            row_slot[tid] = 0;
+            row_off[tid + 1] = 0;
+        }
     }
     __syncthreads();
     if (tid < 64) {   // inclusive scan of the row sizes in one wave
@@ -429,7 +435,7 @@ void add_rk_stage_tiles(Plan &P, int phase, const std::vector<G2PTileArgs> &item
     std::vector<int> nb;
     size_t lds = 0;
     for (const G2PTileArgs &a : items) {
-        nb.push_back(a.p.n > 0 ? a.ntx * a.nty * a.nchz : 0);
+        nb.push_back(a.ntx * a.nty * a.nchz);   // (empty rows: no work)
         lds = std::max(lds, g2p_lds_bytes(a.tz));
     }
     P.add((const void *)k_g2p_tiles, G2P_THREADS, lds, phase, items, nb);

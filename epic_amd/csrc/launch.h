@@ -30,6 +30,8 @@ struct PlaceArgs {
     int64_t n;
     const int *keys, *cell_start;
     int *cursor, *perm;
+    int64_t cap = 0;   // grid size in parcels (0: n)
+    const int64_t *n_dev = nullptr;   // when set, the count is read from here instead of n
 };
 int64_t scan_scratch_ints(int64_t n);
 void add_cell_keys(Plan &P, int phase, const std::vector<KeysArgs> &items);
@@ -194,6 +196,13 @@ void add_split_flag(Plan &P, int phase, const std::vector<SplitArgs> &items);
 void add_split_apply(Plan &P, int phase, const std::vector<SplitArgs> &items);
 
 // ---- parcel merging (merge.hip); direct launches on ctx.s ----
+// device scratch reused across calls: bump allocation from base; a call
+// that needs more takes the rest from hipMalloc and records its total in
+// want, and the next call grows base to it (freed by the owner)
+struct ScratchArena {
+    char *base = nullptr;
+    size_t cap = 0, want = 0;
+};
 struct MergeCtx {
     Grid g;
     ParcelsDev p;          // p.n: parcel count before the merge
@@ -204,6 +213,7 @@ struct MergeCtx {
     float *attr;           // the parcel attribute slab (nattr arrays of cap)
     int64_t cap;
     int nattr;
+    ScratchArena *scratch;   // the member's merge scratch
 };
 // 0: done (*n_merge merged), 1: stopped on a device error bit, -1: HIP failure
 int run_merge(const MergeCtx &c, int64_t *n_merge);

@@ -466,15 +466,43 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     hipStream_t s = c.s;
     *n_merge = 0;
     if (n == 0) return 0;
+    // scratch from the member's arena (hipFree would wait for the whole
+    // device, other members' steps included); an arena too small for this
+    // call is grown first to the largest need seen so far, at least 16
+    // bytes per parcel slot (the first phase's flags and ranks)
+    ScratchArena &sa = *c.scratch;
+    {
+        const size_t need = std::max(sa.want, (size_t)16 * (size_t)c.cap + 4096);
+        if (sa.cap < need) {
+            if (sa.base) {
+                (void)hipStreamSynchronize(s);
+                (void)hipFree(sa.base);
+            }
+            sa.base = nullptr;
+            sa.cap = 0;
+            const size_t grow = need + need / 4;
+            if (hipMalloc((void **)&sa.base, grow) != hipSuccess) return -1;
+            sa.cap = grow;
+        }
+    }
+    size_t used = 0;
     std::vector<void *> owned;
     auto alloc = [&](void **ptr, size_t bytes) -> hipError_t {
-        hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 4));
+        const size_t b = (std::max<size_t>(bytes, 4) + 255) & ~size_t(255);
+        if (used + b <= sa.cap) {
+            *ptr = sa.base + used;
+            used += b;
+            return hipSuccess;
+        }
+        used += b;   // counted for the next call's arena
+        hipError_t e = hipMalloc(ptr, b);
         if (e == hipSuccess) owned.push_back(*ptr);
         return e;
     };
     auto done = [&](int rc) {
         (void)hipStreamSynchronize(s);
         for (void *q : owned) (void)hipFree(q);
+        sa.want = std::max(sa.want, used);
         return rc;
     };
     auto host_int = [&](const int *dptr, int &v) -> hipError_t {

@@ -52,11 +52,12 @@ __global__ __launch_bounds__(256) void k_cell_keys(const KeysArgs *__restrict__ 
     const ItemBlock ib = item_block(bmap);
     const KeysArgs &a = tab[ib.item];
     const Grid &g = a.g;
+    const int64_t n = parcel_count(a.p);
     int e = 0;
 #pragma unroll
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
-        if (q >= a.p.n) break;
+        if (q >= n) break;
         const int key = cell_key(g, gl(a.p.x)[q], gl(a.p.y)[q], gl(a.p.z)[q], e);
         gl(a.keys)[q] = key;
         if (key >= 0) {
@@ -191,11 +192,12 @@ __global__ __launch_bounds__(256) void k_place(const PlaceArgs *__restrict__ tab
     // so a wave waits three memory round trips, not three per round
     constexpr int NR = PER_BLOCK / 256;   // every lane of a wave runs the same rounds
     const int lane = threadIdx.x & 63;
+    const int64_t n = a.n_dev ? *a.n_dev : a.n;
     int key[NR], start[NR], head_lane[NR], got[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
-        key[r] = (q < a.n) ? gl(a.keys)[q] : -1;
+        key[r] = (q < n) ? gl(a.keys)[q] : -1;
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) start[r] = (key[r] >= 0) ? gl(a.cell_start)[key[r]] : 0;
@@ -346,7 +348,8 @@ __global__ __launch_bounds__(256) void k_p2g_max(const P2GMaxArgs *__restrict__ 
     float mx[P2G_NF + 1];
 #pragma unroll
     for (int f = 0; f <= P2G_NF; ++f) mx[f] = 0.0f;
-    for (int64_t q = (int64_t)ib.blk * 256 + threadIdx.x; q < a.p.n; q += (int64_t)a.nblk * 256) {
+    const int64_t n = parcel_count(a.p);
+    for (int64_t q = (int64_t)ib.blk * 256 + threadIdx.x; q < n; q += (int64_t)a.nblk * 256) {
         float v[P2G_NF];
         p2g_values(a.g, a.p, (unsigned)q * 4u, gl(a.p.z)[q], v);
 #pragma unroll
@@ -401,7 +404,7 @@ __global__ __launch_bounds__(64 * TZ * P2G_TPC, 8 / TZ) void k_p2g_tiles(const P
     const int cx = ct & 7, cy = (ct >> 3) & 7, lz = ct >> 6;
     const int k = k0 + lz;
     int s0 = 0, s1 = 0;
-    if (k <= g.nz) {
+    if (k <= g.nz && i0 + cx < g.nx && j0 + cy < g.ny) {   // cells of a partial tile beyond the grid: none
         const int c = ((k + 1) * g.ny + j0 + cy) * g.nx + i0 + cx;
         s0 = gl(a.cell_start)[c];
         s1 = gl(a.cell_start)[c + 1];
@@ -810,11 +813,12 @@ __global__ __launch_bounds__(256) void k_axes(const AxesArgs *__restrict__ tab, 
     const ItemBlock ib = item_block(bmap);
     const AxesArgs &a = tab[ib.item];
     const ParcelsDev &p = a.p;
+    const int64_t n = parcel_count(p);
     int e = 0;
 #pragma unroll
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
-        if (q >= p.n) break;
+        if (q >= n) break;
         const double B[5] = {gl(p.B[0])[q], gl(p.B[1])[q], gl(p.B[2])[q], gl(p.B[3])[q], gl(p.B[4])[q]};
         double ve[3], vt[3];
         if (!support_axes(B, (double)gl(p.vol)[q], ve, vt)) {
@@ -840,17 +844,21 @@ static std::vector<int> parcel_blocks_of(const std::vector<int64_t> &n)
     return b;
 }
 
+// parcels a launch's grid covers: the allocated slots when known, so that a
+// plan stays the same launch sequence when the parcel count changes
+static int64_t grid_parcels(const ParcelsDev &p) { return std::max(p.n, p.cap); }
+
 void add_cell_keys(Plan &P, int phase, const std::vector<KeysArgs> &items)
 {
     std::vector<int64_t> n;
-    for (const KeysArgs &a : items) n.push_back(a.p.n);
+    for (const KeysArgs &a : items) n.push_back(grid_parcels(a.p));
     P.add((const void *)k_cell_keys, 256, 0, phase, items, parcel_blocks_of(n));
 }
 
 void add_place(Plan &P, int phase, const std::vector<PlaceArgs> &items)
 {
     std::vector<int64_t> n;
-    for (const PlaceArgs &a : items) n.push_back(a.n);
+    for (const PlaceArgs &a : items) n.push_back(std::max(a.n, a.cap));
     P.add((const void *)k_place, 256, 0, phase, items, parcel_blocks_of(n));
 }
 
@@ -866,7 +874,7 @@ int p2g_max_blocks(int64_t n) { return (int)std::min<int64_t>((n + 1023) / 1024,
 void add_axes(Plan &P, int phase, const std::vector<AxesArgs> &items)
 {
     std::vector<int64_t> n;
-    for (const AxesArgs &a : items) n.push_back(a.p.n);
+    for (const AxesArgs &a : items) n.push_back(grid_parcels(a.p));
     P.add((const void *)k_axes, 256, 0, phase, items, parcel_blocks_of(n));
 }
 
@@ -880,8 +888,10 @@ P2GLayout p2g_layout(const Grid &g)
 {
     P2GLayout L;
     L.tz = 2;
-    L.ntx = g.nx / P2G_TX;
-    L.nty = g.ny / P2G_TY;
+    // the last tile of a row or column is partial when nx or ny is not a
+    // multiple of 8: its cells beyond the grid hold no parcels
+    L.ntx = (g.nx + P2G_TX - 1) / P2G_TX;
+    L.nty = (g.ny + P2G_TY - 1) / P2G_TY;
     L.nchz = (g.nz + 2 + L.tz - 1) / L.tz;
     L.staging_elems = (int64_t)L.ntx * L.nty * L.nchz * (L.tz + 3) * P2G_BOX * P2G_BOX * P2G_NF;
     return L;

@@ -97,6 +97,13 @@ public:
     hipError_t prepare_graph(hipStream_t s, bool timed = false);
     // after a timed replay has completed: (phase, ms) of every phase run
     hipError_t graph_phase_times(std::vector<std::pair<int, float>> &out);
+    // when o is the same launch sequence as this uploaded plan (kernels,
+    // grids, work-group sizes, LDS and arena layout), takes o's argument
+    // tables into the device arena on s and keeps the captured graph, whose
+    // launches read the tables in place (*same = true); otherwise changes
+    // nothing (*same = false)
+    hipError_t refresh_from(const Plan &o, hipStream_t s, bool *same);
+    void swap(Plan &o);
     void release();
     ~Plan() { release(); }
     Plan() = default;
@@ -125,6 +132,7 @@ private:
     std::vector<Rec> recs_;
     char *dev_ = nullptr;
     size_t dev_size_ = 0;
+    hipStream_t last_stream_ = nullptr;   // the arena returns to the pool behind this stream's work
     bool dev_ready_ = false;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

@@ -2,7 +2,96 @@
 // (optionally with the phase timers captured as event-record nodes).
 #include "plan.h"
 
+#include <mutex>
+#include <utility>
+
 namespace epic {
+
+// Device arenas of released plans, for reuse: hipFree waits for the whole
+// device (every stream), which would stall the members' steps in flight
+// each time a transient plan (a parcel operation's launches) is dropped.
+// A block returns with an event recorded behind the last work that used it;
+// the next user's stream waits for that event before its upload.
+namespace {
+struct PoolBlock {
+    char *p;
+    size_t size;
+    int dev;
+    hipEvent_t ev;   // may be null (never launched)
+};
+std::mutex pool_mu;
+std::vector<PoolBlock> pool;
+constexpr size_t POOL_MAX_BLOCKS = 256;
+
+size_t pool_round(size_t n) { return (n + 4095) & ~size_t(4095); }
+
+hipError_t pool_get(size_t bytes, hipStream_t s, char **out, size_t *got)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const size_t want = pool_round(bytes);
+    {
+        std::lock_guard<std::mutex> lk(pool_mu);
+        size_t best = pool.size();
+        for (size_t i = 0; i < pool.size(); ++i)   // best fit, at most 4x the request
+            if (pool[i].dev == dev && pool[i].size >= want && pool[i].size <= 4 * want &&
+                (best == pool.size() || pool[i].size < pool[best].size))
+                best = i;
+        if (best < pool.size()) {
+            PoolBlock b = pool[best];
+            pool.erase(pool.begin() + (long)best);
+            if (b.ev) {
+                e = hipStreamWaitEvent(s, b.ev, 0);
+                (void)hipEventDestroy(b.ev);
+                if (e != hipSuccess) {
+                    (void)hipFree(b.p);
+                    return e;
+                }
+            }
+            *out = b.p;
+            *got = b.size;
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc((void **)out, want);
+    if (e != hipSuccess) return e;
+    *got = want;
+    return hipSuccess;
+}
+
+void pool_put(char *p, size_t size, hipStream_t last)
+{
+    if (!p) return;
+    int dev = 0;
+    hipEvent_t ev = nullptr;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipFree(p);
+        return;
+    }
+    if (last) {
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+        if (ev && hipEventRecord(ev, last) != hipSuccess) {
+            (void)hipEventDestroy(ev);
+            ev = nullptr;
+        }
+        if (!ev) {   // no event: wait here for the stream instead
+            (void)hipStreamSynchronize(last);
+        }
+    }
+    std::lock_guard<std::mutex> lk(pool_mu);
+    pool.push_back(PoolBlock{p, size, dev, ev});
+    while (pool.size() > POOL_MAX_BLOCKS) {   // the oldest block leaves the pool
+        PoolBlock b = pool.front();
+        pool.erase(pool.begin());
+        if (b.ev) {
+            (void)hipEventSynchronize(b.ev);
+            (void)hipEventDestroy(b.ev);
+        }
+        (void)hipFree(b.p);
+    }
+}
+}  // namespace
 
 void Plan::drop_graph()
 {
@@ -59,13 +148,20 @@ hipError_t Plan::upload(hipStream_t s)
     if (dev_ready_) return hipSuccess;
     drop_graph();
     if (host_.size() > dev_size_) {
-        if (dev_) (void)hipFree(dev_);
+        pool_put(dev_, dev_size_, last_stream_);
         dev_ = nullptr;
         dev_size_ = 0;
-        hipError_t e = hipMalloc((void **)&dev_, host_.size());
+        hipError_t e = pool_get(host_.size(), s, &dev_, &dev_size_);
         if (e != hipSuccess) return e;
-        dev_size_ = host_.size();
+    } else if (dev_ && last_stream_ && last_stream_ != s) {   // earlier launches on another stream read the arena
+        hipEvent_t ev = nullptr;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(ev, last_stream_);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
+        if (ev) (void)hipEventDestroy(ev);
+        if (e != hipSuccess) return e;
     }
+    last_stream_ = s;
     if (!host_.empty()) {
         hipError_t e = hipMemcpyAsync(dev_, host_.data(), host_.size(), hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -79,6 +175,7 @@ hipError_t Plan::run(hipStream_t s, std::vector<hipEvent_t> *ev, std::vector<int
 {
     hipError_t e = upload(s);
     if (e != hipSuccess) return e;
+    last_stream_ = s;
     int cur = -1;
     hipEvent_t stop = nullptr;
     for (const Rec &r : recs_) {
@@ -143,16 +240,64 @@ hipError_t Plan::run_graph(hipStream_t s, bool timed)
 {
     hipError_t e = prepare_graph(s, timed);
     if (e != hipSuccess) return e;
+    last_stream_ = s;
     return hipGraphLaunch(exec_, s);
+}
+
+hipError_t Plan::refresh_from(const Plan &o, hipStream_t s, bool *same)
+{
+    *same = false;
+    if (!dev_ready_ || !dev_ || recs_.size() != o.recs_.size() || host_.size() != o.host_.size()) return hipSuccess;
+    for (size_t i = 0; i < recs_.size(); ++i) {
+        const Rec &a = recs_[i], &b = o.recs_[i];
+        if (a.fn != b.fn || a.nblocks != b.nblocks || a.threads != b.threads || a.lds != b.lds || a.tab_off != b.tab_off ||
+            a.map_off != b.map_off || a.phase != b.phase)
+            return hipSuccess;
+    }
+    *same = true;
+    if (host_ == o.host_) return hipSuccess;
+    host_ = o.host_;
+    maps_ = o.maps_;
+    if (last_stream_ && last_stream_ != s) {   // the arena's readers so far ran on another stream
+        hipEvent_t ev = nullptr;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(ev, last_stream_);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
+        if (ev) (void)hipEventDestroy(ev);
+        if (e != hipSuccess) return e;
+    }
+    last_stream_ = s;
+    hipError_t e = hipMemcpyAsync(dev_, host_.data(), host_.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);   // host_ may change after this returns
+    return e;
+}
+
+void Plan::swap(Plan &o)
+{
+    std::swap(host_, o.host_);
+    std::swap(maps_, o.maps_);
+    std::swap(recs_, o.recs_);
+    std::swap(dev_, o.dev_);
+    std::swap(dev_size_, o.dev_size_);
+    std::swap(last_stream_, o.last_stream_);
+    std::swap(dev_ready_, o.dev_ready_);
+    std::swap(graph_, o.graph_);
+    std::swap(exec_, o.exec_);
+    std::swap(timed_, o.timed_);
+    std::swap(stamps_, o.stamps_);
+    std::swap(stamp_phase_, o.stamp_phase_);
+    std::swap(stamps_h_, o.stamps_h_);
+    std::swap(clock_khz_, o.clock_khz_);
 }
 
 void Plan::release()
 {
     drop_graph();
-    if (dev_) (void)hipFree(dev_);
+    pool_put(dev_, dev_size_, last_stream_);
     dev_ = nullptr;
     dev_size_ = 0;
     dev_ready_ = false;
+    last_stream_ = nullptr;
     host_.clear();
     maps_.clear();
     recs_.clear();

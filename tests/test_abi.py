@@ -28,11 +28,14 @@ def test_create_rejects_bad_grid_without_touching_the_gpu():
     if not os.path.exists(core.LIB_PATH):
         pytest.skip("libepic_gpu.so not built")
     lib = core.load_library()
-    g = core.Grid(12, 16, 16, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))   # nx not a multiple of 8
     p = core.Phys(0.0, 0.0, 0.0, (C.c_double * 3)(0, 0, 0))
     f = core.Flags((C.c_int * 2)(0, 0), 4, 0.0)
     h = C.c_void_p()
-    assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 10, C.byref(h)) == -1
+    # lengths with a factor 7 (stafft.f90 factorises 2, 3, 4, 5, 6), an odd
+    # nx (the x transform packs nx/2 complex points), and fewer than 8 cells
+    for nx, ny, nz in ((14, 16, 16), (16, 28, 16), (16, 16, 14), (15, 16, 16), (6, 16, 16), (16, 16, 3)):
+        g = core.Grid(nx, ny, nz, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
+        assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 10, C.byref(h)) == -1, (nx, ny, nz)
     g = core.Grid(16, 16, 16, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
     f = core.Flags((C.c_int * 2)(0, 0), 2, 0.0)   # unsupported RK order
     assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 10, C.byref(h)) == -6

@@ -59,3 +59,55 @@ def test_merge_then_split_then_step():
     assert gm.n == st.n
     assert np.abs(p["position"] - st.position).max() / max(mem.dx) <= 1e-5
     assert maxrel(p["vorticity"], st.vorticity) <= 1e-5
+
+
+def test_step_graph_follows_parcel_count_changes():
+    """A member keeps its captured step graph across merges and splits (the
+    parcel count changes, the launch sequence does not; only the argument
+    tables are replaced): each later step gives the same bits as a fresh
+    member holding the same parcels, and so does a batch of two members
+    whose counts both change between its steps."""
+    mem, _ = _with_small(W.rising_thermal(16, seed=75), 0.05, 76)
+    a = Member.from_workload(mem, max_parcels=2 * mem.n)
+    t = a.ls_rk_step(0.0, 0.01)
+    for step in range(3):
+        nm = a.parcel_merge()
+        ns = a.parcel_split(lambda_max=3.0)
+        p = a.get_parcels()
+        b = Member.from_workload(dataclasses.replace(mem, **p), max_parcels=2 * mem.n)
+        print(f"step {step}: merged {nm}, split {ns}, n {a.n}", flush=True)
+        ta = a.ls_rk_step(t, 0.01, t_initial=t)
+        tb = b.ls_rk_step(t, 0.01, t_initial=t)
+        assert ta == tb
+        pa, pb = a.get_parcels(), b.get_parcels()
+        for k in pa:
+            assert np.array_equal(pa[k], pb[k]), (step, k)
+        b.close()
+        t = ta
+    assert a.n != mem.n
+
+    from epic_amd.core import Batch
+
+    w1 = _with_small(W.rising_thermal(16, ny=24, seed=77), 0.05, 78)[0]
+    ws = (mem, w1)
+    ms = [Member.from_workload(w, max_parcels=2 * w.n) for w in ws]
+    bt = Batch(ms)
+    tt = np.zeros(2)
+    for step in range(3):
+        tt, st = bt.step(tt, [0.01, 0.01], t_initial=tt)
+        assert (st == 0).all()
+        for m in ms:
+            m.parcel_merge()
+            m.parcel_split(lambda_max=3.0)
+        refs = [Member.from_workload(dataclasses.replace(w, **m.get_parcels()), max_parcels=2 * w.n)
+                for w, m in zip(ws, ms)]
+        t0 = tt.copy()
+        tt, st = bt.step(tt, [0.01, 0.01], t_initial=tt)
+        assert (st == 0).all()
+        for i, (m, r) in enumerate(zip(ms, refs)):
+            assert r.ls_rk_step(t0[i], 0.01, t_initial=t0[i]) == tt[i]
+            got, want = m.get_parcels(), r.get_parcels()
+            for k in got:
+                assert np.array_equal(got[k], want[k]), (step, i, k)
+            r.close()
+    bt.close()

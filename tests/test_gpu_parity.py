@@ -31,6 +31,14 @@ _MAKERS = {
     # nx/2 = 24 and 48 complex points) and y
     "thermal48x32x32": lambda: W.rising_thermal(48, ny=32, nz=32, seed=7),
     "thermal96x48x32": lambda: W.rising_thermal(96, ny=48, nz=32, seed=8),
+    # radix-5 lengths in every transform (nx/2 = 20, ny = 40, 2 nz = 40)
+    "thermal40x40x20": lambda: W.rising_thermal(40, ny=40, nz=20, seed=9),
+    # partial tiles (nx, ny not multiples of 8: 8 + 8 + 4 and 4 x 8 + 4
+    # cells), radix 5 in x (nx/2 = 10) and z (2 nz = 20), radix 3 in y
+    "thermal20x36x10": lambda: W.rising_thermal(20, ny=36, nz=10, seed=10),
+    # fewer than 11 nodes across (a tile's node box wraps onto itself) and
+    # partial tiles in both directions
+    "thermal10x12x8": lambda: W.rising_thermal(10, ny=12, nz=8, seed=11),
 }
 
 
@@ -190,7 +198,8 @@ def test_grid2par_constant_fields():
 # fixed-point maxima stored by the pushes; moist16 over 3 steps, their
 # humidity bounds
 @pytest.mark.parametrize("name,nsteps", [("beltrami16", 2), ("thermal16", 4), ("moist16", 3), ("thermal32x16x24", 1),
-                                          ("thermal48x32x32", 2), ("thermal96x48x32", 1)])
+                                          ("thermal48x32x32", 2), ("thermal96x48x32", 1), ("thermal40x40x20", 2),
+                                          ("thermal20x36x10", 2), ("thermal10x12x8", 2)])
 def test_ls_rk_step(name, nsteps):
     mem = MEMBERS[name]
     st = oracle_state(mem)
