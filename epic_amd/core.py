@@ -12,9 +12,11 @@ when libepic_gpu.so is missing.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -56,6 +58,20 @@ class TimeOpts(C.Structure):
 
 
 _lib = None
+
+# handles still open when the interpreter exits are destroyed by an atexit
+# hook, while the HIP runtime (and a profiler's tool library around it) is
+# still up, never by garbage collection during interpreter teardown
+_live_batches = weakref.WeakSet()
+_live_members = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_handles():
+    for b in list(_live_batches):
+        b.close()
+    for m in list(_live_members):
+        m.close()
 
 
 def load_library(path=None):
@@ -171,6 +187,7 @@ class Member:
             raise EpicError(rc, "epic_create failed")
         self.nx, self.ny, self.nz = nx, ny, nz
         self.n = 0
+        _live_members.add(self)
 
     @classmethod
     def from_workload(cls, mem, rk_order=4, max_parcels=None):
@@ -533,6 +550,7 @@ class Batch:
         rc = L.epic_batch_create(arr, len(self.members), C.byref(self._h))
         if rc != 0:
             raise EpicError(rc, "epic_batch_create failed")
+        _live_batches.add(self)
 
     def step(self, t, dt, t_initial=None):
         """One step of every member; t, dt, t_initial are per-member sequences.

@@ -95,16 +95,12 @@ struct epic_member {
     // physical re-sort into cell order every resort_every steps (0: never)
     int resort_every = 3, steps_since_sort = 0;
     double *sort_tmp = nullptr;
-    // slot order: true while slot s holds caller index s (after set_parcels,
-    // split, merge); a re-sort makes it the cell order, which split and
-    // merge must not see (their pairings and child order follow the index)
-    bool caller_order = true;
     // the stage tendencies (dpos, dvor, dB, strain) are not moved by a
     // re-sort: while tend_displaced, the tendencies of the parcel in slot s
     // are in slot tend_src[s], until settle_tendencies gathers them
     bool tend_displaced = false;
     int *tend_src = nullptr;
-    Plan pl_settle, pl_unsort;
+    Plan pl_settle;
     // p2g's fixed-point maxima (P2G_NF + 1)
     float *pmax_out = nullptr;
     Plan pl_resort;
@@ -132,7 +128,6 @@ struct epic_member {
         pl_lap[1].release();
         pl_resort.release();
         pl_settle.release();
-        pl_unsort.release();
     }
     void release_plans()
     {
@@ -842,7 +837,6 @@ void add_gather_errors(Plan &P, const std::vector<ErrArgs> &items)
 extern "C" {
 
 static int settle_tendencies(epic_member *m);
-static int restore_caller_order(epic_member *m);
 
 int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_flags_t *flags, int64_t max_parcels,
                 epic_handle *out)
@@ -1013,7 +1007,6 @@ int epic_set_parcels(epic_handle m, int64_t n, const double *position, const dou
     if (n) HIPOK(m, hipMemcpyAsync(m->orig, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice, m->s));
     HIPOK(m, hipStreamSynchronize(m->s));
     m->keys_valid = false;
-    m->caller_order = true;
     m->tend_displaced = false;
     return EPIC_OK;
 }
@@ -1304,50 +1297,9 @@ static int resort(epic_member *m)
     if (rc) return rc;
     rc = displace_tendencies(m, m->perm);
     if (rc) return rc;
-    m->caller_order = false;
     return check(m, "epic_resort");
 }
 
-// gathers the persistent arrays back into the caller's order (slot s holds
-// caller index s again), the inverse of every re-sort since the last one
-static int restore_caller_order(epic_member *m)
-{
-    if (m->caller_order) return EPIC_OK;
-    m->caller_order = true;
-    if (m->n == 0) return EPIC_OK;
-    int rc = settle_tendencies(m);
-    if (rc) return rc;
-    rc = ensure_sort_scratch(m);
-    if (rc) return rc;
-    if (m->pl_unsort.empty()) {
-        Plan &P = m->pl_unsort;
-        const int64_t n = m->n, c = m->cap;
-        add_invert(P, PH_SORT, {InvertArgs{m->orig, n, m->perm}});
-        ResortArgs a{};
-        a.perm = m->perm;
-        a.n = n;
-        for (int d = 0; d < RESORT_W; ++d) {
-            a.w[d] = m->pos + d * c;
-            a.tw[d] = m->sort_tmp + d * c;
-        }
-        int k = 0;
-        for (int i = 0; i < 11; ++i) a.f[k++] = m->attr + i * c;
-        a.f[k++] = (float *)m->orig;
-        a.f[k++] = (float *)m->keys;
-        for (int i = 0; i < 6; ++i) a.f[k++] = m->ax_stale[i];
-        float *tf = (float *)(m->sort_tmp + RESORT_W * c);
-        for (int i = 0; i < RESORT_F; ++i) a.tf[i] = tf + i * c;
-        a.gather = 1;
-        add_resort(P, PH_SORT, {a});
-        a.gather = 0;
-        add_resort(P, PH_SORT, {a});
-    }
-    rc = run_plan(m, m->pl_unsort);
-    if (rc) return rc;
-    rc = displace_tendencies(m, m->perm);
-    if (rc) return rc;
-    return check(m, "restore_caller_order");
-}
 
 // one RK step of a member: the plan (fixed or adaptive) and its StepTime
 // written before the launch; *dt_taken receives the step's dt
@@ -1815,11 +1767,15 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
         amax = std::cbrt(0.75 / 3.14159265358979323846) * std::min(g.dx[0], std::min(g.dx[1], g.dx[2]));
     const int64_t n0 = m->n;
     if (n0 == 0) return EPIC_OK;
-    {   // children follow their parents' caller index (parcel_split.f90:87-160)
-        const int rc = restore_caller_order(m);
+    {   // the children's tendencies are written in their own slots
+        int rc = settle_tendencies(m);
+        if (rc == EPIC_OK) rc = ensure_sort_scratch(m);
         if (rc) return rc;
     }
-    // flags in keys, child slots (exclusive prefix, total at [n0]) in perm
+    // flags in keys, child slots (exclusive prefix, total at [n0]) in perm;
+    // the children follow their parents' caller index (parcel_split.f90:87-160):
+    // the flags moved to caller indices and their exclusive prefix, in sort_tmp
+    int *cflag = (int *)m->sort_tmp, *crank = cflag + m->cap + 1;
     SplitArgs a{};
     a.g = g;
     a.p = m->pd;
@@ -1827,11 +1783,14 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
     a.amax = amax;
     a.flag = m->keys;
     a.rank = m->perm;
+    a.crank = crank;
     a.err = m->err_d;
     {
         Plan P;
         add_split_flag(P, PH_OTHER, {a});
         add_exclusive_scan(P, PH_OTHER, {ScanArgs{m->keys, n0, m->perm, m->bsum, nullptr}});
+        add_to_caller(P, PH_OTHER, {ToCallerArgs{n0, m->orig, m->keys, cflag}});
+        add_exclusive_scan(P, PH_OTHER, {ScanArgs{cflag, n0, crank, m->bsum, nullptr}});
         HIPOK(m, P.run(m->s));
     }
     m->keys_valid = false;   // the keys array now holds the flags
@@ -1871,8 +1830,8 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
 {
     if (!m) return EPIC_E_INVALID;
     if (n_merge) *n_merge = 0;
-    {   // pairings, tie-breaks and the hole filling follow the caller index
-        const int rc = restore_caller_order(m);
+    {   // the slot compaction moves the tendencies with their parcels
+        const int rc = settle_tendencies(m);
         if (rc) return rc;
     }
     MergeCtx c{};

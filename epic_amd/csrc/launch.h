@@ -189,7 +189,8 @@ struct SplitArgs {
     ParcelsDev p;          // p.n: parcel count before the split
     double lambda_max, amax;
     int *flag;             // n entries
-    const int *rank;       // exclusive prefix of flag
+    const int *rank;       // exclusive prefix of flag (a child's slot: n + rank)
+    const int *crank;      // exclusive prefix of flag in caller-index order (its caller index: n + crank[orig])
     int *err;
 };
 void add_split_flag(Plan &P, int phase, const std::vector<SplitArgs> &items);
@@ -272,12 +273,14 @@ struct ResortArgs {
 // with_doubles: the RESORT_W double arrays move too; otherwise only the
 // RESORT_F float arrays (the stage tendencies: dpos 3, dvor 3, dB 5, strain 8)
 void add_resort(Plan &P, int phase, const std::vector<ResortArgs> &items, bool with_doubles = true);
-struct InvertArgs {
-    const int *orig;
+// out[orig[s]] = in[s] for s < n: a per-slot array in caller-index order
+struct ToCallerArgs {
     int64_t n;
-    int *inv;
+    const int *orig;
+    const int *in;
+    int *out;
 };
-void add_invert(Plan &P, int phase, const std::vector<InvertArgs> &items);
+void add_to_caller(Plan &P, int phase, const std::vector<ToCallerArgs> &items);
 
 // ---- utilities (abi.hip) ----
 struct FillArgs {

@@ -1,8 +1,13 @@
 // Parcel merging (parcel_merge.f90, parcel_nearest.f90) on the device, with
 // the semantics of the reference on one MPI rank.
 //
+// The reference's index of a parcel is its caller index (orig), whatever its
+// physical slot (a re-sort puts the slots in cell order): every order below
+// is the caller-index order, and the delete relabels caller indices as the
+// reference moves parcels, while the slots are compacted on their own.
+//
 // 1. small parcels: volume < vmin (is_small, parcel_ellipsoid.f90:452-459),
-//    compacted in index order
+//    compacted in caller-index order
 // 2. periodic images: a small parcel whose nearest grid index is 0 or n in x
 //    or y is sent to the rank itself (locate_parcel_in_boundary_cell,
 //    parcel_nearest.f90:1384-1460; apply_periodic_shift :564-598); images are
@@ -19,7 +24,8 @@
 //    order, as the reference's sequential loop does
 // 6. geometric merge (parcel_merge.f90:129-352) per target in FP64, summed
 //    in small-parcel order
-// 7. delete (parcel_container.f90:413-452): holes filled from the end
+// 7. delete (parcel_container.f90:413-452): holes filled from the end, in
+//    caller indices; the slots are compacted the same way
 // These kernels are launched directly (the merge is not part of the step).
 #include "common.h"
 #include "ellipsoid.h"
@@ -70,6 +76,20 @@ __global__ void k_compact(int64_t n, const int *flag, const int *rank, int *out)
 {
     const int64_t i = gtid();
     if (i < n && flag[i]) out[rank[i]] = (int)i;
+}
+
+// out[orig[i]] = in[i]: a per-slot array in caller-index order
+__global__ void k_to_caller(int64_t n, const int *orig, const int *in, int *out)
+{
+    const int64_t i = gtid();
+    if (i < n) out[orig[i]] = in[i];
+}
+
+// the flagged slots listed in caller-index order: out[crank[orig[i]]] = i
+__global__ void k_compact_caller(int64_t n, const int *flag, const int *orig, const int *crank, int *out)
+{
+    const int64_t i = gtid();
+    if (i < n && flag[i]) out[crank[orig[i]]] = (int)i;
 }
 
 __global__ void k_image_flags(Grid g, ParcelsDev p, const int *sml, int ns, int *imgflag)
@@ -157,9 +177,10 @@ __global__ void k_bin_keys(Grid g, Pts P, int64_t ntot, int64_t ncell, int *key,
 }
 
 // nearest other parcel; first minimum in the reference's visiting order
-// (cells iz, iy, ix ascending; inside a cell ascending index)
-__global__ void k_search(Grid g, Pts P, const int *sml, int ns, int nimg, const int *start, const int *node,
-                         const int *ipid, int *iclo, double *dclo)
+// (cells iz, iy, ix ascending; inside a cell ascending index: the caller
+// index of a parcel, images after every parcel in image order)
+__global__ void k_search(Grid g, Pts P, const int *orig, const int *sml, int ns, int nimg, const int *start,
+                         const int *node, const int *ipid, int *iclo, double *dclo)
 {
     const int64_t m = gtid();
     if (m >= (int64_t)ns + nimg) return;
@@ -177,7 +198,7 @@ __global__ void k_search(Grid g, Pts P, const int *sml, int ns, int nimg, const 
             for (int ix = max(0, ix0 - 1); ix <= min(nx + 1, ix0); ++ix) {
                 const int64_t c = (int64_t)ix + (int64_t)(nx + 3) * iy + (int64_t)(nx + 3) * (ny + 3) * iz;
                 double bd = 0.0;
-                int64_t bj = -1;
+                int64_t bj = -1, bk = -1;   // best entry and its reference index
                 for (int k = start[c]; k < start[c + 1]; ++k) {
                     const int64_t j = node[k];
                     if (j == is) continue;
@@ -186,9 +207,11 @@ __global__ void k_search(Grid g, Pts P, const int *sml, int ns, int nimg, const 
                     const double delz = zj - zs;
                     const double delx = xj - xs, dely = yj - ys;
                     const double dsq = delx * delx + dely * dely + delz * delz;
-                    if (bj < 0 || dsq < bd || (dsq == bd && j < bj)) {
+                    const int64_t kj = (j < P.n) ? (int64_t)orig[j] : j;
+                    if (bj < 0 || dsq < bd || (dsq == bd && kj < bk)) {
                         bd = dsq;
                         bj = j;
+                        bk = kj;
                     }
                 }
                 if (bj >= 0 && bd < dmin) {
@@ -420,12 +443,17 @@ __global__ void k_group_merge(Grid g, ParcelsDev p, const int *gstart, int *gent
     for (int c = 0; c < 3; ++c) p.vor[c][ic] = (float)wm[c];
 }
 
-// delete: hole k (k-th smallest merged index below n - nm) takes the k-th
-// valid parcel counted from the end
-__global__ void k_mark_holes(int nm, const int *pis, int *hole)
+// delete (parcel_container.f90:413-452), in caller indices: hole k (the
+// k-th smallest merged caller index below n - nm; the merged parcels pis[]
+// are in caller order) takes the k-th valid caller index counted from the
+// end.  The same rule on slots compacts the arrays: the k-th smallest
+// merged slot below n - nm takes the k-th valid slot from the end.
+__global__ void k_mark_holes(int nm, const int *pis, const int *orig, int *hole_c, int *hole_s)
 {
     const int64_t k = gtid();
-    if (k < nm) hole[pis[k]] = 1;
+    if (k >= nm) return;
+    hole_c[orig[pis[k]]] = 1;
+    hole_s[pis[k]] = 1;
 }
 __global__ void k_tail_flags(int64_t n, int nm, const int *hole, int *tflag)
 {   // tail positions from the top: t = n - 1 - u
@@ -433,25 +461,45 @@ __global__ void k_tail_flags(int64_t n, int nm, const int *hole, int *tflag)
     if (u >= nm) return;
     tflag[u] = hole[n - 1 - u] ? 0 : 1;
 }
-__global__ void k_fill_holes(int64_t n, int nm, const int *pis, const int *tflag, const int *trank, ParcelsDev p,
+// slot of every tail caller index t >= n - nm: tail_slot[t - (n - nm)]
+__global__ void k_tail_slots(int64_t n, int nm, const int *orig, int *tail_slot)
+{
+    const int64_t i = gtid();
+    if (i >= n) return;
+    const int64_t t = orig[i] - (n - nm);
+    if (t >= 0) tail_slot[t] = (int)i;
+}
+// the caller relabel: the k-th valid tail caller index from the top takes the
+// k-th merged caller index (its parcel stays in its slot for now)
+__global__ void k_relabel(int64_t n, int nm, const int *pis, const int *tflag_c, const int *trank_c,
+                          const int *tail_slot, int *orig)
+{
+    const int64_t u = gtid();
+    if (u >= nm || !tflag_c[u]) return;
+    const int k = trank_c[u];
+    orig[tail_slot[nm - 1 - u]] = orig[pis[k]];   // disjoint: valid parcels written, merged ones read
+}
+// slot compaction: hole k of hole_list (merged slots below n - nm, ascending)
+// takes the k-th valid slot from the top, with all its arrays and caller index
+__global__ void k_fill_holes(int64_t n, int nm, const int *hole_list, const int *tflag, const int *trank, ParcelsDev p,
                              float *attr, int64_t cap, int nattr)
 {
     const int64_t u = gtid();
     if (u >= nm || !tflag[u]) return;
     const int k = trank[u];          // k-th valid parcel from the top
-    const int64_t src = n - 1 - u, dst = pis[k];
+    const int64_t src = n - 1 - u, dst = hole_list[k];
     p.x[dst] = p.x[src];
     p.y[dst] = p.y[src];
     p.z[dst] = p.z[src];
     for (int a = 0; a < nattr; ++a) attr[a * cap + dst] = attr[a * cap + src];
+    p.orig[dst] = p.orig[src];
 }
-__global__ void k_wrap_all(Grid g, ParcelsDev p, int *orig)
+__global__ void k_wrap_all(Grid g, ParcelsDev p)
 {
     const int64_t i = gtid();
     if (i >= p.n) return;
     p.x[i] = p.x[i] - g.extent[0] * trunc((p.x[i] - g.center[0]) * g.hli[0]);
     p.y[i] = p.y[i] - g.extent[1] * trunc((p.y[i] - g.center[1]) * g.hli[1]);
-    orig[i] = (int)i;
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
@@ -521,12 +569,16 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
         if (e_ != hipSuccess) return done(-1);            \
     } while (0)
 
-    int *flag, *rank, *bsum;
+    // small parcels in caller-index order: flags by slot, moved to caller
+    // indices, ranked there
+    int *flag, *cflag, *rank, *bsum;
     MOK(alloc((void **)&flag, sizeof(int) * n));
+    MOK(alloc((void **)&cflag, sizeof(int) * n));
     MOK(alloc((void **)&rank, sizeof(int) * (n + 1)));
     MOK(alloc((void **)&bsum, sizeof(int) * scan_scratch_ints(std::max<int64_t>(8 * n, c.ncell) + 1)));
     hipLaunchKernelGGL(k_small, dim3(nblk(n)), dim3(256), 0, s, g, p, c.vmin, flag);
-    MOK(scan(flag, n, rank, bsum));
+    hipLaunchKernelGGL(k_to_caller, dim3(nblk(n)), dim3(256), 0, s, n, p.orig, flag, cflag);
+    MOK(scan(cflag, n, rank, bsum));
     int ns = 0;
     MOK(host_int(rank + n, ns));
     if (ns == 0) return done(0);
@@ -535,7 +587,7 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     MOK(alloc((void **)&imgflag, sizeof(int) * 8 * (int64_t)ns));
     MOK(alloc((void **)&imgrank, sizeof(int) * (8 * (int64_t)ns + 1)));
     MOK(alloc((void **)&img_of, sizeof(int) * 8 * (int64_t)ns));
-    hipLaunchKernelGGL(k_compact, dim3(nblk(n)), dim3(256), 0, s, n, flag, rank, sml);
+    hipLaunchKernelGGL(k_compact_caller, dim3(nblk(n)), dim3(256), 0, s, n, flag, p.orig, rank, sml);
     hipLaunchKernelGGL(k_image_flags, dim3(nblk(8 * (int64_t)ns)), dim3(256), 0, s, g, p, sml, ns, imgflag);
     MOK(scan(imgflag, 8 * (int64_t)ns, imgrank, bsum));
     int nimg = 0;
@@ -572,7 +624,8 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     double *dclo;
     MOK(alloc((void **)&iclo, sizeof(int) * nsm));
     MOK(alloc((void **)&dclo, sizeof(double) * nsm));
-    hipLaunchKernelGGL(k_search, dim3(nblk(nsm)), dim3(256), 0, s, g, P, sml, ns, nimg, start, node, ipid, iclo, dclo);
+    hipLaunchKernelGGL(k_search, dim3(nblk(nsm)), dim3(256), 0, s, g, P, p.orig, sml, ns, nimg, start, node, ipid, iclo,
+                       dclo);
     hipLaunchKernelGGL(k_reconcile, dim3(nblk(ns)), dim3(256), 0, s, ns, img_of, iclo, dclo, c.err);
     int e = 0;
     MOK(host_int(c.err, e));
@@ -636,20 +689,33 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     hipLaunchKernelGGL(k_group_merge, dim3(nblk(n)), dim3(256), 0, s, g, p, gstart, gent, pis, c.err);
     MOK(host_int(c.err, e));
     if (e) return done(1);
-    // delete (holes filled from the end) and the periodic wrap of every parcel
-    int *hole, *tflag, *trank;
-    MOK(alloc((void **)&hole, sizeof(int) * n));
+    // delete (holes filled from the end): caller indices relabelled, then
+    // the slots compacted; and the periodic wrap of every parcel
+    int *hole_c, *hole_s, *tflag, *trank, *tail_slot, *hrank, *hole_list;
+    MOK(alloc((void **)&hole_c, sizeof(int) * n));
+    MOK(alloc((void **)&hole_s, sizeof(int) * n));
     MOK(alloc((void **)&tflag, sizeof(int) * nm));
     MOK(alloc((void **)&trank, sizeof(int) * (nm + 1)));
-    MOK(hipMemsetAsync(hole, 0, sizeof(int) * n, s));
-    hipLaunchKernelGGL(k_mark_holes, dim3(nblk(nm)), dim3(256), 0, s, nm, pis, hole);
-    hipLaunchKernelGGL(k_tail_flags, dim3(nblk(nm)), dim3(256), 0, s, n, nm, hole, tflag);
+    MOK(alloc((void **)&tail_slot, sizeof(int) * nm));
+    MOK(alloc((void **)&hrank, sizeof(int) * (n - nm + 1)));
+    MOK(alloc((void **)&hole_list, sizeof(int) * nm));
+    MOK(hipMemsetAsync(hole_c, 0, sizeof(int) * n, s));
+    MOK(hipMemsetAsync(hole_s, 0, sizeof(int) * n, s));
+    hipLaunchKernelGGL(k_mark_holes, dim3(nblk(nm)), dim3(256), 0, s, nm, pis, p.orig, hole_c, hole_s);
+    hipLaunchKernelGGL(k_tail_flags, dim3(nblk(nm)), dim3(256), 0, s, n, nm, hole_c, tflag);
     MOK(scan(tflag, nm, trank, bsum));
-    hipLaunchKernelGGL(k_fill_holes, dim3(nblk(nm)), dim3(256), 0, s, n, nm, pis, tflag, trank, p, c.attr, c.cap,
+    hipLaunchKernelGGL(k_tail_slots, dim3(nblk(n)), dim3(256), 0, s, n, nm, p.orig, tail_slot);
+    hipLaunchKernelGGL(k_relabel, dim3(nblk(nm)), dim3(256), 0, s, n, nm, pis, tflag, trank, tail_slot, p.orig);
+    // slots: merged slots below n - nm in ascending order, then the fill
+    MOK(scan(hole_s, n - nm, hrank, bsum));
+    hipLaunchKernelGGL(k_compact, dim3(nblk(n - nm)), dim3(256), 0, s, n - nm, hole_s, hrank, hole_list);
+    hipLaunchKernelGGL(k_tail_flags, dim3(nblk(nm)), dim3(256), 0, s, n, nm, hole_s, tflag);
+    MOK(scan(tflag, nm, trank, bsum));
+    hipLaunchKernelGGL(k_fill_holes, dim3(nblk(nm)), dim3(256), 0, s, n, nm, hole_list, tflag, trank, p, c.attr, c.cap,
                        c.nattr);
     ParcelsDev q = p;
     q.n = n - nm;
-    hipLaunchKernelGGL(k_wrap_all, dim3(nblk(q.n)), dim3(256), 0, s, g, q, p.orig);
+    hipLaunchKernelGGL(k_wrap_all, dim3(nblk(q.n)), dim3(256), 0, s, g, q);
     MOK(hipGetLastError());
     *n_merge = nm;
 #undef MOK

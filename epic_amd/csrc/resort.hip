@@ -50,15 +50,15 @@ __global__ __launch_bounds__(256) void k_resort(const ResortArgs *__restrict__ t
     }
 }
 
-// inverse of the caller index: inv[orig[s]] = s
-__global__ __launch_bounds__(256) void k_invert(const InvertArgs *__restrict__ tab, const int2 *__restrict__ bmap)
+// a per-slot array in caller-index order: out[orig[s]] = in[s]
+__global__ __launch_bounds__(256) void k_to_caller(const ToCallerArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
-    const InvertArgs &a = tab[ib.item];
+    const ToCallerArgs &a = tab[ib.item];
     const int64_t s = (int64_t)ib.blk * 256 + threadIdx.x;
     if (s >= a.n) return;
     const int o = gl(a.orig)[s];
-    if (o >= 0 && o < a.n) gl(a.inv)[o] = (int)s;
+    if (o >= 0 && o < a.n) gl(a.out)[o] = gl(a.in)[s];
 }
 
 void add_resort(Plan &P, int phase, const std::vector<ResortArgs> &items, bool with_doubles)
@@ -71,11 +71,11 @@ void add_resort(Plan &P, int phase, const std::vector<ResortArgs> &items, bool w
         P.add((const void *)k_resort<0>, 256, 0, phase, items, nb);
 }
 
-void add_invert(Plan &P, int phase, const std::vector<InvertArgs> &items)
+void add_to_caller(Plan &P, int phase, const std::vector<ToCallerArgs> &items)
 {
     std::vector<int> nb;
-    for (const InvertArgs &a : items) nb.push_back((int)std::max<int64_t>(1, (a.n + 255) / 256));
-    P.add((const void *)k_invert, 256, 0, phase, items, nb);
+    for (const ToCallerArgs &a : items) nb.push_back((int)std::max<int64_t>(1, (a.n + 255) / 256));
+    P.add((const void *)k_to_caller, 256, 0, phase, items, nb);
 }
 
 }  // namespace epic

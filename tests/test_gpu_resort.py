@@ -89,6 +89,21 @@ def test_merge_and_split_after_resorts_follow_caller_order():
     ns_a, ns_b = a.parcel_split(lambda_max=2.5), b.parcel_split(lambda_max=2.5)
     assert ns_a == ns_b > 0
     assert np.array_equal(_state(a), _state(b))
+    for x, y in zip(a.get_tendencies(), b.get_tendencies()):
+        assert np.array_equal(x, y)
+    # merge and split leave the slots out of caller order (relabelled caller
+    # indices, children at the end): further re-sorted steps and parcel
+    # operations still give the same caller-ordered state
+    for cycle in range(2):
+        for _ in range(2):
+            t_a = a.ls_rk_step(t_a, 0.01)
+            t_b = b.ls_rk_step(t_b, 0.01)
+        assert np.array_equal(_state(a), _state(b)), cycle
+        assert a.parcel_merge() == b.parcel_merge()
+        for x, y in zip(a.get_tendencies(), b.get_tendencies()):
+            assert np.array_equal(x, y), cycle
+        assert a.parcel_split(lambda_max=2.5) == b.parcel_split(lambda_max=2.5)
+        assert np.array_equal(_state(a), _state(b)), cycle
 
 
 def test_tendencies_follow_their_parcels_across_resort():
