@@ -290,10 +290,11 @@ def bench_served(n_requests, rates, req_steps, seed0=DEFAULT_SEED + 1):
                       "parcel_steps_per_s_busy": st["parcel_steps_per_s_busy"],
                       "parcel_steps_per_s_wall": st["parcel_steps_per_s_wall"],
                       "wall_s": st["wall_s"], "mean_members_per_tick": st["mean_members_per_tick"],
+                      "parcel_ops_ms_p50": st["ops_ms_p50"], "parcel_ops_ms_mean": st["ops_ms_mean"],
                       "queue_wait_s_p50": st["queue_wait_s_p50"], "failed": failed, "failures": failures[:4]})
         log(f"bench: served at {rate}/s: {st['parcel_steps_per_s_busy']:.3e} parcel-steps/s busy, "
             f"{st['parcel_steps_per_s_wall']:.3e} wall, p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms, "
-            f"{st['mean_members_per_tick']:.1f} members per tick")
+            f"{st['mean_members_per_tick']:.1f} members per tick, parcel operations {st['ops_ms_mean']:.1f} ms per tick")
     top = max(sweep, key=lambda s: s["rate_per_s"])
     return {
         "workload": f"{n_requests} requests, grids from {{32,64,96,128}}^2 x {{32,64}}, 8 parcels/cell, "

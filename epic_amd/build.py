@@ -23,6 +23,11 @@ HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # no packed-FP32 instructions in the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fused within a source expression only: with fusion across
+    # statements the choice depends on the surrounding code, so unrolled
+    # copies of one loop body (or one function inlined in two kernels) could
+    # round differently and a parcel's FP64 result would depend on its slot
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 

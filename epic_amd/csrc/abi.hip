@@ -83,6 +83,11 @@ struct epic_member {
     std::string msg;
     // launch plans of the call surface (built on first use)
     Plan pl_keys, pl_p2g_fresh, pl_p2g_reuse, pl_vor2vel, pl_vtend, pl_g2p[2], pl_step, pl_step_adaptive;
+    // the cached half axes (ax_stale) after a change of the parcels that did
+    // not recompute them (set_parcels, merge, split): recomputed from the
+    // current B before anything reads them (ensure_axes)
+    bool axes_dirty = true;
+    Plan pl_axes;
     Plan pl_lap[2];   // parcel correction: forward / inverse transforms
     // parcel correction: the target volume-weighted vorticity and the FP64
     // reduction scratch (allocated on first use)
@@ -118,6 +123,7 @@ struct epic_member {
     {
         ++gen;
         pl_keys.release();
+        pl_axes.release();
         pl_p2g_fresh.release();
         pl_p2g_reuse.release();
         pl_vor2vel.release();
@@ -134,14 +140,19 @@ struct epic_member {
         invalidate_plans();
         pl_step.release();
         pl_step_adaptive.release();
+        pl_axes.release();
     }
     // the parcel count changed (set_parcels, split, merge): the step plans
-    // read the count from n_dev and stay valid; the plans of the single
-    // calls hold it in their tables and are rebuilt on their next use
+    // and the keys and axes plans read the count from n_dev and stay valid;
+    // the other plans of the single calls hold it in their tables and are
+    // rebuilt on their next use
     void counts_changed()
     {
         const uint64_t keep = gen;
+        Plan k;
+        k.swap(pl_keys);
         invalidate_plans();
+        pl_keys.swap(k);
         gen = keep;
     }
 };
@@ -764,6 +775,17 @@ int ensure_keys(epic_member *m)
     return rc;
 }
 
+// the cached half axes from the current B, when stale (enqueued on the
+// member stream)
+int ensure_axes(epic_member *m)
+{
+    if (!m->axes_dirty) return EPIC_OK;
+    if (m->pl_axes.empty()) build_axes(m->pl_axes, {m}, {m->ax_stale});
+    const int rc = run_plan(m, m->pl_axes);
+    if (rc == EPIC_OK) m->axes_dirty = false;
+    return rc;
+}
+
 int field_ncomp(int which)
 {
     switch (which) {
@@ -1007,6 +1029,7 @@ int epic_set_parcels(epic_handle m, int64_t n, const double *position, const dou
     if (n) HIPOK(m, hipMemcpyAsync(m->orig, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice, m->s));
     HIPOK(m, hipStreamSynchronize(m->s));
     m->keys_valid = false;
+    m->axes_dirty = true;
     m->tend_displaced = false;
     return EPIC_OK;
 }
@@ -1153,8 +1176,13 @@ int epic_par2grid(epic_handle m, int l_reuse)
         if (l_reuse != 1) build_axes(P, {m}, {m->ax_stale});
         build_p2g(P, {m}, {P2GSel{m->ax_stale, nullptr, nullptr, 0, -1}});
     }
+    if (l_reuse == 1) {
+        rc = ensure_axes(m);
+        if (rc) return rc;
+    }
     rc = run_plan(m, P);
     if (rc) return rc;
+    if (l_reuse != 1) m->axes_dirty = false;
     return check(m, "epic_par2grid");
 }
 
@@ -1306,6 +1334,7 @@ static int resort(epic_member *m)
 static int run_step(epic_member *m, double *t, double t_initial, bool adaptive, double *dt_taken, const char *where)
 {
     int rc = ensure_keys(m);
+    if (rc == EPIC_OK) rc = ensure_axes(m);
     if (rc) return rc;
     Plan &P = adaptive ? m->pl_step_adaptive : m->pl_step;
     uint64_t &pg = m->step_gen[adaptive ? 1 : 0];
@@ -1659,12 +1688,18 @@ const char *epic_batch_last_error(epic_batch_handle b) { return b ? b->msg.c_str
 static int batch_prepare(epic_batch *b)
 {
     const int nm = (int)b->ms.size();
-    for (int i = 0; i < nm; ++i) {
+    for (int i = 0; i < nm; ++i) {   // keys and cached axes of changed members, on their streams
         epic_member *m = b->ms[i];
-        if (!m->keys_valid) {
-            const int rc = ensure_keys(m);
+        if (!m->keys_valid || m->axes_dirty) {
+            int rc = ensure_keys(m);
+            if (rc == EPIC_OK) rc = ensure_axes(m);
             if (rc) return batch_fail(b, rc, m->msg);
-            BHIPOK(b, hipStreamSynchronize(m->s));
+            hipEvent_t ev = nullptr;
+            BHIPOK(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            hipError_t e = hipEventRecord(ev, m->s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->s, ev, 0);
+            (void)hipEventDestroy(ev);
+            BHIPOK(b, e);
         }
     }
     bool same = b->gens.size() == (size_t)nm;
@@ -1808,19 +1843,12 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
         HIPOK(m, bind_parcels(m));
         m->counts_changed();
         // the cached support-point axes of every parcel from its current B
-        // (what the next correction's vol2grid stores, parcel_interpl.f90:92-106)
-        Plan Q;
-        build_axes(Q, {m}, {m->ax_stale});
-        HIPOK(m, Q.run(m->s));
+        // (what the next correction's vol2grid stores, parcel_interpl.f90:92-106),
+        // computed before they are next read
+        m->axes_dirty = true;
         rc = check(m, "epic_parcel_split");
         if (rc) return rc;
     }
-    // the cell keys and histogram of the new parcel set now, in the same
-    // wait, rather than before the next step
-    rc = ensure_keys(m);
-    if (rc) return rc;
-    rc = check(m, "epic_parcel_split");
-    if (rc) return rc;
     if (n_split) *n_split = total;
     m->n_splits += total;
     return EPIC_OK;
@@ -1854,14 +1882,8 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
         m->n -= nm;
         HIPOK(m, bind_parcels(m));
         m->counts_changed();
-        m->keys_valid = false;
-        Plan Q;
-        build_axes(Q, {m}, {m->ax_stale});
-        HIPOK(m, Q.run(m->s));
-        rc = ensure_keys(m);   // keys and histogram of the new parcel set
-        if (rc) return rc;
-        rc = check(m, "epic_parcel_merge");
-        if (rc) return rc;
+        m->keys_valid = false;   // keys, histogram and cached axes of the new
+        m->axes_dirty = true;    // parcel set: computed before they are next read
     }
     if (n_merge) *n_merge = nm;
     m->n_merges += nm;

@@ -302,17 +302,49 @@ __global__ void k_tree6(const int *sml, int ns, const unsigned char *merged, con
     dom[m] = (leaf[is] != 0) ? 1 : 0;   // merged & leaf, or unmerged leaf
     cand[m] = (!merged[is] && !leaf[is] && !avail[is]) ? 1 : 0;
 }
-__global__ void k_tree7(const int *sml, const int *clist, int ncand, const int *iclo, unsigned char *avail, int *dom,
-                        int *iso)
-{   // one thread: the reference's loop over the candidates in small-parcel order
-    if (gtid() != 0) return;
-    for (int u = 0; u < ncand; ++u) {
-        const int m = clist[u];
-        if (avail[iclo[m]]) dom[m] = 1;
-        else {
-            iso[m] = 1;
-            avail[sml[m]] = 1;
+// The reference's sequential loop over the candidates in small-parcel order
+// (parcel_nearest.f90, second stage of resolve_tree): candidate m merges if
+// its target iclo[m] is available, else it is an isolated dual link and
+// makes its own parcel available.  The only availability the loop changes is
+// a candidate's own parcel, so m's decision depends on the candidate p that
+// owns iclo[m], and only when p comes first: then iclo[m] is available
+// exactly when p was isolated.  Each candidate therefore follows that chain
+// (whose positions strictly decrease) to its first link to a parcel that is
+// not an earlier candidate, and flips the answer there once per step: the
+// decisions of the sequential loop, in parallel.  cpos[slot] is the
+// candidate position of a candidate's own parcel, -1 for every other slot.
+__global__ void k_cand_pos(int ncand, const int *clist, const int *sml, int *cpos)
+{
+    const int64_t u = gtid();
+    if (u < ncand) cpos[sml[clist[u]]] = (int)u;
+}
+__global__ void k_tree7(const int *sml, const int *clist, int ncand, const int *iclo, const int *cpos,
+                        unsigned char *avail, int *dom, int *iso)
+{
+    const int64_t u0 = gtid();
+    if (u0 >= ncand) return;
+    int64_t u = u0;
+    bool flip = false, val;
+    for (;;) {
+        const int ic = iclo[clist[u]];
+        const int q = cpos[ic];
+        if (q < 0) {         // not a candidate's parcel: its availability is fixed during the loop
+            val = avail[ic] != 0;
+            break;
         }
+        if (q >= u) {        // a later candidate's parcel: not yet made available
+            val = false;
+            break;
+        }
+        flip = !flip;        // available exactly when that earlier candidate was isolated
+        u = q;
+    }
+    const bool merges = flip ? !val : val;
+    const int m = clist[u0];
+    if (merges) dom[m] = 1;
+    else {
+        iso[m] = 1;
+        avail[sml[m]] = 1;   // a candidate's own parcel: no other thread reads it above
     }
 }
 __global__ void k_tree8(int ns, const int *iclo, const unsigned char *avail, const int *iso, int *dom)
@@ -665,7 +697,14 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
         MOK(host_int(crank + ns, ncand));
         MOK(alloc((void **)&clist, sizeof(int) * std::max(ncand, 1)));
         hipLaunchKernelGGL(k_compact, dim3(nblk(ns)), dim3(256), 0, s, (int64_t)ns, cand, crank, clist);
-        hipLaunchKernelGGL(k_tree7, dim3(1), dim3(64), 0, s, sml, clist, ncand, iclo, avail, dom, iso);
+        if (ncand > 0) {
+            int *cpos;
+            MOK(alloc((void **)&cpos, sizeof(int) * n));
+            MOK(hipMemsetAsync(cpos, 0xff, sizeof(int) * n, s));
+            hipLaunchKernelGGL(k_cand_pos, dim3(nblk(ncand)), dim3(256), 0, s, ncand, clist, sml, cpos);
+            hipLaunchKernelGGL(k_tree7, dim3(nblk(ncand)), dim3(256), 0, s, sml, clist, ncand, iclo, cpos, avail, dom,
+                               iso);
+        }
     }
     hipLaunchKernelGGL(k_tree8, dim3(nblk(ns)), dim3(256), 0, s, ns, iclo, avail, iso, dom);
     // pairs in small-parcel order, grouped by target

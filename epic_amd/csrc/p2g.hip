@@ -815,9 +815,7 @@ __global__ __launch_bounds__(256) void k_axes(const AxesArgs *__restrict__ tab, 
     const ParcelsDev &p = a.p;
     const int64_t n = parcel_count(p);
     int e = 0;
-    // one copy of the loop body, so that a parcel's axes do not depend on its
-    // slot (unrolled copies may contract the FP64 arithmetic differently)
-#pragma unroll 1
+#pragma unroll
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
         if (q >= n) break;

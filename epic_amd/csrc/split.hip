@@ -33,10 +33,7 @@ __global__ __launch_bounds__(256) void k_split_flag(const SplitArgs *__restrict_
     const ItemBlock ib = item_block(bmap);
     const SplitArgs &a = tab[ib.item];
     int e = 0;
-    // one copy of the loop body: an unrolled copy per round could contract
-    // its FP64 arithmetic differently, and a parcel's result would then
-    // depend on its slot
-#pragma unroll 1
+#pragma unroll
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
         if (q >= a.p.n) break;
@@ -61,10 +58,7 @@ __global__ __launch_bounds__(256) void k_split_apply(const SplitArgs *__restrict
     const Grid &g = a.g;
     const ParcelsDev &p = a.p;
     const double dh = 0.5 * sqrt(3.0 / 5.0);
-    // one copy of the loop body: an unrolled copy per round could contract
-    // its FP64 arithmetic differently, and a parcel's result would then
-    // depend on its slot
-#pragma unroll 1
+#pragma unroll
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
         if (q >= p.n) break;

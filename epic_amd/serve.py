@@ -17,11 +17,13 @@ concurrently on the GPU, and while they run the server
 
 Then it waits for the tick (``Batch.wait``).  With ``parcel_ops_every = K``
 every member runs the parcel operations of EPIC's run loop after each K-th
-of its steps, merge then split (epic3d.f90:143-148), on its own stream;
-members are allocated with room for the children of the splits, and a
-cohort whose members changed their parcel count is captured again at its
-next launch.  Long runs need them: without splitting, ellipsoids stretch
-until their support points leave the neighbouring cells.  A member that
+of its steps, merge then split (epic3d.f90:143-148), on its own stream and
+its own host thread, concurrently with the other members'; members are
+allocated with room for the children of the splits.  The step graphs read
+the parcel counts from device memory, so a cohort keeps its captured graph
+when its members' counts change.  Long runs need the operations: without
+splitting, ellipsoids stretch until their support points leave the z halo
+layers.  A member that
 has taken its requested steps, or whose step or parcel operation failed
 (its status is returned in the result), leaves its cohort; only that
 cohort's graph is rebuilt.  Device
@@ -38,6 +40,7 @@ from __future__ import annotations
 import dataclasses
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -79,6 +82,17 @@ class _Entry:
     message: str = ""
 
 
+def _parcel_ops(e):
+    """EPIC's run-loop parcel operations after a step (epic3d.f90:143-148):
+    merge, then split; returns the EpicError of a failure, or None."""
+    try:
+        e.member.parcel_merge()
+        e.member.parcel_split()
+    except EpicError as err:
+        return err
+    return None
+
+
 class _Cohort:
     def __init__(self, entries):
         self.entries = list(entries)
@@ -91,7 +105,8 @@ class _Cohort:
 
 class Server:
     def __init__(self, hbm_budget_bytes: int, max_members: int = 64, autotune: bool = False,
-                 parcel_ops_every: int = 0, capacity_factor: float = 2.0):
+                 parcel_ops_every: int = 0, capacity_factor: float = 2.0, ops_threads: int = 8):
+        self.ops_threads = max(1, int(ops_threads))
         self.budget = int(hbm_budget_bytes)
         self.max_members = int(max_members)
         self.autotune = autotune
@@ -119,7 +134,8 @@ class Server:
         to_free = []           # members released after the next wait
         used = 0
         results = {}
-        tick_ms, tick_parcels, tick_members = [], [], []
+        tick_ms, tick_parcels, tick_members, ops_ms = [], [], [], []
+        pool = ThreadPoolExecutor(max_workers=self.ops_threads)
         t0 = clock()
 
         def n_active():
@@ -184,9 +200,9 @@ class Server:
             stepped = 0
             nmem = 0
             retired_any = []
+            ops = []
             for c in cohorts:
                 ts, status = c.batch.wait()
-                keep = []
                 for e, t, st in zip(c.entries, ts, status):
                     nmem += 1
                     if st == 0:
@@ -195,17 +211,23 @@ class Server:
                         stepped += e.member.n
                         if (self.parcel_ops_every and e.steps % self.parcel_ops_every == 0
                                 and e.steps < e.req.steps):
-                            try:   # epic3d.f90:143-148: merge, then split
-                                e.member.parcel_merge()
-                                e.member.parcel_split()
-                            except EpicError as err:
-                                st = err.code
-                                e.status = int(err.code)
-                                e.message = str(err)
+                            ops.append(e)
                     else:
                         e.status = int(st)
                         e.message = e.member.last_error()
-                    if st != 0 or e.steps >= e.req.steps:
+            # the members' parcel operations run concurrently, one host
+            # thread per member (each on its own stream; the library takes
+            # one host thread per handle)
+            o0 = clock()
+            for e, err in zip(ops, pool.map(_parcel_ops, ops) if len(ops) > 1 else map(_parcel_ops, ops)):
+                if err is not None:
+                    e.status = int(err.code)
+                    e.message = str(err)
+            ops_ms.append((clock() - o0) * 1e3)
+            for c in cohorts:
+                keep = []
+                for e in c.entries:
+                    if e.status != 0 or e.steps >= e.req.steps:
                         to_collect.append(e)   # its memory counts until released
                     else:
                         keep.append(e)
@@ -224,6 +246,7 @@ class Server:
             if cohorts_new is not None:
                 cohorts.append(cohorts_new)
         wall = clock() - t0
+        pool.shutdown()
         for c in cohorts:
             c.close()
         lat = np.array(tick_ms) if tick_ms else np.zeros(1)
@@ -234,6 +257,10 @@ class Server:
             "ticks": len(tick_ms),
             "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)),
+            # the parcel operations' share of a tick (merge + split of the
+            # members that stepped, on concurrent host threads)
+            "ops_ms_p50": float(np.percentile(ops_ms, 50)) if ops_ms else 0.0,
+            "ops_ms_mean": float(np.mean(ops_ms)) if ops_ms else 0.0,
             "parcel_steps": parcel_steps,
             "wall_s": wall,
             "parcel_steps_per_s_wall": parcel_steps / wall if wall > 0 else 0.0,

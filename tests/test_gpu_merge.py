@@ -27,7 +27,7 @@ def _with_small(mem, frac, seed, s=0.03):
     return dataclasses.replace(mem, volume=f32(vol), B=f32(B)), int(idx.sum())
 
 
-@pytest.mark.parametrize("frac", [0.02, 0.1, 0.3])
+@pytest.mark.parametrize("frac", [0.02, 0.1, 0.3, 0.5])
 def test_merge_matches_oracle(frac):
     mem, nsmall = _with_small(W.rising_thermal(16, seed=71, humidity_plume=True, n_blobs=2), frac, 72)
     print(f"merge: thermal16, {nsmall} small parcels", flush=True)
