@@ -127,6 +127,7 @@ def load_library(path=None):
         "epic_member_bytes": ([P(Grid), i64], i64),
         "epic_parcel_split": ([h, d, d, P(i64)], i),
         "epic_parcel_merge": ([h, P(i64)], i),
+        "epic_parcel_merge_stats": ([h, P(i64)], i),
         "epic_apply_laplace": ([h, i], i),
         "epic_apply_gradient": ([h, d, d, i], i),
         "epic_init_parcel_correction": ([h, dp], i),
@@ -346,6 +347,14 @@ class Member:
         self._chk(load_library().epic_parcel_split(self._h, lambda_max, amax, C.byref(ns)))
         self.n = int(load_library().epic_num_parcels(self._h))
         return ns.value
+
+    def parcel_merge_stats(self):
+        """n_way_parcel_mergers (parcel_merge.f90:381-391) since the last
+        diagnostics reset: entry j-1 counts the targets that took j small
+        parcels (the last entry: seven or more)."""
+        out = np.zeros(7, dtype=np.int64)
+        self._chk(load_library().epic_parcel_merge_stats(self._h, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
 
     def parcel_merge(self):
         """parcel_merge (parcel_merge.f90) on one rank; returns the number of

@@ -94,8 +94,10 @@ struct epic_member {
     double vor_bar[3] = {0.0, 0.0, 0.0};
     double *vw_d = nullptr;
     double *vw_h = nullptr;
-    // diagnostics: split / merge counts since the last reset, FP64 scratch
+    // diagnostics: split / merge counts since the last reset, FP64 scratch;
+    // collect_merge_stats' counts: [0] n_big_close, [1..7] n-way mergers
     int64_t n_splits = 0, n_merges = 0;
+    int64_t merge_stats[8] = {};
     double *diag_d = nullptr, *diag_h = nullptr;
     // physical re-sort into cell order every resort_every steps (0: never)
     int resort_every = 3, steps_since_sort = 0;
@@ -1873,6 +1875,7 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
     c.cap = m->cap;
     c.nattr = NATTR;
     c.scratch = &m->merge_scratch;
+    c.stats = m->merge_stats;
     int64_t nm = 0;
     const int r = run_merge(c, &nm);
     if (r < 0) return fail(m, EPIC_E_HIP, "epic_parcel_merge: HIP failure");
@@ -1887,6 +1890,13 @@ int epic_parcel_merge(epic_handle m, int64_t *n_merge)
     }
     if (n_merge) *n_merge = nm;
     m->n_merges += nm;
+    return EPIC_OK;
+}
+
+int epic_parcel_merge_stats(epic_handle m, int64_t *n_way)
+{
+    if (!m || !n_way) return EPIC_E_INVALID;
+    for (int j = 0; j < 7; ++j) n_way[j] = m->merge_stats[1 + j];
     return EPIC_OK;
 }
 
@@ -1994,8 +2004,12 @@ static void parcel_counts_of(epic_member *m, double *stats, int reset_counters)
 {
     stats[11] = (double)m->n;
     stats[13] = (double)m->n_splits;
+    stats[14] = (double)m->merge_stats[0];   // n_big_close
     stats[15] = (double)m->n_merges;
-    if (reset_counters) m->n_splits = m->n_merges = 0;   // as after a write (parcel_diagnostics_netcdf.f90:181-183)
+    if (reset_counters) {   // as after a write (parcel_diagnostics_netcdf.f90:181-184)
+        m->n_splits = m->n_merges = 0;
+        for (int j = 0; j < 8; ++j) m->merge_stats[j] = 0;
+    }
 }
 
 // run a diagnostics reduction and copy its nv totals to the host

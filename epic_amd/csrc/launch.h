@@ -215,6 +215,7 @@ struct MergeCtx {
     int64_t cap;
     int nattr;
     ScratchArena *scratch;   // the member's merge scratch
+    int64_t *stats;          // host: += collect_merge_stats' counts ([0] n_big_close, [1..7] n-way mergers)
 };
 // 0: done (*n_merge merged), 1: stopped on a device error bit, -1: HIP failure
 int run_merge(const MergeCtx &c, int64_t *n_merge);

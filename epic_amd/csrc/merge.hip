@@ -373,6 +373,22 @@ __global__ void k_group_place(int nm, const int *pic, const int *gstart, int *gc
     gent[gstart[ic] + atomicAdd(&gcur[ic], 1)] = (int)k;
 }
 
+// collect_merge_stats (parcel_merge.f90:365-393), before the volumes change:
+// cnt[0] counts the pairs whose target is big, cnt[j] the targets that
+// min(7, their number of pairs) = j
+__global__ void k_merge_stats_big(int nm, const int *pic, ParcelsDev p, double vmin, int *cnt)
+{
+    const int64_t k = gtid();
+    if (k < nm && (double)p.vol[pic[k]] > vmin) atomicAdd(&cnt[0], 1);
+}
+__global__ void k_merge_stats_way(int64_t n, const int *gcnt, int *cnt)
+{
+    const int64_t i = gtid();
+    if (i >= n) return;
+    const int c = gcnt[i];
+    if (c > 0) atomicAdd(&cnt[c < 7 ? c : 7], 1);
+}
+
 __device__ inline double pdelta(double d, double L) { return d - L * round(d * (1.0 / L)); }
 
 // one thread per target parcel with mergers: do_group_merge + geometric_merge
@@ -725,6 +741,13 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     hipLaunchKernelGGL(k_pairs, dim3(nblk(ns)), dim3(256), 0, s, ns, dom, prank, sml, iclo, pis, pic, gcnt);
     MOK(scan(gcnt, n, gstart, bsum));
     hipLaunchKernelGGL(k_group_place, dim3(nblk(nm)), dim3(256), 0, s, nm, pic, gstart, gcur, gent);
+    int *mstat;
+    MOK(alloc((void **)&mstat, sizeof(int) * 8));
+    MOK(hipMemsetAsync(mstat, 0, sizeof(int) * 8, s));
+    hipLaunchKernelGGL(k_merge_stats_big, dim3(nblk(nm)), dim3(256), 0, s, nm, pic, p, c.vmin, mstat);
+    hipLaunchKernelGGL(k_merge_stats_way, dim3(nblk(n)), dim3(256), 0, s, n, gcnt, mstat);
+    int mstat_h[8] = {};
+    MOK(hipMemcpyAsync(mstat_h, mstat, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_group_merge, dim3(nblk(n)), dim3(256), 0, s, g, p, gstart, gent, pis, c.err);
     MOK(host_int(c.err, e));
     if (e) return done(1);
@@ -758,7 +781,10 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     MOK(hipGetLastError());
     *n_merge = nm;
 #undef MOK
-    return done(0);
+    const int rc = done(0);   // synchronises: mstat_h holds the counts
+    if (c.stats)
+        for (int j = 0; j < 8; ++j) c.stats[j] += mstat_h[j];
+    return rc;
 }
 
 }  // namespace epic

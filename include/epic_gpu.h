@@ -202,9 +202,16 @@ int epic_apply_vortcor(epic_handle h);
  * min_buoy, max_buoy).  The velocity is delta_pos, so call it after
  * epic_grid2par, as ls_rk_step does (ls_rk.f90:108-110).  nsplits and
  * nmerges count epic_parcel_split / epic_parcel_merge since the last reset;
- * reset_counters != 0 clears them (parcel_diagnostics_netcdf.f90:181-183);
- * nbig_iclo is not tracked (0).  Sums are FP64 in a fixed order. */
+ * nbig_iclo is collect_merge_stats' n_big_close (parcel_merge.f90:365-393),
+ * the merged pairs whose target was big; reset_counters != 0 clears the
+ * three and the n-way counts (parcel_diagnostics_netcdf.f90:181-184).  Sums
+ * are FP64 in a fixed order. */
 int epic_parcel_diagnostics(epic_handle h, double *stats, int reset_counters);
+/* parcel_merge_stats (parcel_diagnostics.f90:132-133): n_way[j-1] = the
+ * merges since the last reset whose target took j small parcels (j = 7:
+ * seven or more), collect_merge_stats' n_way_parcel_mergers(7)
+ * (parcel_merge.f90:381-391). */
+int epic_parcel_merge_stats(epic_handle h, int64_t *n_way);
 /* No counterpart in the reference: epic_parcel_diagnostics for callers that
  * cannot allocate device or pinned memory (code under HIP graph capture,
  * serving loops, harnesses that forbid hipMalloc).  Same stats[18] and

@@ -1860,6 +1860,15 @@ static double fnint(double v) { return (double)lround(v); }   /* Fortran nint: h
 
 int orc_parcel_merge(const orc_dom *d, orc_parcels *p, int64_t *n_merge_out)
 {
+    return orc_parcel_merge_stats(d, p, n_merge_out, NULL);
+}
+
+/* parcel_merge with collect_merge_stats (parcel_merge.f90:365-393): stats[0]
+ * += the pairs whose target is big (volume > vmin, before the merge),
+ * stats[j] += the targets with min(7, pairs into it) = j, j = 1..7
+ * (n_big_close and n_way_parcel_mergers) */
+int orc_parcel_merge_stats(const orc_dom *d, orc_parcels *p, int64_t *n_merge_out, int64_t *stats)
+{
     const int nx = d->nx, ny = d->ny, nz = d->nz;
     const int hx = nx + 3, hy = ny + 3;
     const double vmin = d->vmin > zero ? d->vmin : d->vcell / 20.0;
@@ -2064,6 +2073,24 @@ int orc_parcel_merge(const orc_dom *d, orc_parcels *p, int64_t *n_merge_out)
                 ic_m[nm] = iclo[m];
                 ++nm;
             }
+    }
+    /* collect_merge_stats (parcel_merge.f90:365-393), before the volumes change */
+    if (rc == ORC_OK && nm > 0 && stats) {
+        int64_t *loca = (int64_t *)calloc((size_t)n, sizeof(int64_t));
+        for (int64_t m = 0; m < nm; ++m) {
+            const int64_t ic = ic_m[m];
+            stats[0] += (p->volume[ic] > vmin) ? 1 : 0;
+            loca[ic] += 1;
+        }
+        for (int64_t m = 0; m < nm; ++m) {
+            const int64_t ic = ic_m[m];
+            const int64_t cnt = loca[ic];
+            if (cnt > 0) {
+                loca[ic] = -1;
+                stats[cnt < 7 ? cnt : 7] += 1;
+            }
+        }
+        free(loca);
     }
     /* geometric_merge / do_group_merge (parcel_merge.f90:129-317) */
     if (rc == ORC_OK && nm > 0) {

@@ -152,6 +152,9 @@ int orc_parcel_split(const orc_dom *d, orc_parcels *p, int64_t cap, double lambd
 
 /* ---- parcel merge (parcel_merge.f90, parcel_nearest.f90), one rank ---- */
 int orc_parcel_merge(const orc_dom *d, orc_parcels *p, int64_t *n_merge);
+/* the same, adding collect_merge_stats' counts (parcel_merge.f90:365-393) to
+ * stats[8] (may be NULL): [0] n_big_close, [1..7] n_way_parcel_mergers */
+int orc_parcel_merge_stats(const orc_dom *d, orc_parcels *p, int64_t *n_merge, int64_t *stats);
 
 /* ---- parcel correction (parcel_correction.f90) ---- */
 int orc_apply_laplace(const orc_dom *d, orc_inv *v, orc_parcels *p, orc_fields *f, int l_reuse);

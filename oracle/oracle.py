@@ -137,6 +137,7 @@ def _declare(L):
     L.orc_default_amax.restype = d
     L.orc_parcel_split.argtypes = [P(Dom), P(Parcels), C.c_int64, d, d, P(C.c_int64)]
     L.orc_parcel_merge.argtypes = [P(Dom), P(Parcels), P(C.c_int64)]
+    L.orc_parcel_merge_stats.argtypes = [P(Dom), P(Parcels), P(C.c_int64), P(C.c_int64)]
     L.orc_apply_laplace.argtypes = [P(Dom), C.c_void_p, P(Parcels), P(Fields), C.c_int]
     L.orc_apply_gradient.argtypes = [P(Dom), P(Parcels), P(Fields), d, d, C.c_int]
     L.orc_volume_weighted_vorticity.argtypes = [P(Parcels), P(d)]
@@ -325,10 +326,15 @@ class State:
 
     def parcel_merge(self):
         """parcel_merge (parcel_merge.f90, parcel_nearest.f90) on one rank;
-        returns the number of merged small parcels."""
+        returns the number of merged small parcels.  self.merge_stats
+        accumulates collect_merge_stats' counts (parcel_merge.f90:365-393):
+        [n_big_close, n_way_parcel_mergers(1..7)]."""
         nm = C.c_int64(0)
         p = self._parcels()
-        rc = lib().orc_parcel_merge(C.byref(self.dom), C.byref(p), C.byref(nm))
+        if not hasattr(self, "merge_stats"):
+            self.merge_stats = np.zeros(8, dtype=np.int64)
+        rc = lib().orc_parcel_merge_stats(C.byref(self.dom), C.byref(p), C.byref(nm),
+                                          self.merge_stats.ctypes.data_as(C.POINTER(C.c_int64)))
         self.n = int(p.n)
         self._trim()
         _chk(rc)

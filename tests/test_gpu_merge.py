@@ -36,6 +36,10 @@ def test_merge_matches_oracle(frac):
     st = oracle_state(mem)
     nm_ref = st.parcel_merge()
     assert nm == nm_ref and gm.n == st.n
+    # collect_merge_stats: n_big_close (the diagnostics' nbig_iclo) and the
+    # n-way merger counts
+    assert gm.parcel_diagnostics()["nbig_iclo"] == st.merge_stats[0]
+    assert gm.parcel_merge_stats().tolist() == st.merge_stats[1:].tolist()
     p = gm.get_parcels()
     assert np.abs(p["position"] - st.position).max() <= 1e-12 * max(mem.extent)
     assert maxrel(p["volume"], st.volume) <= 1e-6

@@ -94,6 +94,8 @@ def test_two_body_merge_is_the_parallel_axis_second_moment():
     st = _state(lower, extent, pos, vol, B)
     big_B, big_x, big_v = B[ic].copy(), pos[ic].copy(), vol[ic]
     assert st.parcel_merge() == 1
+    # collect_merge_stats: one pair, its target big; one 2-way merger
+    assert st.merge_stats.tolist() == [1, 1, 0, 0, 0, 0, 0, 0]
     assert st.n == len(vol) - 1   # the small parcel was the last one
     V = big_v + v_small
     xc = (big_v * big_x + v_small * p_small) / V
@@ -140,6 +142,7 @@ def test_small_parcel_merges_with_brute_force_nearest():
     st = _state(lower, extent, pos2, vol2, B2)
     vols_before = vol2.copy()
     assert st.parcel_merge() == 12
+    assert st.merge_stats.tolist() == [12, 12, 0, 0, 0, 0, 0, 0]
     assert st.n == n0
     for s, w in zip(range(n0, n0 + 12), want):
         assert st.volume[w] == pytest.approx(vols_before[w] + v_small, rel=1e-14), (s, w)
@@ -158,6 +161,7 @@ def test_isolated_dual_link_merges_once_into_the_earlier_parcel():
     B2 = np.vstack([B, [a2s, 0, 0, a2s, 0], [a2s, 0, 0, a2s, 0]])
     st = _state(lower, extent, pos2, vol2, B2)
     assert st.parcel_merge() == 1
+    assert st.merge_stats.tolist() == [0, 1, 0, 0, 0, 0, 0, 0]   # the target is small
     assert st.n == n0 + 1
     # parcel n0 (the earlier one) survives with both volumes
     assert st.volume[n0] == pytest.approx(2 * v_small, rel=1e-15)
@@ -177,7 +181,36 @@ def test_delete_fills_holes_from_the_end():
         B2[k] = [a2s, 0, 0, a2s, 0]
     st = _state(lower, extent, pos2, vol2, B2)
     assert st.parcel_merge() == 2
+    assert st.merge_stats.tolist() == [2, 2, 0, 0, 0, 0, 0, 0]
     # holes 10, 20 get the last two parcels, the last one into the first hole
     assert np.array_equal(st.position[10], pos2[n0 - 1])
     assert np.array_equal(st.position[20], pos2[n0 - 2])
     assert st.n == n0 - 2
+
+
+def test_merge_stats_count_big_targets_and_n_way_mergers():
+    """collect_merge_stats (parcel_merge.f90:365-393): n_big_close counts the
+    pairs whose target is big; n_way_parcel_mergers(j) the targets that take
+    j small parcels (the last entry seven or more).  Three small parcels
+    round lattice parcel 300 (closer to it than to anything else) and one
+    next to parcel 100: one 4-way and one 2-way merger, four big targets;
+    the counts add up over calls."""
+    lower, extent, dx, pos, vol, B = _lattice(seed=7)
+    n0 = len(vol)
+    offs = np.array([[0.15, 0.0, 0.0], [-0.15, 0.0, 0.0], [0.0, 0.15, 0.0]]) * dx
+    extra = np.vstack([pos[300] + offs, pos[100] + np.array([0.0, 0.0, 0.15]) * dx])
+    v_small = vol[0] / 50
+    a2s = (0.75 * v_small / math.pi) ** (2 / 3)
+    pos2 = np.vstack([pos, extra])
+    vol2 = np.append(vol, np.full(4, v_small))
+    B2 = np.vstack([B, np.tile([a2s, 0, 0, a2s, 0], (4, 1))])
+    st = _state(lower, extent, pos2, vol2, B2)
+    assert st.parcel_merge() == 4
+    assert st.merge_stats.tolist() == [4, 1, 0, 1, 0, 0, 0, 0]
+    assert st.volume[300] == pytest.approx(vol[300] + 3 * v_small, rel=1e-14)
+    # a second call adds a 2-way merger
+    st2 = _state(lower, extent, np.vstack([pos, pos[50] + np.array([0.1, 0.1, 0.0]) * dx]),
+                 np.append(vol, v_small), np.vstack([B, [a2s, 0, 0, a2s, 0]]))
+    st2.merge_stats = st.merge_stats.copy()
+    assert st2.parcel_merge() == 1
+    assert st2.merge_stats.tolist() == [5, 2, 0, 1, 0, 0, 0, 0]
