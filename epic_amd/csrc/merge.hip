@@ -163,17 +163,16 @@ __device__ inline int64_t halo_cell(const Grid &g, double x, double y, double z)
 __global__ void k_bin_keys(Grid g, Pts P, int64_t ntot, int64_t ncell, int *key, int *cnt, int *err)
 {
     const int64_t e = gtid();
-    if (e >= ntot) return;
-    double x, y, z;
-    P.get(e, x, y, z);
-    const int64_t c = halo_cell(g, x, y, z);
-    if (c < 0 || c >= ncell) {
-        key[e] = -1;
-        atomicOr(err, ERR_OUT_OF_GRID);
-        return;
+    int k = -1;   // lanes past the end take part in the wave's histogram add with no key
+    if (e < ntot) {
+        double x, y, z;
+        P.get(e, x, y, z);
+        const int64_t c = halo_cell(g, x, y, z);
+        if (c < 0 || c >= ncell) atomicOr(err, ERR_OUT_OF_GRID);
+        else k = (int)c;
+        key[e] = k;
     }
-    key[e] = (int)c;
-    atomicAdd(&cnt[c], 1);
+    wave_run_add(k, cnt);   // one add per run of equal keys (parcels mostly in cell order)
 }
 
 // nearest other parcel; first minimum in the reference's visiting order

@@ -53,17 +53,22 @@ __global__ __launch_bounds__(256) void k_cell_keys(const KeysArgs *__restrict__ 
     const KeysArgs &a = tab[ib.item];
     const Grid &g = a.g;
     const int64_t n = parcel_count(a.p);
+    if ((int64_t)ib.blk * PER_BLOCK >= n) return;   // uniform over the block
     int e = 0;
+    // every lane runs every round (the histogram adds are wave-aggregated
+    // over runs of equal keys: consecutive parcels mostly share a cell)
 #pragma unroll
     for (int r = 0; r < PER_BLOCK / 256; ++r) {
         const int64_t q = (int64_t)ib.blk * PER_BLOCK + r * 256 + threadIdx.x;
-        if (q >= n) break;
-        const int key = cell_key(g, gl(a.p.x)[q], gl(a.p.y)[q], gl(a.p.z)[q], e);
-        gl(a.keys)[q] = key;
-        if (key >= 0) {
-            atomicAdd(&gl(a.count)[key], 1);
-            if (gl(a.p.vol)[q] <= g.vmin) atomicAdd(&gl(a.scount)[key], 1);
+        int key = -1;
+        bool small = false;
+        if (q < n) {
+            key = cell_key(g, gl(a.p.x)[q], gl(a.p.y)[q], gl(a.p.z)[q], e);
+            gl(a.keys)[q] = key;
+            small = key >= 0 && gl(a.p.vol)[q] <= g.vmin;
         }
+        wave_run_add(key, gl(a.count));
+        wave_run_add(small ? key : -1, gl(a.scount));
     }
     if (e) atomicOr(a.err, e);
 }
