@@ -17,10 +17,11 @@ around every phase inside the graph; --eager: the timed steps launch
 eagerly between HIP event pairs, as the counter runs under profiles/ do).
 
 batched: 64 members with grids drawn from {32,48,64,96}^2 x {32,64}, 8
-parcels per cell, each with its own seed, 20 steps of one batched step (one
-launch per kernel over all members, one captured HIP graph) against the same
-members stepped one after another with the plain step; each member's
-max-relative error after its first batched step against the FP64 oracle.
+parcels per cell, each with its own seed, tile z chunks from the autotuner
+(once per member shape), 20 steps of one batched step (one launch per
+kernel over all members, one captured HIP graph) against the same members
+stepped one after another with the plain step; each member's max-relative
+error after its first batched step against the FP64 oracle.
 
 served: requests with grids drawn from {32,64,96,128}^2 x {32,64}, 50-500
 steps each, Poisson arrivals at each rate of the sweep, admitted against a
@@ -141,8 +142,39 @@ def error_table(mem, snap, st):
         err[k] = maxrel(p[k], getattr(st, k))
     for f in FIELDS:
         err[f] = maxrel(snap[f][sl], getattr(st, f)[sl])
-    exact = all(np.array_equal(snap[f][sl], getattr(st, f)[sl]) for f in ("nparg", "nsparg"))
-    return {k: float(f"{v:.3e}") for k, v in err.items()}, exact
+    # integer count differences after the step (positions differ from the
+    # FP64 ones by ~1e-8 dx, so a parcel that close to a cell face may count
+    # in the neighbouring cell)
+    diff = int(max(np.abs(snap[f][sl].astype(np.int64) - getattr(st, f)[sl].astype(np.int64)).sum()
+                   for f in ("nparg",)))
+    return {k: float(f"{v:.3e}") for k, v in err.items()}, diff
+
+
+def counts_match_host(mem, gm):
+    """nparg / nsparg of a par2grid against a host count of the same
+    parcels with the reference's get_index (fields.f90:165-172) and the
+    count folds of parcel_interpl.f90:201-205, 239-243: True when equal."""
+    p = gm.get_parcels()
+    nx, ny, nz = mem.nx, mem.ny, mem.nz
+    lo = np.asarray(mem.lower, dtype=np.float64)
+    dxi = 1.0 / np.asarray(mem.dx, dtype=np.float64)
+    ijk = np.floor((p["position"] - lo) * dxi).astype(np.int64)
+    i, j, k = np.mod(ijk[:, 0], nx), np.mod(ijk[:, 1], ny), ijk[:, 2]
+    if (k < -1).any() or (k > nz).any():
+        return False
+    key = ((k + 1) * ny + j) * nx + i
+    vmin = mem.vcell / 20.0
+    cnt = np.bincount(key, minlength=(nz + 2) * ny * nx).reshape(nz + 2, ny, nx)
+    scnt = np.bincount(key[p["volume"] <= vmin], minlength=(nz + 2) * ny * nx).reshape(nz + 2, ny, nx)
+    ok = True
+    for f, c in (("nparg", cnt), ("nsparg", scnt)):
+        want = np.zeros((nz + 3, ny, nx), dtype=np.int64)   # planes -1..nz+1
+        want[:nz + 2] = c
+        want[1] += c[0]        # plane 0 takes plane -1
+        want[nz] += c[nz + 1]  # plane nz-1 takes plane nz
+        got = gm.get_field(f)[0, 1:nx + 1, 1:ny + 1, :].transpose(2, 1, 0)
+        ok = ok and np.array_equal(got.astype(np.int64), want)
+    return bool(ok)
 
 
 def cpu_baseline(mem, snap):
@@ -210,8 +242,25 @@ def bench_batched(n_members, steps, errors=True, seed0=DEFAULT_SEED):
     log(f"bench: batched ensemble of {n_members} members, {npar} parcels, generated in "
         f"{time.perf_counter() - t0:.1f} s")
     dt = 0.01
+    # tile z chunks from the autotuner, once per grid and parcel count, for
+    # both the plain and the batched members
+    t1 = time.perf_counter()
+    tiles = {}
+    for w in ws:
+        key = (w.nx, w.ny, w.nz, w.n)
+        if key not in tiles:
+            m = Member.from_workload(w)
+            tiles[key] = m.autotune()
+            m.close()
+    log(f"bench: batched: autotuned {len(tiles)} member shapes in {time.perf_counter() - t1:.1f} s")
+
+    def tuned(w):
+        m = Member.from_workload(w)
+        m.set_tiling(*tiles[(w.nx, w.ny, w.nz, w.n)])
+        return m
+
     # plain: every member stepped alone, one after the other
-    plain = [Member.from_workload(w) for w in ws]
+    plain = [tuned(w) for w in ws]
     ts = [0.0] * n_members
     for i, m in enumerate(plain):
         ts[i] = m.ls_rk_step(ts[i], dt)   # warm-up (plan build + graph capture)
@@ -224,7 +273,7 @@ def bench_batched(n_members, steps, errors=True, seed0=DEFAULT_SEED):
         m.close()
     log(f"bench: batched: plain reference {plain_s * 1e3:.1f} ms per ensemble step")
     # batched: the same members in one batch
-    mem = [Member.from_workload(w) for w in ws]
+    mem = [tuned(w) for w in ws]
     b = Batch(mem)
     tb = np.zeros(n_members)
     tb, st = b.step(tb, dt)   # warm-up, step 1
@@ -255,8 +304,10 @@ def bench_batched(n_members, steps, errors=True, seed0=DEFAULT_SEED):
     lat = np.array(lat)
     out = {
         "workload": f"{n_members} rising-thermal members, grids from {{32,48,64,96}}^2 x {{32,64}}, "
-                    f"8 parcels/cell, seeds {seed0 + 1}..{seed0 + n_members}, {steps} steps",
+                    f"8 parcels/cell, seeds {seed0 + 1}..{seed0 + n_members}, {steps} steps, "
+                    f"tile z chunks from the autotuner",
         "members": n_members, "parcels": npar, "steps": steps,
+        "tilings": {f"{k[0]}x{k[1]}x{k[2]}": list(v) for k, v in sorted(tiles.items())},
         "plain_parcel_steps_per_s": npar / plain_s,
         "batched_parcel_steps_per_s": npar / float(lat.mean()),
         "ratio": plain_s / float(lat.mean()),
@@ -372,6 +423,13 @@ def main():
     nprof = max(1, args.steps)
     phases = {k: round(v[0] / nprof, 4) for k, v in tm.items() if v[1]}
     log(f"bench: {args.steps} timed steps, mean {ms:.2f} ms; per step {phases}")
+    # cell indexing and counts on the same positions: a par2grid of the
+    # final parcels against a host recount of them
+    counts_exact = None
+    if want_err:
+        gm.par2grid()
+        counts_exact = counts_match_host(mem, gm)
+        log(f"bench: nparg/nsparg of the final parcels equal the host recount: {counts_exact}")
     gm.close()
 
     # roofline of the tile kernels from their event times over the timed
@@ -436,7 +494,9 @@ def main():
         line["speedup_vs_cpu"] = round(cdt / (ms * 1e-3), 1)
         if errors is not None:
             line["max_rel_error_vs_fp64_step1"] = errors
-            line["counts_bit_exact_step1"] = exact
+            line["nparg_total_abs_diff_vs_fp64_step1"] = exact
+    if counts_exact is not None:
+        line["counts_bit_exact_same_positions"] = counts_exact
         log(f"bench: cpu baseline {cb['value']:.3e} parcel-steps/s; step-1 errors {errors}")
     print(json.dumps(line), flush=True)
 
