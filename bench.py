@@ -172,7 +172,8 @@ def counts_match_host(mem, gm):
         want[:nz + 2] = c
         want[1] += c[0]        # plane 0 takes plane -1
         want[nz] += c[nz + 1]  # plane nz-1 takes plane nz
-        got = gm.get_field(f)[0, 1:nx + 1, 1:ny + 1, :].transpose(2, 1, 0)
+        a = gm.get_field(f)   # (nx+3, ny+3, nz+3), [ix+1, iy+1, iz+1]
+        got = a.reshape(a.shape[-3:])[1:nx + 1, 1:ny + 1, :].transpose(2, 1, 0)
         ok = ok and np.array_equal(got.astype(np.int64), want)
     return bool(ok)
 
