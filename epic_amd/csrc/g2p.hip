@@ -227,37 +227,54 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
             if (r + step < nrow && row_off[r + step] <= q) r += step;
         return r;
     };
-    int r_nx = 0, pi_nx = 0;   // the next round's entry, loaded one round ahead
+    // The next round's entry is loaded one round ahead, and the position and
+    // half axes of the parcel it names are requested in the middle of the
+    // current round (after the gather), so the support points of a round
+    // start without waiting for memory.
+    int r_nx = 0, pi_nx = 0;
+    double pn[3] = {0.0, 0.0, 0.0};
+    float an[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    auto prefetch = [&](int pj) {
+        const unsigned oj = (unsigned)pj * 4u;
+        pn[0] = gl(a.p.x)[pj];
+        pn[1] = gl(a.p.y)[pj];
+        pn[2] = gl(a.p.z)[pj];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) an[c] = at(a.axes[c], oj);
+    };
     if (tid < total) {
         r_nx = row_of(tid);
         pi_nx = gl(a.perm)[row_slot[r_nx] + (tid - row_off[r_nx])];
+        prefetch(pi_nx);
     }
     for (int base = 0; base < total; base += G2P_THREADS) {   // uniform rounds: the ballot needs every lane
         const int q = base + tid;
         const int r = r_nx, pi = pi_nx;
         const unsigned o4 = (unsigned)pi * 4u;   // byte offset (pi < 2^30): scalar base + 32-bit offset addressing
-        if (q + G2P_THREADS < total) {
+        const bool more = q + G2P_THREADS < total;
+        if (more) {
             r_nx = row_of(q + G2P_THREADS);
             pi_nx = gl(a.perm)[row_slot[r_nx] + (q + G2P_THREADS - row_off[r_nx])];
         }
+        double pos[3] = {pn[0], pn[1], pn[2]};
+        const double ve[3] = {an[0], an[1], an[2]};
+        const double vt[3] = {an[3], an[4], an[5]};
         int key = -1;
         bool small = false;
         auto process = [&]() {
             if (q >= total) return;
             const int k = k0 + r / P2G_TY;
-            double pos[3] = {gl(a.p.x)[pi], gl(a.p.y)[pi], gl(a.p.z)[pi]};
             double B[5];
 #pragma unroll
             for (int c = 0; c < 5; ++c) B[c] = (double)at(a.p.B[c], o4);
             const double vol = (double)at(a.p.vol, o4);
-            const double ve[3] = {at(a.axes[0], o4), at(a.axes[1], o4), at(a.axes[2], o4)};
-            const double vt[3] = {at(a.axes[3], o4), at(a.axes[4], o4), at(a.axes[5], o4)};
             int cc[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) cc[d] = (int)floor((pos[d] - g.lower[d]) * g.dxi[d]);
             const int cx = wrap(cc[0], g.nx) - i0, cy = wrap(cc[1], g.ny) - j0;
             if (cx < 0 || cx >= P2G_TX || cy < 0 || cy >= P2G_TY || cc[2] != k) {
                 e |= ERR_CELL_MISMATCH;
+                if (more) prefetch(pi_nx);
                 return;
             }
             const bool acc_prev = a.stage > 1;
@@ -330,6 +347,7 @@ __global__ __launch_bounds__(G2P_THREADS, 4) void k_g2p_tiles(const G2PTileArgs 
 #pragma unroll
                 for (int c = 0; c < 3; ++c) dv[c] += 0.25f * s[11 + c];
             }
+            if (more) prefetch(pi_nx);   // the next round's position and axes, in flight during the rest
             // dB/dt (FP64, B33 from the volume constraint)
             double S[8], dB[5];
 #pragma unroll
