@@ -177,47 +177,73 @@ __global__ void k_bin_keys(Grid g, Pts P, int64_t ntot, int64_t ncell, int *key,
 
 // nearest other parcel; first minimum in the reference's visiting order
 // (cells iz, iy, ix ascending; inside a cell ascending index: the caller
-// index of a parcel, images after every parcel in image order)
-__global__ void k_search(Grid g, Pts P, const int *orig, const int *sml, int ns, int nimg, const int *start,
-                         const int *node, const int *ipid, int *iclo, double *dclo)
+// index of a parcel, images after every parcel in image order).  Eight
+// lanes per small parcel or image, lane cs = 4 dz + 2 dy + dx taking the
+// cell (iz0-1+dz, iy0-1+dy, ix0-1+dx), which numbers the cells in that
+// visiting order: each lane finds its cell's minimum, and the eight combine
+// to the smallest distance, the earlier cell on a tie, as the reference's
+// strict comparison over cells keeps the first.
+constexpr int SEARCH_LANES = 8;
+__global__ __launch_bounds__(256) void k_search(Grid g, Pts P, const int *orig, const int *sml, int ns, int nimg,
+                                                const int *start, const int *node, const int *ipid, int *iclo,
+                                                double *dclo)
 {
-    const int64_t m = gtid();
-    if (m >= (int64_t)ns + nimg) return;
-    const int64_t is = (m < ns) ? sml[m] : P.n + (m - ns);
-    double xs, ys, zs;
-    P.get(is, xs, ys, zs);
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int ix0 = (int)fnint(g.dxi[0] * (xs - (g.lower[0] - g.dx[0])));
-    const int iy0 = (int)fnint(g.dxi[1] * (ys - (g.lower[1] - g.dx[1])));
-    const int iz0 = (int)fnint(g.dxi[2] * (zs - g.lower[2]));
-    double dmin = g.extent[0] * g.extent[0] + g.extent[1] * g.extent[1] + g.extent[2] * g.extent[2];
-    int64_t ic = -1;
-    for (int iz = max(0, iz0 - 1); iz <= min(nz - 1, iz0); ++iz)
-        for (int iy = max(0, iy0 - 1); iy <= min(ny + 1, iy0); ++iy)
-            for (int ix = max(0, ix0 - 1); ix <= min(nx + 1, ix0); ++ix) {
-                const int64_t c = (int64_t)ix + (int64_t)(nx + 3) * iy + (int64_t)(nx + 3) * (ny + 3) * iz;
-                double bd = 0.0;
-                int64_t bj = -1, bk = -1;   // best entry and its reference index
-                for (int k = start[c]; k < start[c + 1]; ++k) {
-                    const int64_t j = node[k];
-                    if (j == is) continue;
-                    double xj, yj, zj;
-                    P.get(j, xj, yj, zj);
-                    const double delz = zj - zs;
-                    const double delx = xj - xs, dely = yj - ys;
-                    const double dsq = delx * delx + dely * dely + delz * delz;
-                    const int64_t kj = (j < P.n) ? (int64_t)orig[j] : j;
-                    if (bj < 0 || dsq < bd || (dsq == bd && kj < bk)) {
-                        bd = dsq;
-                        bj = j;
-                        bk = kj;
-                    }
-                }
-                if (bj >= 0 && bd < dmin) {
-                    dmin = bd;
-                    ic = bj;
+    const int64_t t = gtid();
+    const int64_t m = t / SEARCH_LANES;   // the 8 lanes of one search are neighbours in a wave
+    const int cs = (int)(t % SEARCH_LANES);
+    const bool valid = m < (int64_t)ns + nimg;
+    double bd = HUGE_VAL;
+    int64_t bj = -1;
+    int64_t is = -1;
+    if (valid) {
+        is = (m < ns) ? sml[m] : P.n + (m - ns);
+        double xs, ys, zs;
+        P.get(is, xs, ys, zs);
+        const int nx = g.nx, ny = g.ny, nz = g.nz;
+        const int ix0 = (int)fnint(g.dxi[0] * (xs - (g.lower[0] - g.dx[0])));
+        const int iy0 = (int)fnint(g.dxi[1] * (ys - (g.lower[1] - g.dx[1])));
+        const int iz0 = (int)fnint(g.dxi[2] * (zs - g.lower[2]));
+        const int iz = iz0 - 1 + (cs >> 2), iy = iy0 - 1 + ((cs >> 1) & 1), ix = ix0 - 1 + (cs & 1);
+        if (iz >= 0 && iz <= nz - 1 && iy >= 0 && iy <= ny + 1 && ix >= 0 && ix <= nx + 1) {
+            const int64_t c = (int64_t)ix + (int64_t)(nx + 3) * iy + (int64_t)(nx + 3) * (ny + 3) * iz;
+            int64_t bk = -1;   // the best entry's reference index
+            for (int k = start[c]; k < start[c + 1]; ++k) {
+                const int64_t j = node[k];
+                if (j == is) continue;
+                double xj, yj, zj;
+                P.get(j, xj, yj, zj);
+                const double delz = zj - zs;
+                const double delx = xj - xs, dely = yj - ys;
+                const double dsq = delx * delx + dely * dely + delz * delz;
+                const int64_t kj = (j < P.n) ? (int64_t)orig[j] : j;
+                if (bj < 0 || dsq < bd || (dsq == bd && kj < bk)) {
+                    bd = dsq;
+                    bj = j;
+                    bk = kj;
                 }
             }
+        }
+    }
+    // the eight lanes' minima: the smallest distance, then the earlier cell
+    int bc = cs;
+#pragma unroll
+    for (int o = 1; o < SEARCH_LANES; o <<= 1) {
+        const double od = __shfl_xor(bd, o, 64);
+        const int64_t oj = __shfl_xor(bj, o, 64);
+        const int oc = __shfl_xor(bc, o, 64);
+        if (od < bd || (od == bd && oc < bc)) {
+            bd = od;
+            bj = oj;
+            bc = oc;
+        }
+    }
+    if (!valid || cs != 0) return;
+    double dmin = g.extent[0] * g.extent[0] + g.extent[1] * g.extent[1] + g.extent[2] * g.extent[2];
+    int64_t ic = -1;
+    if (bj >= 0 && bd < dmin) {
+        dmin = bd;
+        ic = bj;
+    }
     // update_remote_indices (parcel_nearest.f90:746-788)
     if (is >= P.n && ic >= P.n) dmin = HUGE_VAL;
     else if (ic >= P.n) ic = ipid[ic - P.n];
@@ -671,8 +697,8 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     double *dclo;
     MOK(alloc((void **)&iclo, sizeof(int) * nsm));
     MOK(alloc((void **)&dclo, sizeof(double) * nsm));
-    hipLaunchKernelGGL(k_search, dim3(nblk(nsm)), dim3(256), 0, s, g, P, p.orig, sml, ns, nimg, start, node, ipid, iclo,
-                       dclo);
+    hipLaunchKernelGGL(k_search, dim3(nblk((int64_t)SEARCH_LANES * nsm)), dim3(256), 0, s, g, P, p.orig, sml, ns, nimg,
+                       start, node, ipid, iclo, dclo);
     hipLaunchKernelGGL(k_reconcile, dim3(nblk(ns)), dim3(256), 0, s, ns, img_of, iclo, dclo, c.err);
     int e = 0;
     MOK(host_int(c.err, e));
