@@ -406,12 +406,31 @@ __global__ void k_merge_stats_big(int nm, const int *pic, ParcelsDev p, double v
     const int64_t k = gtid();
     if (k < nm && (double)p.vol[pic[k]] > vmin) atomicAdd(&cnt[0], 1);
 }
-__global__ void k_merge_stats_way(int64_t n, const int *gcnt, int *cnt)
+// The histogram's bins are few and hot: each wave counts its lanes per bin
+// with ballots over a grid-stride loop, the block sums its waves in LDS, and
+// one atomic per block and bin reaches global memory (an atomic per target
+// on 7 addresses serialises millions of updates).
+constexpr int STATS_WAY_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void k_merge_stats_way(int64_t n, const int *gcnt, int *cnt)
 {
-    const int64_t i = gtid();
-    if (i >= n) return;
-    const int c = gcnt[i];
-    if (c > 0) atomicAdd(&cnt[c < 7 ? c : 7], 1);
+    __shared__ int h[8];
+    if (threadIdx.x < 8) h[threadIdx.x] = 0;
+    __syncthreads();
+    int w[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // wave-uniform counts
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const int c = (i < n) ? gcnt[i] : 0;
+        const int b = c < 7 ? c : 7;
+#pragma unroll
+        for (int j = 1; j < 8; ++j) w[j] += __popcll(__ballot(b == j));
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 1; j < 8; ++j)
+            if (w[j]) atomicAdd(&h[j], w[j]);
+    }
+    __syncthreads();
+    if (threadIdx.x >= 1 && threadIdx.x < 8 && h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
 
 __device__ inline double pdelta(double d, double L) { return d - L * round(d * (1.0 / L)); }
@@ -770,7 +789,8 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     MOK(alloc((void **)&mstat, sizeof(int) * 8));
     MOK(hipMemsetAsync(mstat, 0, sizeof(int) * 8, s));
     hipLaunchKernelGGL(k_merge_stats_big, dim3(nblk(nm)), dim3(256), 0, s, nm, pic, p, c.vmin, mstat);
-    hipLaunchKernelGGL(k_merge_stats_way, dim3(nblk(n)), dim3(256), 0, s, n, gcnt, mstat);
+    hipLaunchKernelGGL(k_merge_stats_way, dim3((unsigned)std::min<int64_t>(nblk(n), STATS_WAY_BLOCKS)), dim3(256), 0,
+                       s, n, gcnt, mstat);
     int mstat_h[8] = {};
     MOK(hipMemcpyAsync(mstat_h, mstat, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_group_merge, dim3(nblk(n)), dim3(256), 0, s, g, p, gstart, gent, pis, c.err);
