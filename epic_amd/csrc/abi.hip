@@ -869,6 +869,7 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     *out = nullptr;
     if (grid->nx < 8 || grid->ny < 8 || grid->nx % 2 || grid->nz < 4) return EPIC_E_INVALID;
     if (!factor235(grid->nx / 2) || !factor235(grid->ny) || !factor235(2 * grid->nz)) return EPIC_E_INVALID;
+    if (grid->nx / 2 > FFT_MAX_N || grid->ny > FFT_MAX_N || 2 * grid->nz > FFT_MAX_N) return EPIC_E_INVALID;
     if (flags->rk_order != 3 && flags->rk_order != 4) return EPIC_E_RK_ORDER;
     for (int i = 0; i < 3; ++i)
         if (!(grid->extent[i] > 0.0)) return EPIC_E_INVALID;
@@ -903,6 +904,18 @@ int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_fla
     for (int i = 0; i < 3; ++i) g.f_cor[i] = phys->f_cor[i];
     m->rk_order = flags->rk_order;
     m->cap = std::max<int64_t>(max_parcels, 1);
+    {   // a transform work-group holds ZB lines (or a column) in LDS
+        int dev = 0, lds_max = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+            epic_destroy(m);
+            return EPIC_E_HIP;
+        }
+        if (transform_lds_bytes(g) > (size_t)lds_max) {
+            epic_destroy(m);
+            return EPIC_E_INVALID;
+        }
+    }
 
     auto bad = [&](hipError_t e) {
         if (e != hipSuccess) {

@@ -90,10 +90,35 @@ struct ParcelsDev {
 // tables leave it there
 __device__ inline int64_t parcel_count(const ParcelsDev &p) { return p.n_dev ? *p.n_dev : p.n; }
 
+// Integer quotients x / d for 0 <= x < 2^16 and 1 <= d <= 2^16 as one high
+// product: with m = ceil(2^32 / d) = (2^32 + e) / d, 0 <= e < d, the
+// product x m / 2^32 exceeds x / d by x e / (d 2^32) < 2^-16 <= 1/d, which
+// never reaches the next integer (x / d is at most q + 1 - 1/d).  d = 1
+// (m would need 33 bits) keeps m = 0 and returns x.
+struct FastDiv {
+    unsigned m = 0;
+    int d = 1;
+};
+inline FastDiv make_fastdiv(int d)
+{
+    FastDiv f;
+    f.d = d;
+    f.m = (d > 1) ? (unsigned)((0x100000000ull + (unsigned long long)d - 1) / (unsigned long long)d) : 0u;
+    return f;
+}
+__device__ inline int fdiv(int x, const FastDiv &f) { return f.d == 1 ? x : (int)__umulhi((unsigned)x, f.m); }
+
+// The index arithmetic of the transforms stays below 2^16 (at most ZB lines
+// of FFT_MAX_N points).
+constexpr int FFT_MAX_N = 2048;
 struct FftPlan {
     int n;          // complex length
     int nst;        // number of stages
     int radix[16];
+    FastDiv nb[16];   // per stage: n / radix
+    FastDiv ns[16];   // per stage: the product of the earlier radices
+    int tstep[16];    // per stage: n / (ns radix), the twiddle stride
+    FastDiv n1, n2;   // n and 2 n
 };
 
 // Global-address-space views of pointers kept in argument tables.  Loads

@@ -79,18 +79,19 @@ template <typename T2> struct TwHalf {   // first half of the table only (n even
 
 // one Stockham stage of radix R over L lines of n points (registers only:
 // the radix is a compile-time constant)
+// (nb and ns: the plan's quotients by n / R and by Ns)
 template <int R, typename T2, typename TW>
-__device__ inline void fft_stage(const T2 *in, T2 *out, int L, int n, int Ns, const TW &tw, bool inv)
+__device__ inline void fft_stage(const T2 *in, T2 *out, int L, int n, const FastDiv &nbd, const FastDiv &nsd,
+                                 int tstep, const TW &tw, bool inv)
 {
-    const int nb = n / R;
-    const int tstep = n / (Ns * R);
+    const int nb = nbd.d, Ns = nsd.d;
     for (int t = threadIdx.x; t < L * nb; t += blockDim.x) {
-        const int l = t / nb, jj = t - l * nb;
+        const int l = fdiv(t, nbd), jj = t - l * nb;
         const T2 *src = in + l * n;
         T2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = src[jj + r * nb];
-        const int kk = jj % Ns;
+        const int q = fdiv(jj, nsd), kk = jj - q * Ns;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
             T2 w = tw(r * kk * tstep);
@@ -98,7 +99,7 @@ __device__ inline void fft_stage(const T2 *in, T2 *out, int L, int n, int Ns, co
             v[r] = cmul(v[r], w);
         }
         butterfly<R>(v, inv);
-        T2 *dst = out + l * n + (jj / Ns) * Ns * R + kk;
+        T2 *dst = out + l * n + q * Ns * R + kk;
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
     }
@@ -111,19 +112,17 @@ template <typename T2, typename TW>
 __device__ T2 *lds_fft_tw(T2 *a, T2 *b, int L, const FftPlan &pl, const TW &tw, bool inv)
 {
     const int n = pl.n;
-    int Ns = 1;
     T2 *in = a, *out = b;
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.radix[st];
-        if (R == 4) fft_stage<4>(in, out, L, n, Ns, tw, inv);
-        else if (R == 2) fft_stage<2>(in, out, L, n, Ns, tw, inv);
-        else if (R == 3) fft_stage<3>(in, out, L, n, Ns, tw, inv);
-        else fft_stage<5>(in, out, L, n, Ns, tw, inv);
+        if (R == 4) fft_stage<4>(in, out, L, n, pl.nb[st], pl.ns[st], pl.tstep[st], tw, inv);
+        else if (R == 2) fft_stage<2>(in, out, L, n, pl.nb[st], pl.ns[st], pl.tstep[st], tw, inv);
+        else if (R == 3) fft_stage<3>(in, out, L, n, pl.nb[st], pl.ns[st], pl.tstep[st], tw, inv);
+        else fft_stage<5>(in, out, L, n, pl.nb[st], pl.ns[st], pl.tstep[st], tw, inv);
         __syncthreads();
         T2 *tmp = in;
         in = out;
         out = tmp;
-        Ns *= R;
     }
     return in;
 }
@@ -144,7 +143,19 @@ inline FftPlan make_plan(int n)
     while (m % 2 == 0) { p.radix[p.nst++] = 2; m /= 2; }
     while (m % 3 == 0) { p.radix[p.nst++] = 3; m /= 3; }
     while (m % 5 == 0) { p.radix[p.nst++] = 5; m /= 5; }
-    if (m != 1) p.nst = -1;   // unsupported length
+    if (m != 1 || n > FFT_MAX_N) {   // unsupported length
+        p.nst = -1;
+        return p;
+    }
+    int ns = 1;
+    for (int st = 0; st < p.nst; ++st) {
+        p.nb[st] = make_fastdiv(n / p.radix[st]);
+        p.ns[st] = make_fastdiv(ns);
+        p.tstep[st] = n / (ns * p.radix[st]);
+        ns *= p.radix[st];
+    }
+    p.n1 = make_fastdiv(n);
+    p.n2 = make_fastdiv(2 * n);
     return p;
 }
 

@@ -71,7 +71,7 @@ __device__ inline void stage_rows(const FftXArgs &a, T2 *buf, int j, int zp0, in
 #pragma unroll
         for (int u = 0; u < LOAD_BATCH; ++u) {
             const int t = min(t0 + u * (int)blockDim.x, total - 1);
-            const int r = t / m, q = t - r * m;
+            const int r = fdiv(t, a.plan.n1), q = t - r * m;   // the x plan's length is m
             const int zp = zp0 + r;
             const int64_t row = (int64_t)j * g.nx + 2 * q;
             if constexpr (HALO) {
@@ -92,7 +92,7 @@ __device__ inline void stage_rows(const FftXArgs &a, T2 *buf, int j, int zp0, in
         for (int u = 0; u < LOAD_BATCH; ++u) {
             const int t = t0 + u * blockDim.x;
             const int zmax = HALO ? nz + 2 : nz;
-            if (t < total) buf[t] = (zp0 + t / m <= zmax) ? z[u] : T2{0, 0};
+            if (t < total) buf[t] = (zp0 + fdiv(t, a.plan.n1) <= zmax) ? z[u] : T2{0, 0};
         }
     }
 }
@@ -159,8 +159,8 @@ template <typename T2> __global__ __launch_bounds__(256) void k_fft_xf(const Fft
     // untangle the packed real transform: X_k, k = 0..m
     for (int t = threadIdx.x; t < ZB * (m + 1); t += blockDim.x) {
         const int kx = t / ZB, r = t - kx * ZB;
-        const T2 zk = Z[r * m + (kx % m)];
-        const T2 zc = cconj(Z[r * m + ((m - kx) % m)]);
+        const T2 zk = Z[r * m + (kx == m ? 0 : kx)];              // kx mod m, kx in [0, m]
+        const T2 zc = cconj(Z[r * m + (kx == 0 ? 0 : m - kx)]);   // (m - kx) mod m
         const T2 w = tw[kx];   // e^{-2 pi i kx / nx}
         const T2 s = cadd(zk, zc), d = csub(zk, zc);
         const T2 wd = cmul(w, d);
@@ -232,7 +232,7 @@ template <typename T2> __global__ __launch_bounds__(256) void k_fft_xi(const Fft
     stage_columns<false>((const T2 *)a.in, (const T2 *)nullptr, bufb, nxh, 1, j * nxh, g, zp0, (R)0);
     __syncthreads();
     for (int t = threadIdx.x; t < ZB * m; t += blockDim.x) {
-        const int r = t / m, k = t - r * m;
+        const int r = fdiv(t, a.plan.n1), k = t - r * m;
         const T2 xk = bufb[r * nxh + k];
         const T2 xc = cconj(bufb[r * nxh + (m - k)]);
         const T2 w = cconj(tw[k]);   // e^{+2 pi i k / nx}
@@ -246,7 +246,7 @@ template <typename T2> __global__ __launch_bounds__(256) void k_fft_xi(const Fft
     const int64_t FS = field_stride(g), PS = plane_stride(g);
     const int nz = g.nz;
     for (int t = threadIdx.x; t < ZB * g.nx; t += blockDim.x) {
-        const int r = t / g.nx, i = t - r * g.nx;
+        const int r = fdiv(t, a.plan.n2), i = t - r * g.nx;   // 2 m = nx
         const int zi = zp0 + r;
         // node plane written: 0..nz, and with HALO the z indices nz + 1 and
         // nz + 2 are node planes -1 and nz + 1
@@ -287,6 +287,8 @@ template <typename T2> __global__ __launch_bounds__(256) void k_fft_xi(const Fft
 // ---------------------------------------------------------------------------
 template <typename T2> static size_t x_lds(const Grid &g) { return (2 * ZB * (size_t)(g.nx / 2 + 1) + g.nx) * sizeof(T2); }
 template <typename T2> static size_t y_lds(const Grid &g) { return (2 * ZB * (size_t)g.ny + g.ny) * sizeof(T2); }
+
+size_t transform_lds_bytes(const Grid &g) { return std::max({x_lds<double2>(g), y_lds<double2>(g), col_lds_bytes(g)}); }
 
 // every item of one launch has the same precision
 template <class A> static bool fp64_items(const std::vector<A> &items) { return !items.empty() && items[0].fp64; }

@@ -179,6 +179,9 @@ struct FixArgs {
     float *fld;
 };
 size_t col_lds_bytes(const Grid &g);
+// the largest LDS allocation of the transform work-groups (FP64 x and y
+// passes, vertical columns)
+size_t transform_lds_bytes(const Grid &g);
 void add_vor2vel_columns(Plan &P, int phase, const std::vector<ColArgs> &items);
 void add_vor2vel_fixup(Plan &P, int phase, const std::vector<FixArgs> &items);
 void add_vtend_fixup(Plan &P, int phase, const std::vector<FixArgs> &items);

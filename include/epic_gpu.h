@@ -88,7 +88,10 @@ enum {
  * (parcel_ellipsoid.f90:84-134), init_inversion (inversion_utils.f90:79-275),
  * ls_rk_setup (ls_rk.f90:64-80).  nx even, nx, ny >= 8; nx/2, ny and nz of
  * the form 2^a 3^b 5^c (the radices of stafft.f90:128-190), nz >= 4;
- * max_parcels <= EPIC_MAX_PARCELS (2^30 - 1). */
+ * max_parcels <= EPIC_MAX_PARCELS (2^30 - 1).  A transform work-group holds
+ * 16 lines of a horizontal FP64 transform, or one vertical column, in LDS:
+ * with 160 KB per work-group (gfx950) nx <= 600, ny <= 310 and nz <= 930.
+ * Otherwise EPIC_E_INVALID. */
 #define EPIC_MAX_PARCELS ((int64_t)0x3fffffff)
 int epic_create(const epic_grid_t *grid, const epic_phys_t *phys, const epic_flags_t *flags,
                 int64_t max_parcels, epic_handle *out);
