@@ -22,6 +22,10 @@ namespace epic {
 
 #define EPIC_PI 3.14159265358979323846
 
+// columns of up to 64 COL_REG_SLOTS values (nz <= 191) keep their arrays in
+// registers
+constexpr int COL_REG_SLOTS = 3;
+
 struct ColCtx {
     int nz;
     double2 *fa, *fb;
@@ -29,75 +33,174 @@ struct ColCtx {
     const double2 *tw;   // e^{-2 pi i t / (2 nz)}, t < nz (the member's table; cached reads)
 };
 
+// One column of nz + 1 values per wave.  With NS > 0 the value at z = lane +
+// 64 i sits in register slot i of the lane (NS slots, nz + 1 <= 64 NS): the
+// column arrays then take no LDS, only the two FFT buffers do, and more
+// columns share a CU.  NS == 0 keeps every column array in LDS (any nz).
+template <int NS, typename T> struct ZCol {
+    T r[NS > 0 ? NS : 1];
+    T *l = nullptr;
+    __device__ T &operator()(int i, int z)
+    {
+        if constexpr (NS > 0) return r[i];
+        else return l[z];
+    }
+};
+
+__device__ inline double read_lane(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// the column's value at z (z uniform over the wave; with LDS storage, after
+// a barrier that follows its last store)
+template <int NS> __device__ __forceinline__ double2 zval(ZCol<NS, double2> &c, int z)
+{
+    if constexpr (NS > 0) {
+        // every slot read from lane z mod 64, then the slot z / 64 chosen
+        // among the scalar results (a select among the register slots
+        // themselves would become an indexed access, which keeps the slots
+        // in scratch memory)
+        const int k = z >> 6, l = z & 63;
+        double2 s = double2{0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const double2 v = double2{read_lane(c.r[i].x, l), read_lane(c.r[i].y, l)};
+            if (k == i) s = v;
+        }
+        return s;
+    } else {
+        return c.l[z];
+    }
+}
+
+// slots of this lane: z = lane + 64 i for i < nsl (z < nz + 1 checked by the caller)
+#define ZSLOTS(i, z) \
+    _Pragma("unroll") for (int i = 0; i < nsl; ++i) if (const int z = (int)threadIdx.x + 64 * i; true)
+
 // DST-I of col[1..nz-1] in place (col[0], col[nz] untouched)
-__device__ void zdst(const ColCtx &c, double2 *col)
+template <int NS> __device__ __forceinline__ void zdst(const ColCtx &c, ZCol<NS, double2> &col, int nsl)
 {
     const int nz = c.nz, n2 = 2 * nz;
-    for (int t = threadIdx.x; t < n2; t += blockDim.x) {
-        double2 v = {0.0, 0.0};
-        if (t > 0 && t < nz) v = col[t];
-        else if (t > nz) v = double2{-col[n2 - t].x, -col[n2 - t].y};
-        c.fa[t] = v;
+    ZSLOTS(i, z)
+    {
+        if (z > 0 && z < nz) {
+            const double2 v = col(i, z);
+            c.fa[z] = v;
+            c.fa[n2 - z] = double2{-v.x, -v.y};   // the odd extension
+        } else if (z == 0 || z == nz) {
+            c.fa[z] = double2{0.0, 0.0};
+        }
     }
     __syncthreads();
     double2 *R = lds_fft_tw(c.fa, c.fb, 1, *c.plan, TwHalf<double2>{c.tw, nz}, false);
     const double s = 0.5 * sqrt(2.0 / (double)nz);
-    for (int t = threadIdx.x + 1; t < nz; t += blockDim.x) col[t] = double2{-s * R[t].y, s * R[t].x};   // s i R
+    ZSLOTS(i, z)
+    {
+        if (z > 0 && z < nz) col(i, z) = double2{-s * R[z].y, s * R[z].x};   // s i R
+    }
     __syncthreads();
 }
 
 // DCT-I of col[0..nz] in place
-__device__ void zdct(const ColCtx &c, double2 *col)
+template <int NS> __device__ __forceinline__ void zdct(const ColCtx &c, ZCol<NS, double2> &col, int nsl)
 {
     const int nz = c.nz, n2 = 2 * nz;
-    for (int t = threadIdx.x; t < n2; t += blockDim.x) c.fa[t] = (t <= nz) ? col[t] : col[n2 - t];
+    ZSLOTS(i, z)
+    {
+        if (z <= nz) {
+            const double2 v = col(i, z);
+            c.fa[z] = v;
+            if (z > 0 && z < nz) c.fa[n2 - z] = v;   // the even extension
+        }
+    }
     __syncthreads();
     double2 *R = lds_fft_tw(c.fa, c.fb, 1, *c.plan, TwHalf<double2>{c.tw, nz}, false);
     const double s = 0.5 * sqrt(2.0 / (double)nz);
-    for (int t = threadIdx.x; t <= nz; t += blockDim.x) col[t] = double2{s * R[t].x, s * R[t].y};
+    ZSLOTS(i, z)
+    {
+        if (z <= nz) col(i, z) = double2{s * R[z].x, s * R[z].y};
+    }
     __syncthreads();
 }
 
 // field_decompose_semi_spectral (inversion_utils.f90:372-401)
-__device__ void decompose(const ColCtx &c, double2 *col, const double *phim, const double *phip)
+template <int NS>
+__device__ __forceinline__ void decompose(const ColCtx &c, ZCol<NS, double2> &col, ZCol<NS, double> &phim, ZCol<NS, double> &phip,
+                          int nsl)
 {
     const int nz = c.nz;
-    const double2 b0 = col[0], bn = col[nz];
+    const double2 b0 = zval(col, 0), bn = zval(col, nz);
     __syncthreads();
-    for (int z = threadIdx.x + 1; z < nz; z += blockDim.x)
-        col[z] = double2{col[z].x - (b0.x * phim[z] + bn.x * phip[z]), col[z].y - (b0.y * phim[z] + bn.y * phip[z])};
+    ZSLOTS(i, z)
+    {
+        if (z > 0 && z < nz) {
+            const double2 v = col(i, z);
+            const double pm = phim(i, z), pp = phip(i, z);
+            col(i, z) = double2{v.x - (b0.x * pm + bn.x * pp), v.y - (b0.y * pm + bn.y * pp)};
+        }
+    }
     __syncthreads();
-    zdst(c, col);
+    zdst(c, col, nsl);
 }
 
 // field_combine_semi_spectral (inversion_utils.f90:426-456)
-__device__ void combine(const ColCtx &c, double2 *col, const double *phim, const double *phip)
+template <int NS>
+__device__ __forceinline__ void combine(const ColCtx &c, ZCol<NS, double2> &col, ZCol<NS, double> &phim, ZCol<NS, double> &phip,
+                        int nsl)
 {
     const int nz = c.nz;
-    zdst(c, col);
-    const double2 b0 = col[0], bn = col[nz];
-    for (int z = threadIdx.x + 1; z < nz; z += blockDim.x)
-        col[z] = double2{col[z].x + b0.x * phim[z] + bn.x * phip[z], col[z].y + b0.y * phim[z] + bn.y * phip[z]};
+    zdst(c, col, nsl);
+    const double2 b0 = zval(col, 0), bn = zval(col, nz);
+    ZSLOTS(i, z)
+    {
+        if (z > 0 && z < nz) {
+            const double2 v = col(i, z);
+            const double pm = phim(i, z), pp = phip(i, z);
+            col(i, z) = double2{v.x + b0.x * pm + bn.x * pp, v.y + b0.y * pm + bn.y * pp};
+        }
+    }
     __syncthreads();
 }
 
 __device__ inline double2 muli(double2 a, double s) { return double2{-s * a.y, s * a.x}; }   // i s a
 
+// (the slot loops of the LDS form, NS == 0, have a run-time trip count: no
+// full unrolling there)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wpass-failed"
+template <int NS>
 __global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restrict__ tab, const int2 *__restrict__ bmap)
 {
     const ItemBlock ib = item_block(bmap);
     const ColArgs &a = tab[ib.item];
     const Grid &g = a.g;
     const int nz = g.nz, nz1 = nz + 1;
+    const int nsl = NS > 0 ? NS : (nz1 + 63) / 64;
     const int col = ib.blk;
     const int kx = col % g.nxh, ky = col / g.nxh;
     const bool zero_mode = (kx == 0 && ky == 0);
 
     extern __shared__ double2 sm[];
-    double2 *SX = sm, *SY = SX + nz1, *C = SY + nz1, *E = C + nz1, *DS = E + nz1, *AS = DS + nz1;
-    double2 *fa = AS + nz1, *fb = fa + 2 * nz;
-    double *phim = (double *)(fb + 2 * nz), *phip = phim + nz1;
-    __shared__ double2 s_ds0, s_dsn, s_y0, s_yn, s_x0, s_xn;
+    ZCol<NS, double2> SX, SY, C, E, DS, AS;
+    ZCol<NS, double> phim, phip;
+    double2 *fa, *fb;
+    if constexpr (NS > 0) {
+        fa = sm;
+        fb = fa + 2 * nz;
+    } else {
+        SX.l = sm;
+        SY.l = SX.l + nz1;
+        C.l = SY.l + nz1;
+        E.l = C.l + nz1;
+        DS.l = E.l + nz1;
+        AS.l = DS.l + nz1;
+        fa = AS.l + nz1;
+        fb = fa + 2 * nz;
+        phim.l = (double *)(fb + 2 * nz);
+        phip.l = phim.l + nz1;
+    }
     ColCtx cc{nz, fa, fb, &a.zplan, a.ztw};   // twiddles from global memory (cached)
 
     // wavenumbers (sta3dfft.f90:88-107, sta2dfft.f90:59-64): r = (pi/L) 2k
@@ -127,198 +230,240 @@ __global__ __launch_bounds__(64) void k_vor2vel_columns(const ColArgs *__restric
 
     // ---- load the three vorticity spectra and the phi tables ----
     const int64_t base = ((int64_t)ky * g.nxh + kx) * g.nzp;
-    for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        const float2 x = gl(a.in[0])[base + z], y = gl(a.in[1])[base + z], w = gl(a.in[2])[base + z];
-        SX[z] = double2{x.x, x.y};
-        SY[z] = double2{y.x, y.y};
-        C[z] = double2{w.x, w.y};
-        double zm, zp;
-        zcoord(z, zm, zp);
-        if (zero_mode) {
-            phim[z] = zm / Lz;
-            phip[z] = zp / Lz;
-        } else {
-            const double ep = exp(-(kl * zp)), em = exp(-(kl * zm));
-            phim[z] = dv * (ep - ef * em);
-            phip[z] = dv * (em - ef * ep);
+    ZSLOTS(i, z)
+    {
+        if (z < nz1) {
+            const float2 x = gl(a.in[0])[base + z], y = gl(a.in[1])[base + z], w = gl(a.in[2])[base + z];
+            SX(i, z) = double2{x.x, x.y};
+            SY(i, z) = double2{y.x, y.y};
+            C(i, z) = double2{w.x, w.y};
+            double zm, zp;
+            zcoord(z, zm, zp);
+            if (zero_mode) {
+                phim(i, z) = zm / Lz;
+                phip(i, z) = zp / Lz;
+            } else {
+                const double ep = exp(-(kl * zp)), em = exp(-(kl * zm));
+                phim(i, z) = dv * (ep - ef * em);
+                phip(i, z) = dv * (em - ef * ep);
+            }
         }
     }
     __syncthreads();
 
     // ---- decompose + filter (inversion.f90:40-43) ----
-    decompose(cc, SX, phim, phip);
-    decompose(cc, SY, phim, phip);
-    decompose(cc, C, phim, phip);
-    for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        double f = fh;
-        if (z > 0 && z < nz) f = fh * exp(-36.0 * pow(kzmaxi * (scz * (double)z), 36.0));
-        if (zero_mode) f = 1.0;
-        SX[z] = double2{f * SX[z].x, f * SX[z].y};
-        SY[z] = double2{f * SY[z].x, f * SY[z].y};
-        C[z] = double2{f * C[z].x, f * C[z].y};
+    decompose(cc, SX, phim, phip, nsl);
+    decompose(cc, SY, phim, phip, nsl);
+    decompose(cc, C, phim, phip, nsl);
+    ZSLOTS(i, z)
+    {
+        if (z < nz1) {
+            double f = fh;
+            if (z > 0 && z < nz) f = fh * exp(-36.0 * pow(kzmaxi * (scz * (double)z), 36.0));
+            if (zero_mode) f = 1.0;
+            const double2 sx = SX(i, z), sy = SY(i, z), cz = C(i, z);
+            SX(i, z) = double2{f * sx.x, f * sx.y};
+            SY(i, z) = double2{f * sy.x, f * sy.y};
+            C(i, z) = double2{f * cz.x, f * cz.y};
+        }
     }
     __syncthreads();
 
     // ---- solenoidal projection (inversion.f90:50-87) ----
-    combine(cc, C, phim, phip);   // zeta, semi-spectral
+    combine(cc, C, phim, phip, nsl);   // zeta, semi-spectral
     {
+        // neighbours through the FFT buffer
+        ZSLOTS(i, z)
+        {
+            if (z < nz1) fa[z] = C(i, z);
+        }
+        __syncthreads();
         const double dzi = g.dxi[2], hdzi = 0.5 * g.dxi[2];
-        for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-            double2 d;
-            if (z == 0) d = double2{dzi * (C[1].x - C[0].x), dzi * (C[1].y - C[0].y)};
-            else if (z == nz) d = double2{dzi * (C[nz].x - C[nz - 1].x), dzi * (C[nz].y - C[nz - 1].y)};
-            else d = double2{(C[z + 1].x - C[z - 1].x) * hdzi, (C[z + 1].y - C[z - 1].y) * hdzi};
-            E[z] = d;
+        ZSLOTS(i, z)
+        {
+            if (z < nz1) {
+                double2 d;
+                if (z == 0) d = double2{dzi * (fa[1].x - fa[0].x), dzi * (fa[1].y - fa[0].y)};
+                else if (z == nz) d = double2{dzi * (fa[nz].x - fa[nz - 1].x), dzi * (fa[nz].y - fa[nz - 1].y)};
+                else d = double2{(fa[z + 1].x - fa[z - 1].x) * hdzi, (fa[z + 1].y - fa[z - 1].y) * hdzi};
+                E(i, z) = d;
+            }
         }
         __syncthreads();
     }
-    decompose(cc, E, phim, phip);
+    decompose(cc, E, phim, phip, nsl);
     if (!zero_mode) {
-        for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-            // D = B_x - A_y;  A = k2l2i (E_x + D_y), B = k2l2i (E_y - D_x)
-            const double2 d = double2{muli(SY[z], ikx).x - muli(SX[z], iky).x, muli(SY[z], ikx).y - muli(SX[z], iky).y};
-            const double2 ex = muli(E[z], ikx), ey = muli(E[z], iky), dx_ = muli(d, ikx), dy_ = muli(d, iky);
-            SX[z] = double2{k2l2i * (ex.x + dy_.x), k2l2i * (ex.y + dy_.y)};
-            SY[z] = double2{k2l2i * (ey.x - dx_.x), k2l2i * (ey.y - dx_.y)};
+        ZSLOTS(i, z)
+        {
+            if (z < nz1) {
+                // D = B_x - A_y;  A = k2l2i (E_x + D_y), B = k2l2i (E_y - D_x)
+                const double2 sx = SX(i, z), sy = SY(i, z), e = E(i, z);
+                const double2 d = double2{muli(sy, ikx).x - muli(sx, iky).x, muli(sy, ikx).y - muli(sx, iky).y};
+                const double2 ex = muli(e, ikx), ey = muli(e, iky), dx_ = muli(d, ikx), dy_ = muli(d, iky);
+                SX(i, z) = double2{k2l2i * (ex.x + dy_.x), k2l2i * (ex.y + dy_.y)};
+                SY(i, z) = double2{k2l2i * (ey.x - dx_.x), k2l2i * (ey.y - dx_.y)};
+            }
         }
     }
     __syncthreads();
 
     // ---- filtered solenoidal vorticity (inversion.f90:91-93) ----
-    for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        DS[z] = SX[z];
-        AS[z] = SY[z];
+    ZSLOTS(i, z)
+    {
+        if (z < nz1) {
+            DS(i, z) = SX(i, z);
+            AS(i, z) = SY(i, z);
+        }
     }
     __syncthreads();
-    combine(cc, DS, phim, phip);
-    combine(cc, AS, phim, phip);
-    for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        gl(a.out[0])[base + z] = float2{(float)DS[z].x, (float)DS[z].y};
-        gl(a.out[1])[base + z] = float2{(float)AS[z].x, (float)AS[z].y};
-        gl(a.out[2])[base + z] = float2{(float)C[z].x, (float)C[z].y};
+    combine(cc, DS, phim, phip, nsl);
+    combine(cc, AS, phim, phip, nsl);
+    ZSLOTS(i, z)
+    {
+        if (z < nz1) {
+            const double2 ds = DS(i, z), as = AS(i, z), cz = C(i, z);
+            gl(a.out[0])[base + z] = float2{(float)ds.x, (float)ds.y};
+            gl(a.out[1])[base + z] = float2{(float)as.x, (float)as.y};
+            gl(a.out[2])[base + z] = float2{(float)cz.x, (float)cz.y};
+        }
     }
     __syncthreads();
 
     // ---- vertical velocity (inversion.f90:97-150) ----
-    for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        const double2 dy_ = muli(SX[z], iky), dx_ = muli(SY[z], ikx);
-        DS[z] = double2{dy_.x - dx_.x, dy_.y - dx_.y};
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        s_ds0 = DS[0];
-        s_dsn = DS[nz];
-    }
-    __syncthreads();
-    const double2 ds0 = s_ds0, dsn = s_dsn;
-    for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        // es = ds(0) dtheta_-(z) + ds(nz) dtheta_+(z)
-        double dtm = 0.0, dtp = 0.0;
-        if (!zero_mode) {
-            double zm, zp;
-            zcoord(z, zm, zp);
-            const double Lm = kl * zm, Lp = kl * zp;
-            const double ep = exp(-Lp), em = exp(-Lm);
-            const double pm = dv * (ep - ef * em), pp = dv * (em - ef * ep);
-            const double dpm = -kl * dv * (ep + ef * em), dpp = kl * dv * (em + ef * ep);
-            (void)pm;
-            (void)pp;
-            dtm = -k2ifac * ((Q * Lp - 1.0) * dpm - Rr * Lm * dpp);
-            dtp = -k2ifac * ((Q * Lm - 1.0) * dpp - Rr * Lp * dpm);
-        }
-        E[z] = double2{ds0.x * dtm + dsn.x * dtp, ds0.y * dtm + dsn.y * dtp};
-        if (z > 0 && z < nz) {
-            const double rz = scz * (double)z;
-            const double green = -1.0 / (k2l2 + rz * rz);
-            DS[z] = double2{green * DS[z].x, green * DS[z].y};
-            AS[z] = double2{rz * DS[z].x, rz * DS[z].y};
-        } else {
-            AS[z] = double2{0.0, 0.0};
+    ZSLOTS(i, z)
+    {
+        if (z < nz1) {
+            const double2 dy_ = muli(SX(i, z), iky), dx_ = muli(SY(i, z), ikx);
+            DS(i, z) = double2{dy_.x - dx_.x, dy_.y - dx_.y};
         }
     }
     __syncthreads();
-    zdct(cc, AS);
-    zdst(cc, DS);
-    for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        if (z == 0 || z == nz) {
-            DS[z] = double2{0.0, 0.0};
-        } else {
-            double tm = 0.0, tp = 0.0;   // bs = ds(0) theta_-(z) + ds(nz) theta_+(z)
+    const double2 ds0 = zval(DS, 0), dsn = zval(DS, nz);
+    __syncthreads();
+    ZSLOTS(i, z)
+    {
+        if (z < nz1) {
+            // es = ds(0) dtheta_-(z) + ds(nz) dtheta_+(z)
+            double dtm = 0.0, dtp = 0.0;
             if (!zero_mode) {
                 double zm, zp;
                 zcoord(z, zm, zp);
                 const double Lm = kl * zm, Lp = kl * zp;
                 const double ep = exp(-Lp), em = exp(-Lm);
-                const double pm = dv * (ep - ef * em), pp = dv * (em - ef * ep);
-                tm = k2ifac * (Rr * Lm * pp - Q * Lp * pm);
-                tp = k2ifac * (Rr * Lp * pm - Q * Lm * pp);
+                const double dpm = -kl * dv * (ep + ef * em), dpp = kl * dv * (em + ef * ep);
+                dtm = -k2ifac * ((Q * Lp - 1.0) * dpm - Rr * Lm * dpp);
+                dtp = -k2ifac * ((Q * Lm - 1.0) * dpp - Rr * Lp * dpm);
             }
-            DS[z] = double2{DS[z].x + (ds0.x * tm + dsn.x * tp), DS[z].y + (ds0.y * tm + dsn.y * tp)};
+            E(i, z) = double2{ds0.x * dtm + dsn.x * dtp, ds0.y * dtm + dsn.y * dtp};
+            if (z > 0 && z < nz) {
+                const double rz = scz * (double)z;
+                const double green = -1.0 / (k2l2 + rz * rz);
+                const double2 ds = DS(i, z);
+                const double2 gd = double2{green * ds.x, green * ds.y};
+                DS(i, z) = gd;
+                AS(i, z) = double2{rz * gd.x, rz * gd.y};
+            } else {
+                AS(i, z) = double2{0.0, 0.0};
+            }
         }
-        E[z] = double2{E[z].x + AS[z].x, E[z].y + AS[z].y};   // dw/dz
+    }
+    __syncthreads();
+    zdct(cc, AS, nsl);
+    zdst(cc, DS, nsl);
+    ZSLOTS(i, z)
+    {
+        if (z < nz1) {
+            if (z == 0 || z == nz) {
+                DS(i, z) = double2{0.0, 0.0};
+            } else {
+                double tm = 0.0, tp = 0.0;   // bs = ds(0) theta_-(z) + ds(nz) theta_+(z)
+                if (!zero_mode) {
+                    double zm, zp;
+                    zcoord(z, zm, zp);
+                    const double Lm = kl * zm, Lp = kl * zp;
+                    const double ep = exp(-Lp), em = exp(-Lm);
+                    const double pm = dv * (ep - ef * em), pp = dv * (em - ef * ep);
+                    tm = k2ifac * (Rr * Lm * pp - Q * Lp * pm);
+                    tp = k2ifac * (Rr * Lp * pm - Q * Lm * pp);
+                }
+                const double2 ds = DS(i, z);
+                DS(i, z) = double2{ds.x + (ds0.x * tm + dsn.x * tp), ds.y + (ds0.y * tm + dsn.y * tp)};
+            }
+            const double2 e = E(i, z), as = AS(i, z);
+            E(i, z) = double2{e.x + as.x, e.y + as.y};   // dw/dz
+        }
     }
     __syncthreads();
 
     // ---- horizontally averaged flow (inversion.f90:161-176) ----
     if (zero_mode) {
-        if (threadIdx.x == 0) {
-            s_y0 = SY[0];
-            s_yn = SY[nz];
-            s_x0 = SX[0];
-            s_xn = SX[nz];
-        }
+        const double2 y0 = zval(SY, 0), yn = zval(SY, nz), x0 = zval(SX, 0), xn = zval(SX, nz);
         __syncthreads();
-        for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-            if (z == 0 || z == nz) {
-                AS[z] = double2{0.0, 0.0};
-                SX[z] = double2{0.0, 0.0};
-            } else {
-                const double rzi = 1.0 / (scz * (double)z);
-                AS[z] = double2{-rzi * SY[z].x, -rzi * SY[z].y};   // ubar
-                SX[z] = double2{rzi * SX[z].x, rzi * SX[z].y};     // vbar
+        ZSLOTS(i, z)
+        {
+            if (z < nz1) {
+                if (z == 0 || z == nz) {
+                    AS(i, z) = double2{0.0, 0.0};
+                    SX(i, z) = double2{0.0, 0.0};
+                } else {
+                    const double rzi = 1.0 / (scz * (double)z);
+                    const double2 sy = SY(i, z), sx = SX(i, z);
+                    AS(i, z) = double2{-rzi * sy.x, -rzi * sy.y};   // ubar
+                    SX(i, z) = double2{rzi * sx.x, rzi * sx.y};     // vbar
+                }
             }
         }
         __syncthreads();
-        zdct(cc, AS);
-        zdct(cc, SX);
-        for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-            double zm, zp;
-            zcoord(z, zm, zp);
-            const double pt = zp / Lz;
-            double zm2, zp2;
-            zcoord(nz - z, zm2, zp2);
-            const double pb = zp2 / Lz;
-            const double gt = 0.5 * Lz * (pt * pt - 1.0 / 3.0);
-            const double gb = 0.5 * Lz * (pb * pb - 1.0 / 3.0);
-            AS[z] = double2{AS[z].x + s_yn.x * gt - s_y0.x * gb, AS[z].y + s_yn.y * gt - s_y0.y * gb};
-            SX[z] = double2{SX[z].x - s_xn.x * gt + s_x0.x * gb, SX[z].y - s_xn.y * gt + s_x0.y * gb};
+        zdct(cc, AS, nsl);
+        zdct(cc, SX, nsl);
+        ZSLOTS(i, z)
+        {
+            if (z < nz1) {
+                double zm, zp;
+                zcoord(z, zm, zp);
+                const double pt = zp / Lz;
+                double zm2, zp2;
+                zcoord(nz - z, zm2, zp2);
+                const double pb = zp2 / Lz;
+                const double gt = 0.5 * Lz * (pt * pt - 1.0 / 3.0);
+                const double gb = 0.5 * Lz * (pb * pb - 1.0 / 3.0);
+                const double2 as = AS(i, z), sx = SX(i, z);
+                AS(i, z) = double2{as.x + yn.x * gt - y0.x * gb, as.y + yn.y * gt - y0.y * gb};
+                SX(i, z) = double2{sx.x - xn.x * gt + x0.x * gb, sx.y - xn.y * gt + x0.y * gb};
+            }
         }
         __syncthreads();
     }
 
     // ---- u, v, w and the velocity-gradient spectra (inversion.f90:180-233, 259-317) ----
-    for (int z = threadIdx.x; z < nz1; z += blockDim.x) {
-        double2 u, v;
-        if (zero_mode) {
-            u = AS[z];
-            v = SX[z];
-        } else {
-            const double2 ex = muli(E[z], ikx), ey = muli(E[z], iky), cx = muli(C[z], ikx), cy = muli(C[z], iky);
-            u = double2{k2l2i * (ex.x + cy.x), k2l2i * (ex.y + cy.y)};
-            v = double2{k2l2i * (ey.x - cx.x), k2l2i * (ey.y - cx.y)};
+    ZSLOTS(i, z)
+    {
+        if (z < nz1) {
+            double2 u, v;
+            if (zero_mode) {
+                u = AS(i, z);
+                v = SX(i, z);
+            } else {
+                const double2 e = E(i, z), cz = C(i, z);
+                const double2 ex = muli(e, ikx), ey = muli(e, iky), cx = muli(cz, ikx), cy = muli(cz, iky);
+                u = double2{k2l2i * (ex.x + cy.x), k2l2i * (ex.y + cy.y)};
+                v = double2{k2l2i * (ey.x - cx.x), k2l2i * (ey.y - cx.y)};
+            }
+            const double2 w = DS(i, z);
+            auto st = [&](int o, double2 val) { gl(a.out[o])[base + z] = float2{(float)val.x, (float)val.y}; };
+            st(3, u);
+            st(4, v);
+            st(5, w);
+            st(6, muli(u, ikx));    // du/dx
+            st(7, muli(u, iky));    // du/dy
+            st(8, muli(v, iky));    // dv/dy
+            st(9, muli(w, ikx));    // dw/dx
+            st(10, muli(w, iky));   // dw/dy
         }
-        const double2 w = DS[z];
-        auto st = [&](int o, double2 val) { gl(a.out[o])[base + z] = float2{(float)val.x, (float)val.y}; };
-        st(3, u);
-        st(4, v);
-        st(5, w);
-        st(6, muli(u, ikx));    // du/dx
-        st(7, muli(u, iky));    // du/dy
-        st(8, muli(v, iky));    // dv/dy
-        st(9, muli(w, ikx));    // dw/dx
-        st(10, muli(w, iky));   // dw/dy
     }
 }
+#pragma clang diagnostic pop
+#undef ZSLOTS
 
 // ---------------------------------------------------------------------------
 // physical-space completion
@@ -388,21 +533,40 @@ __global__ __launch_bounds__(256) void k_vtend_fixup(const FixArgs *__restrict__
     }
 }
 
+// register slots per lane for a column of nz + 1 values (0: the LDS form)
+static int col_slots(const Grid &g)
+{
+    const int ns = (g.nz + 1 + 63) / 64;
+    return ns <= COL_REG_SLOTS ? ns : 0;
+}
+
 size_t col_lds_bytes(const Grid &g)
 {
     const int nz1 = g.nz + 1;
-    return (size_t)(6 * nz1 + 4 * g.nz) * sizeof(double2) + 2 * (size_t)nz1 * sizeof(double);
+    const size_t fft = (size_t)4 * g.nz * sizeof(double2);   // the two FFT buffers of 2 nz points
+    if (col_slots(g) > 0) return fft;
+    return (size_t)6 * nz1 * sizeof(double2) + fft + 2 * (size_t)nz1 * sizeof(double);
 }
 
+// one launch per column form present (members of one batch may differ in nz)
 void add_vor2vel_columns(Plan &P, int phase, const std::vector<ColArgs> &items)
 {
-    std::vector<int> nb;
-    size_t lds = 0;
-    for (const ColArgs &a : items) {
-        nb.push_back(a.g.ny * a.g.nxh);
-        lds = std::max(lds, col_lds_bytes(a.g));
+    static const void *const kern[COL_REG_SLOTS + 1] = {(const void *)k_vor2vel_columns<0>,
+                                                        (const void *)k_vor2vel_columns<1>,
+                                                        (const void *)k_vor2vel_columns<2>,
+                                                        (const void *)k_vor2vel_columns<3>};
+    for (int ns = 0; ns <= COL_REG_SLOTS; ++ns) {
+        std::vector<ColArgs> sel;
+        std::vector<int> nb;
+        size_t lds = 0;
+        for (const ColArgs &a : items) {
+            if (col_slots(a.g) != ns) continue;
+            sel.push_back(a);
+            nb.push_back(a.g.ny * a.g.nxh);
+            lds = std::max(lds, col_lds_bytes(a.g));
+        }
+        if (!sel.empty()) P.add(kern[ns], 64, lds, phase, sel, nb);
     }
-    P.add((const void *)k_vor2vel_columns, 64, lds, phase, items, nb);
 }
 
 static std::vector<int> column_blocks(const std::vector<FixArgs> &items)
