@@ -44,6 +44,7 @@ import os
 import re
 import sys
 import time
+import traceback
 
 import numpy as np
 
@@ -483,12 +484,22 @@ def main():
         "roofline": roof,
         "roofline_kernels": kern,
     }
+    # a failure of the batched or served form is reported in its entry (and
+    # on stderr with its traceback); the plain form's line is still printed
+    def form(name, fn):
+        try:
+            line[name] = fn()
+        except Exception as err:   # noqa: BLE001
+            traceback.print_exc()
+            log(f"bench: {name} form failed: {err}")
+            line[name] = {"error": f"{type(err).__name__}: {err}"}
+
     if "batched" in forms and args.members > 0:
-        line["batched"] = bench_batched(args.members, max(1, args.ens_steps), errors=not args.no_member_errors)
+        form("batched", lambda: bench_batched(args.members, max(1, args.ens_steps), errors=not args.no_member_errors))
     if "served" in forms and args.requests > 0:
         rates = [float(r) for r in args.rates.split(",") if r]
         lo, hi = (int(v) for v in args.req_steps.split(","))
-        line["served"] = bench_served(args.requests, rates, (lo, hi))
+        form("served", lambda: bench_served(args.requests, rates, (lo, hi)))
     if not args.no_cpu:
         cb, cdt, errors, exact = cpu_baseline(mem, snap)
         line["cpu_baseline"] = cb

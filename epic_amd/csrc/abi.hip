@@ -1780,7 +1780,12 @@ int epic_batch_wait(epic_batch_handle b, double *t, int *status)
     if (!b->in_flight) return batch_fail(b, EPIC_E_INVALID, "epic_batch_wait: no step in flight");
     b->in_flight = false;
     BHIPOK(b, hipStreamSynchronize(b->s));
-    BHIPOK(b, hipGetLastError());
+    // The step's launch errors came back from hipGraphLaunch and its
+    // execution errors from the synchronisation.  The thread's last-error
+    // slot may still hold an error of an unrelated earlier call (a cleanup
+    // call whose status is ignored, or one the runtime made while another
+    // batch was captured): it is cleared here, not taken as this step's.
+    (void)hipGetLastError();
     const int nm = (int)b->ms.size();
     int first = EPIC_OK;
     for (int i = 0; i < nm; ++i) {
@@ -1810,6 +1815,7 @@ int epic_batch_step(epic_batch_handle b, double *t, const double *dt, const doub
 int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_split)
 {
     if (!m) return EPIC_E_INVALID;
+    (void)hipGetLastError();   // only this call's launch errors are read below
     if (n_split) *n_split = 0;
     const Grid &g = m->g;
     if (!(lambda_max > 0.0)) lambda_max = 4.0;   // options.f90:89
@@ -1872,6 +1878,7 @@ int epic_parcel_split(epic_handle m, double lambda_max, double amax, int64_t *n_
 int epic_parcel_merge(epic_handle m, int64_t *n_merge)
 {
     if (!m) return EPIC_E_INVALID;
+    (void)hipGetLastError();   // only this call's launch errors are read below
     if (n_merge) *n_merge = 0;
     {   // the slot compaction moves the tendencies with their parcels
         const int rc = settle_tendencies(m);

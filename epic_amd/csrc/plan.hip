@@ -11,7 +11,13 @@ namespace epic {
 // device (every stream), which would stall the members' steps in flight
 // each time a transient plan (a parcel operation's launches) is dropped.
 // A block returns with an event recorded behind the last work that used it;
-// the next user's stream waits for that event before its upload.
+// the next user's stream waits for that event before its upload.  The
+// runtime has refused to synchronise on such an event ("operation not
+// permitted on an event last recorded in a capturing stream") although its
+// stream was not capturing when it was recorded (streams that held a
+// capture earlier); a refused wait falls back to a device synchronisation,
+// and the refusal is cleared from the thread's last-error slot, where it
+// would otherwise be read as the failure of a later, unrelated launch.
 namespace {
 struct PoolBlock {
     char *p;
@@ -43,7 +49,11 @@ hipError_t pool_get(size_t bytes, hipStream_t s, char **out, size_t *got)
             pool.erase(pool.begin() + (long)best);
             if (b.ev) {
                 e = hipStreamWaitEvent(s, b.ev, 0);
-                (void)hipEventDestroy(b.ev);
+                if (e != hipSuccess) {   // a refused wait: every stream drained instead
+                    (void)hipGetLastError();
+                    e = hipDeviceSynchronize();
+                }
+                if (hipEventDestroy(b.ev) != hipSuccess) (void)hipGetLastError();
                 if (e != hipSuccess) {
                     (void)hipFree(b.p);
                     return e;
@@ -85,8 +95,10 @@ void pool_put(char *p, size_t size, hipStream_t last)
         PoolBlock b = pool.front();
         pool.erase(pool.begin());
         if (b.ev) {
-            (void)hipEventSynchronize(b.ev);
-            (void)hipEventDestroy(b.ev);
+            if (hipEventSynchronize(b.ev) != hipSuccess) {   // refused: hipFree below drains the device
+                (void)hipGetLastError();
+            }
+            if (hipEventDestroy(b.ev) != hipSuccess) (void)hipGetLastError();
         }
         (void)hipFree(b.p);
     }
