@@ -186,9 +186,11 @@ def cpu_baseline(mem, snap):
 
     nthreads = host_threads()
     st = oracle_state(mem, nthreads=nthreads)
+    log(f"bench: cpu baseline: oracle state of {mem.n} parcels, {nthreads} threads")
     t0 = time.perf_counter()
     st.ls_rk_step(0.0, 0.01)
     t_warm = time.perf_counter() - t0
+    log(f"bench: cpu baseline: step 1 (warm-up) in {t_warm:.1f} s")
     errors, exact = error_table(mem, snap, st) if snap is not None else (None, None)
     t0 = time.perf_counter()
     st.ls_rk_step(0.01, 0.01)
