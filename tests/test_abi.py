@@ -33,7 +33,10 @@ def test_create_rejects_bad_grid_without_touching_the_gpu():
     h = C.c_void_p()
     # lengths with a factor 7 (stafft.f90 factorises 2, 3, 4, 5, 6), an odd
     # nx (the x transform packs nx/2 complex points), and fewer than 8 cells
-    for nx, ny, nz in ((14, 16, 16), (16, 28, 16), (16, 16, 14), (15, 16, 16), (6, 16, 16), (16, 16, 3)):
+    # and transform lengths beyond 2048 points (ny = 2560, 2 nz = 2560: the
+    # stages' index arithmetic stays below 2^16)
+    for nx, ny, nz in ((14, 16, 16), (16, 28, 16), (16, 16, 14), (15, 16, 16), (6, 16, 16), (16, 16, 3),
+                       (16, 2560, 16), (16, 16, 1280)):
         g = core.Grid(nx, ny, nz, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))
         assert lib.epic_create(C.byref(g), C.byref(p), C.byref(f), 10, C.byref(h)) == -1, (nx, ny, nz)
     g = core.Grid(16, 16, 16, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1))

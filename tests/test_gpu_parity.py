@@ -354,3 +354,14 @@ def test_vor2vel_mean_flow_and_filter():
         e = maxrel(interior(gm.get_field(f), nx, ny, 1, nz + 2), interior(getattr(st, f), nx, ny, 1, nz + 2))
         print(f"  mean flow + filter {f}: {e:.2e}", flush=True)
         assert e <= TOL, (f, e)
+
+
+def test_create_rejects_grid_beyond_transform_lds():
+    """16 lines of a 320-point FP64 y transform need 169 KB of LDS, more than
+    a work-group may hold: epic_create refuses the grid (EPIC_E_INVALID)
+    instead of a later launch failing."""
+    with pytest.raises(EpicError) as ei:
+        Member(16, 320, 16, (0.0, 0.0, 0.0), (1.0, 20.0, 1.0))
+    assert ei.value.code == -1
+    m = Member(16, 300, 16, (0.0, 0.0, 0.0), (1.0, 20.0, 1.0))   # 155 KB: accepted
+    m.close()
