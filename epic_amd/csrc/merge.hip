@@ -206,21 +206,38 @@ __global__ __launch_bounds__(256) void k_search(Grid g, Pts P, const int *orig, 
         const int iz = iz0 - 1 + (cs >> 2), iy = iy0 - 1 + ((cs >> 1) & 1), ix = ix0 - 1 + (cs & 1);
         if (iz >= 0 && iz <= nz - 1 && iy >= 0 && iy <= ny + 1 && ix >= 0 && ix <= nx + 1) {
             const int64_t c = (int64_t)ix + (int64_t)(nx + 3) * iy + (int64_t)(nx + 3) * (ny + 3) * iz;
-            int64_t bk = -1;   // the best entry's reference index
-            for (int k = start[c]; k < start[c + 1]; ++k) {
-                const int64_t j = node[k];
-                if (j == is) continue;
-                double xj, yj, zj;
-                P.get(j, xj, yj, zj);
+            // the cell's minimum under (distance, reference index), which does
+            // not depend on the order the entries are taken in: four entries'
+            // loads are issued together, and the reference index (a further
+            // dependent load) is read only on a tie of distances
+            auto ref = [&](int64_t j) { return (j < P.n) ? (int64_t)orig[j] : j; };
+            auto take = [&](int64_t j, double xj, double yj, double zj) {
+                if (j == is) return;
                 const double delz = zj - zs;
                 const double delx = xj - xs, dely = yj - ys;
                 const double dsq = delx * delx + dely * dely + delz * delz;
-                const int64_t kj = (j < P.n) ? (int64_t)orig[j] : j;
-                if (bj < 0 || dsq < bd || (dsq == bd && kj < bk)) {
+                if (bj < 0 || dsq < bd || (dsq == bd && ref(j) < ref(bj))) {
                     bd = dsq;
                     bj = j;
-                    bk = kj;
                 }
+            };
+            int k = start[c];
+            const int kend = start[c + 1];
+            for (; k + 4 <= kend; k += 4) {
+                int64_t j4[4];
+                double x4[4], y4[4], z4[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) j4[u] = node[k + u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) P.get(j4[u], x4[u], y4[u], z4[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) take(j4[u], x4[u], y4[u], z4[u]);
+            }
+            for (; k < kend; ++k) {
+                const int64_t j = node[k];
+                double xj, yj, zj;
+                P.get(j, xj, yj, zj);
+                take(j, xj, yj, zj);
             }
         }
     }
@@ -728,7 +745,7 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     MOK(alloc((void **)&leaf, n));
     MOK(alloc((void **)&avail, n));
     MOK(alloc((void **)&merged, n));
-    MOK(alloc((void **)&cont, sizeof(int)));
+    MOK(alloc((void **)&cont, 2 * sizeof(int)));
     MOK(alloc((void **)&dom, sizeof(int) * ns));
     MOK(alloc((void **)&cand, sizeof(int) * ns));
     MOK(alloc((void **)&iso, sizeof(int) * ns));
@@ -736,16 +753,23 @@ int run_merge(const MergeCtx &c, int64_t *n_merge)
     MOK(hipMemsetAsync(avail, 0, n, s));
     MOK(hipMemsetAsync(merged, 0, n, s));
     MOK(hipMemsetAsync(iso, 0, sizeof(int) * ns, s));
-    for (int it = 0;; ++it) {
-        MOK(hipMemsetAsync(cont, 0, sizeof(int), s));
-        hipLaunchKernelGGL(k_tree1, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf, avail);
-        hipLaunchKernelGGL(k_tree2, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf);
-        hipLaunchKernelGGL(k_tree3, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf, avail);
-        hipLaunchKernelGGL(k_tree4, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf, avail, cont);
+    // the leaf iterations, two per read-back of the "merged any" flag: an
+    // iteration that merges nothing leaves merged, leaf and avail as they
+    // were (each kernel sets or clears the same bits again), so one past
+    // convergence changes nothing, and the second flag of a pair says
+    // whether the pair ended converged
+    for (int it = 0;; it += 2) {
+        MOK(hipMemsetAsync(cont, 0, 2 * sizeof(int), s));
+        for (int h = 0; h < 2; ++h) {
+            hipLaunchKernelGGL(k_tree1, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf, avail);
+            hipLaunchKernelGGL(k_tree2, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf);
+            hipLaunchKernelGGL(k_tree3, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf, avail);
+            hipLaunchKernelGGL(k_tree4, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf, avail, cont + h);
+        }
         int more = 0;
-        MOK(host_int(cont, more));
+        MOK(host_int(cont + 1, more));
         if (!more) break;
-        if (it > ns) return done(-1);   // every iteration merges at least one parcel
+        if (it > ns) return done(-1);   // every iteration that continues merges at least one parcel
     }
     hipLaunchKernelGGL(k_tree5, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, iclo, merged, leaf, avail);
     hipLaunchKernelGGL(k_tree6, dim3(nblk(ns)), dim3(256), 0, s, sml, ns, merged, leaf, avail, dom, cand);
