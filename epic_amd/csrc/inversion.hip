@@ -4,7 +4,8 @@
 // After the horizontal FFTs every horizontal operator of vor2vel is diagonal
 // in (kx, ky), so the whole vertical part of the inversion is independent per
 // spectral column.  k_vor2vel_columns gives each column to one wavefront and
-// runs it in FP64 in LDS: decomposition into boundary values + sine series,
+// runs it in FP64, the column in registers (nz <= 191) or in LDS:
+// decomposition into boundary values + sine series,
 // Hou-Li filter, solenoidal projection, vertical Green's function, the
 // hyperbolic boundary corrections theta/phi, the zero-mode mean flow and the
 // velocity and velocity-gradient spectra.  The z transforms (DST-I / DCT-I
