@@ -151,10 +151,12 @@ struct epic_member {
     void counts_changed()
     {
         const uint64_t keep = gen;
-        Plan k;
+        Plan k, a;
         k.swap(pl_keys);
+        a.swap(pl_axes);
         invalidate_plans();
         pl_keys.swap(k);
+        pl_axes.swap(a);
         gen = keep;
     }
 };
