@@ -923,6 +923,8 @@ __global__ __launch_bounds__(64) void k_far_reset(const FarClearArgs *__restrict
     if (threadIdx.x == 0) gl(tab[ib.item].far_flag)[0] = 0;
 }
 
+constexpr size_t P2G_LDS_FLOOR = 160 * 1024 / 4 + 256;   // admits three work-groups per CU, not four
+
 void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::vector<FinalizeArgs> &fin,
              const std::vector<FarClearArgs> &far)
 {
@@ -941,7 +943,12 @@ void add_p2g(Plan &P, int phase, const std::vector<P2GArgs> &tiles, const std::v
             nb.push_back(a.ntx * a.nty * a.nchz);
         }
         const void *k = tz == 2 ? (const void *)k_p2g_tiles<2> : tz == 4 ? (const void *)k_p2g_tiles<4> : (const void *)k_p2g_tiles<8>;
-        P.add(k, 64 * tz * P2G_TPC, p2g_lds_bytes(tz), phase == PH_P2G ? (int)PH_P2G_TILES : phase, items, nb);
+        // at least a third of the CU's LDS a work-group: with four work-groups
+        // of tz = 2 on a CU, the rows of parcels their waves touch leave the
+        // L2 between a wave's rounds (the tile scatter 1.6 % slower than with
+        // three); larger tz need more than that anyway
+        const size_t lds = std::max<size_t>(p2g_lds_bytes(tz), P2G_LDS_FLOOR);
+        P.add(k, 64 * tz * P2G_TPC, lds, phase == PH_P2G ? (int)PH_P2G_TILES : phase, items, nb);
     }
     std::vector<int> nf;
     for (const FinalizeArgs &f : fin) nf.push_back((int)(((int64_t)f.g.nx * f.g.ny * (f.g.nz + 3) + 255) / 256));
